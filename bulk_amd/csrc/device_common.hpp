@@ -1,0 +1,390 @@
+// Device-side building blocks shared by every kernel of the library:
+// operator functors, wavefront (64-lane) DPP scans and reductions, the
+// decoupled look-back tile-status protocol, and vectorised tile loads.
+//
+// Target: gfx950 (CDNA4), wave64.  Nothing here is portable on purpose.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <limits>
+#include <type_traits>
+
+namespace bulkx {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------------------
+// Operators.  Integer addition wraps (computed in the unsigned type), which
+// makes it associative and lets any evaluation order be bit-exact.
+// min / max follow thrust::minimum / thrust::maximum: lhs < rhs ? lhs : rhs.
+// ---------------------------------------------------------------------------
+struct OpPlus {
+  template <class T>
+  __host__ __device__ __forceinline__ static T apply(T a, T b) {
+    if constexpr (std::is_integral_v<T>) {
+      using U = std::make_unsigned_t<T>;
+      return T(U(a) + U(b));
+    } else {
+      return a + b;
+    }
+  }
+  // -0.0 is the exact additive identity for IEEE floats (x + -0.0 == x for every x).
+  template <class T>
+  __host__ __device__ __forceinline__ static T identity() {
+    if constexpr (std::is_floating_point_v<T>) return T(-0.0);
+    else return T(0);
+  }
+};
+
+struct OpMin {
+  template <class T>
+  __host__ __device__ __forceinline__ static T apply(T a, T b) { return a < b ? a : b; }
+  template <class T>
+  __host__ __device__ __forceinline__ static T identity() {
+    if constexpr (std::is_floating_point_v<T>) return std::numeric_limits<T>::infinity();
+    else return std::numeric_limits<T>::max();
+  }
+};
+
+struct OpMax {
+  template <class T>
+  __host__ __device__ __forceinline__ static T apply(T a, T b) { return a < b ? b : a; }
+  template <class T>
+  __host__ __device__ __forceinline__ static T identity() {
+    if constexpr (std::is_floating_point_v<T>) return -std::numeric_limits<T>::infinity();
+    else return std::numeric_limits<T>::lowest();
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Double-double accumulator: value = hi + lo with |lo| <= ulp(hi)/2.  Carries
+// float64 sums across threads, tiles and the look-back chain so that the
+// error of a float64 scan or reduction stays within a few ulps of sum|x|
+// however long the chain of tiles is.  Addition is Knuth's TwoSum followed by
+// renormalisation (Dekker, 1971; Hida, Li and Bailey's "sloppy" add).
+// Non-finite sums short-circuit so inf stays inf and NaN stays NaN.
+// ---------------------------------------------------------------------------
+struct dd {
+  double hi, lo;
+};
+
+__host__ __device__ __forceinline__ dd dd_add(dd a, dd b) {
+#pragma clang fp contract(off)
+  double s = a.hi + b.hi;
+  if (!__builtin_isfinite(s)) return dd{s, 0.0};
+  double bb = s - a.hi;
+  double e = (a.hi - (s - bb)) + (b.hi - bb);
+  e += a.lo + b.lo;
+  double hi = s + e;
+  double lo = e - (hi - s);
+  return dd{hi, lo};
+}
+
+struct OpPlusDD {
+  __host__ __device__ __forceinline__ static dd apply(dd a, dd b) { return dd_add(a, b); }
+  template <class T>
+  __host__ __device__ __forceinline__ static dd identity() { return dd{-0.0, 0.0}; }
+};
+
+// Accumulator policy: which type and operator carry partial results between
+// threads and tiles.  float sums accumulate in double, double sums in
+// double-double; everything else (integers, min, max) stays in T and is exact.
+template <class T, class Op>
+struct Acc {
+  using type = T;
+  using op = Op;
+  __host__ __device__ __forceinline__ static type from(T x) { return x; }
+  __host__ __device__ __forceinline__ static T to(type a) { return a; }
+};
+template <>
+struct Acc<float, OpPlus> {
+  using type = double;
+  using op = OpPlus;
+  __host__ __device__ __forceinline__ static double from(float x) { return double(x); }
+  __host__ __device__ __forceinline__ static float to(double a) { return float(a); }
+};
+template <>
+struct Acc<double, OpPlus> {
+  using type = dd;
+  using op = OpPlusDD;
+  __host__ __device__ __forceinline__ static dd from(double x) { return dd{x, 0.0}; }
+  __host__ __device__ __forceinline__ static double to(dd a) {
+    return __builtin_isfinite(a.hi) ? a.hi + a.lo : a.hi;
+  }
+};
+
+struct CmpLess {
+  template <class T>
+  __host__ __device__ __forceinline__ static bool apply(T a, T b) { return a < b; }
+};
+struct CmpGreater {
+  template <class T>
+  __host__ __device__ __forceinline__ static bool apply(T a, T b) { return b < a; }
+};
+
+// ---------------------------------------------------------------------------
+// Raw bit access.
+// ---------------------------------------------------------------------------
+template <class To, class From>
+__device__ __forceinline__ To bitcast(From x) {
+  static_assert(sizeof(To) == sizeof(From), "size mismatch");
+  return __builtin_bit_cast(To, x);
+}
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// ---------------------------------------------------------------------------
+// DPP moves on any trivially copyable type whose size is a multiple of 4:
+// each 32-bit chunk goes through update_dpp(old, src, ctrl, row_mask,
+// bank_mask, bound_ctrl=false); lanes whose source is out of range, or whose
+// row is masked off, keep `old`.
+// ---------------------------------------------------------------------------
+template <int N>
+struct Chunks {
+  uint32_t w[N];
+};
+
+template <int CTRL, int ROW_MASK = 0xf, int BANK_MASK = 0xf, class T>
+__device__ __forceinline__ T dpp_move(T old, T src) {
+  static_assert(sizeof(T) % 4 == 0, "dpp_move needs a size that is a multiple of 4");
+  constexpr int N = sizeof(T) / 4;
+  auto o = bitcast<Chunks<N>>(old);
+  auto s = bitcast<Chunks<N>>(src);
+  Chunks<N> r;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+    r.w[i] = uint32_t(__builtin_amdgcn_update_dpp(int(o.w[i]), int(s.w[i]), CTRL, ROW_MASK,
+                                                  BANK_MASK, false));
+  return bitcast<T>(r);
+}
+
+// DPP control codes (GFX9 encoding).
+constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
+constexpr int kWaveShr1 = 0x138;
+constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143;
+
+template <class T>
+__device__ __forceinline__ T readlane(T x, int lane) {
+  constexpr int N = sizeof(T) / 4;
+  auto c = bitcast<Chunks<N>>(x);
+#pragma unroll
+  for (int i = 0; i < N; ++i) c.w[i] = uint32_t(__builtin_amdgcn_readlane(int(c.w[i]), lane));
+  return bitcast<T>(c);
+}
+
+// Inclusive scan across the 64 lanes of a wavefront: 4 row shifts inside
+// each 16-lane row, then two row broadcasts join the rows.
+template <class Op, class T>
+__device__ __forceinline__ T wave_inclusive_scan(T x) {
+  const T id = Op::template identity<T>();
+  x = Op::apply(dpp_move<kRowShr1>(id, x), x);
+  x = Op::apply(dpp_move<kRowShr2>(id, x), x);
+  x = Op::apply(dpp_move<kRowShr4>(id, x), x);
+  x = Op::apply(dpp_move<kRowShr8>(id, x), x);
+  x = Op::apply(dpp_move<kRowBcast15, 0xa>(id, x), x);
+  x = Op::apply(dpp_move<kRowBcast31, 0xc>(id, x), x);
+  return x;
+}
+
+// Shift a wave-wide inclusive scan right by one lane (lane 0 gets identity),
+// turning it into the exclusive scan.
+template <class Op, class T>
+__device__ __forceinline__ T wave_shift_right(T x) {
+  return dpp_move<kWaveShr1>(Op::template identity<T>(), x);
+}
+
+// Reduction over the wavefront; every lane receives the total.
+template <class Op, class T>
+__device__ __forceinline__ T wave_reduce(T x) {
+  return readlane(wave_inclusive_scan<Op>(x), kWave - 1);
+}
+
+// ---------------------------------------------------------------------------
+// Block-level helpers (blockDim.x == BLOCK, BLOCK a multiple of 64).
+// ---------------------------------------------------------------------------
+template <int BLOCK, class Op, class T>
+__device__ __forceinline__ T block_reduce(T x, T* s_wave /* BLOCK/64 */) {
+  constexpr int NW = BLOCK / kWave;
+  T w = wave_reduce<Op>(x);
+  const int wid = threadIdx.x / kWave;
+  if (lane_id() == 0) s_wave[wid] = w;
+  __syncthreads();
+  T r = s_wave[0];
+#pragma unroll
+  for (int i = 1; i < NW; ++i) r = Op::apply(r, s_wave[i]);
+  __syncthreads();
+  return r;
+}
+
+// Exclusive block scan of one value per thread.  Returns the thread's
+// exclusive prefix; *total receives the block aggregate.
+template <int BLOCK, class Op, class T>
+__device__ __forceinline__ T block_exclusive_scan(T x, T* s_wave, T* total) {
+  constexpr int NW = BLOCK / kWave;
+  const int wid = threadIdx.x / kWave;
+  T inc = wave_inclusive_scan<Op>(x);
+  T exc = wave_shift_right<Op>(inc);
+  if (lane_id() == kWave - 1) s_wave[wid] = inc;
+  __syncthreads();
+  T wp = Op::template identity<T>();
+  T agg = Op::template identity<T>();
+#pragma unroll
+  for (int i = 0; i < NW; ++i) {
+    T s = s_wave[i];
+    if (i < wid) wp = Op::apply(wp, s);
+    agg = Op::apply(agg, s);
+  }
+  __syncthreads();
+  *total = agg;
+  return Op::apply(wp, exc);
+}
+
+// ---------------------------------------------------------------------------
+// Agent-scope relaxed atomics (coherent across the 8 XCDs' L2 slices).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t ld_agent(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------------------
+// Decoupled look-back tile status.
+//
+// Each tile owns W = sizeof(T)/4 64-bit status words.  Word j holds bits
+// [32j, 32j+32) of the value in its low half and the status kind in its
+// high half.  Kinds: 0 = not yet published, 1 = tile aggregate,
+// 2 = inclusive prefix.  A reader accepts a tile only when all of its words
+// carry the same non-zero kind, so a value is never torn across words
+// without any memory fence.  Words are cleared by the preparation kernel of
+// each call.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kStatusInvalid = 0, kStatusAggregate = 1, kStatusPrefix = 2;
+
+// Bounded spin: a wave that waits this many polls for a predecessor gives up,
+// flags the error word and treats the predecessor as an identity prefix, so
+// no kernel can hang forever on a protocol bug.
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+template <class T>
+struct TileStatus {
+  static constexpr int W = sizeof(T) / 4;
+  uint64_t* words;  // W words per tile
+
+  __device__ __forceinline__ void publish(int64_t tile, uint32_t kind, T value) const {
+    auto c = bitcast<Chunks<W>>(value);
+#pragma unroll
+    for (int j = 0; j < W; ++j) st_agent(words + tile * W + j, (uint64_t(kind) << 32) | c.w[j]);
+  }
+
+  // Returns the kind (0 when not yet published or not yet consistent) and the value.
+  __device__ __forceinline__ uint32_t read(int64_t tile, T* value) const {
+    Chunks<W> c;
+    uint32_t kind = 0;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+      uint64_t w = ld_agent(words + tile * W + j);
+      uint32_t k = uint32_t(w >> 32);
+      if (j == 0) kind = k;
+      else if (k != kind) kind = kStatusInvalid;
+      c.w[j] = uint32_t(w);
+    }
+    *value = bitcast<T>(c);
+    return kind;
+  }
+};
+
+// Wave-parallel look-back run by one full wavefront: lane l inspects tile
+// (pos - l); the nearest inclusive prefix ends the walk, aggregates in front
+// of it are folded in.  Returns the exclusive prefix of `tile` (tile >= 1).
+template <class Op, class T>
+__device__ T lookback_exclusive_prefix(const TileStatus<T>& st, int64_t tile, uint32_t* err) {
+  const int lane = lane_id();
+  const T id = Op::template identity<T>();
+  T acc = id;  // combination of every tile in (pos, tile)
+  int64_t pos = tile - 1;
+  while (true) {
+    const int64_t idx = pos - lane;
+    T val = id;
+    uint32_t kind = kStatusPrefix;  // before tile 0: identity prefix
+    if (idx >= 0) kind = st.read(idx, &val);
+    uint32_t spins = 0;
+    while (__any(kind == kStatusInvalid)) {
+      __builtin_amdgcn_s_sleep(1);
+      if (kind == kStatusInvalid) kind = st.read(idx, &val);
+      if (++spins > kSpinLimit) {
+        if (kind == kStatusInvalid) {
+          atomicOr(err, 1u);
+          kind = kStatusPrefix;
+          val = id;
+        }
+      }
+    }
+    const uint64_t pmask = __ballot(kind == kStatusPrefix);
+    if (pmask) {
+      const int p = __builtin_ctzll(pmask);
+      T c = lane <= p ? val : id;
+      return Op::apply(wave_reduce<Op>(c), acc);
+    }
+    acc = Op::apply(wave_reduce<Op>(val), acc);
+    pos -= kWave;
+  }
+}
+
+// Native 16-byte vector (the HIP uint4 is a struct; builtins want a vector).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// Tile loads/stores: each thread owns ITEMS consecutive elements ("blocked"
+// arrangement).  Full tiles with 16-byte aligned bases use dwordx4 accesses.
+// ---------------------------------------------------------------------------
+template <class T, int ITEMS>
+__device__ __forceinline__ void load_blocked(const T* __restrict__ p, int64_t valid, T (&v)[ITEMS],
+                                             T fill, bool full_aligned) {
+  static_assert((ITEMS * sizeof(T)) % 16 == 0, "ITEMS*sizeof(T) must be a multiple of 16");
+  if (full_aligned) {
+    const u32x4* q = reinterpret_cast<const u32x4*>(p);
+    constexpr int NV = ITEMS * sizeof(T) / 16;
+    u32x4 tmp[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) tmp[i] = q[i];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      T* d = reinterpret_cast<T*>(&tmp[i]);
+#pragma unroll
+      for (int j = 0; j < int(16 / sizeof(T)); ++j) v[i * (16 / sizeof(T)) + j] = d[j];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) v[i] = i < valid ? p[i] : fill;
+  }
+}
+
+template <class T, int ITEMS>
+__device__ __forceinline__ void store_blocked(T* __restrict__ p, int64_t valid, const T (&v)[ITEMS],
+                                              bool full_aligned) {
+  if (full_aligned) {
+    u32x4* q = reinterpret_cast<u32x4*>(p);
+    constexpr int NV = ITEMS * sizeof(T) / 16;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      u32x4 t;
+      T* s = reinterpret_cast<T*>(&t);
+#pragma unroll
+      for (int j = 0; j < int(16 / sizeof(T)); ++j) s[j] = v[i * (16 / sizeof(T)) + j];
+      q[i] = t;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i)
+      if (i < valid) p[i] = v[i];
+  }
+}
+
+__device__ __forceinline__ bool aligned16(const void* p) {
+  return (reinterpret_cast<uintptr_t>(p) & 15) == 0;
+}
+
+}  // namespace bulkx

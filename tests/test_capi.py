@@ -1,0 +1,78 @@
+"""C-ABI checks that need no GPU: the library loads, exports every symbol
+include/bulkx.h declares, and rejects bad arguments before touching a device."""
+import ctypes
+
+import pytest
+
+from bulk_amd import _capi as C
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.lib()
+    declared = C.header_symbols()
+    assert len(declared) >= 20
+    missing = [s for s in declared if not hasattr(lib, s)]
+    assert missing == []
+    # the ctypes table binds every declared function with a signature
+    assert sorted(set(declared) - set(C.SIGNATURES)) == []
+
+
+def test_status_strings():
+    lib = C.lib()
+    for code, name in C.STATUS_NAMES.items():
+        assert lib.bulkx_status_string(code).decode() == name
+    assert lib.bulkx_status_string(999).decode() == "BULKX_ERR_UNKNOWN"
+    assert lib.bulkx_version().decode().count(".") == 2
+
+
+def test_dtype_sizes():
+    lib = C.lib()
+    assert [lib.bulkx_dtype_size(d) for d in range(6)] == [4, 4, 8, 8, 4, 8]
+    assert lib.bulkx_dtype_size(6) == 0
+
+
+@pytest.mark.parametrize("call", [
+    lambda l: l.bulkx_inclusive_scan(99, 0, None, None, 10, None, None, 0, None),
+    lambda l: l.bulkx_inclusive_scan(0, 7, None, None, 10, None, None, 0, None),
+    lambda l: l.bulkx_inclusive_scan(0, 0, None, None, -1, None, None, 0, None),
+    lambda l: l.bulkx_inclusive_scan(0, 0, None, None, 10, None, None, 0, None),
+    lambda l: l.bulkx_exclusive_scan(0, 0, None, None, 10, None, None, 0, None),
+    lambda l: l.bulkx_reduce(0, 0, None, 10, None, None, None, 0, None),
+    lambda l: l.bulkx_merge(0, 5, None, 1, None, 1, None, None, 0, None),
+    lambda l: l.bulkx_stable_sort_by_key(0, 0, 0, None, None, -3, None, 0, None),
+    lambda l: l.bulkx_reduce_by_key(0, 4, 0, 0, None, None, 10, None, None, None, None, 0, None),
+    lambda l: l.bulkx_reduce_intervals(0, 0, None, 10, 2, 0, None, None),
+])
+def test_invalid_arguments_rejected_before_launch(call):
+    status = call(C.lib())
+    assert status in (1, 2, 3)
+
+
+def test_zero_length_calls_are_noops():
+    lib = C.lib()
+    assert lib.bulkx_inclusive_scan(0, 0, None, None, 0, None, None, 0, None) == 0
+    assert lib.bulkx_stable_sort_by_key(0, 0, 0, None, None, 0, None, 0, None) == 0
+    assert lib.bulkx_merge(0, 0, None, 0, None, 0, None, None, 0, None) == 0
+
+
+def test_workspace_size_query_is_host_only():
+    lib = C.lib()
+    nb = ctypes.c_size_t()
+    assert lib.bulkx_workspace_size(C.ALGO_INCLUSIVE_SCAN, C.INT32, -1, 1 << 24, 0,
+                                    ctypes.byref(nb)) == 0
+    # one status word per 4096-element tile plus a header
+    assert nb.value == 256 + 8 * ((1 << 24) // 4096)
+    assert lib.bulkx_workspace_size(C.ALGO_INCLUSIVE_SCAN, C.FLOAT64, -1, 1 << 20, 0,
+                                    ctypes.byref(nb)) == 0
+    assert nb.value == 256 + 32 * ((1 << 20) // 2048)
+    assert lib.bulkx_workspace_size(C.ALGO_INCLUSIVE_SCAN, 9, -1, 10, 0, ctypes.byref(nb)) == 2
+
+
+def test_device_info_without_gpu_reports_no_device():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    lib = C.lib()
+    cus = ctypes.c_int()
+    st = lib.bulkx_device_info(ctypes.byref(cus), None, None, None, None)
+    assert st in (5, 6)
