@@ -15,6 +15,6 @@ __all__ = ["BulkError", "BulkLibraryMissing", "api"]
 def __getattr__(name):
     # api imports torch; load it lazily so the C-ABI checks run without it.
     if name == "api":
-        from . import api
-        return api
+        import importlib
+        return importlib.import_module(__name__ + ".api")
     raise AttributeError(name)

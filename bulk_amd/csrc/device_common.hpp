@@ -161,7 +161,7 @@ __device__ __forceinline__ T dpp_move(T old, T src) {
 
 // DPP control codes (GFX9 encoding).
 constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
-constexpr int kWaveShr1 = 0x138;
+constexpr int kWaveShl1 = 0x130, kWaveShr1 = 0x138;
 constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143;
 
 template <class T>
@@ -296,40 +296,68 @@ struct TileStatus {
   }
 };
 
-// Wave-parallel look-back run by one full wavefront: lane l inspects tile
-// (pos - l); the nearest inclusive prefix ends the walk, aggregates in front
-// of it are folded in.  Returns the exclusive prefix of `tile` (tile >= 1).
-template <class Op, class T>
+// Wave-parallel look-back run by one full wavefront over windows of 64*Q
+// predecessors: lane l inspects the Q consecutive tiles starting at
+// (pos - 64*Q + 1 + l*Q), so lane order is tile order (oldest on the left)
+// and non-commutative operators combine correctly.  The newest inclusive
+// prefix in a window ends the walk; the aggregates after it are folded in.
+// One window costs one memory round trip, so the inclusive-prefix front
+// advances 64*Q tiles per round trip.  Returns the exclusive prefix of
+// `tile` (tile >= 1).
+template <class Op, int Q = 4, class T>
 __device__ T lookback_exclusive_prefix(const TileStatus<T>& st, int64_t tile, uint32_t* err) {
+  constexpr int WIN = kWave * Q;
   const int lane = lane_id();
   const T id = Op::template identity<T>();
   T acc = id;  // combination of every tile in (pos, tile)
-  int64_t pos = tile - 1;
+  int64_t pos = tile - 1;  // newest tile of the current window
   while (true) {
-    const int64_t idx = pos - lane;
-    T val = id;
-    uint32_t kind = kStatusPrefix;  // before tile 0: identity prefix
-    if (idx >= 0) kind = st.read(idx, &val);
+    const int64_t first = pos - WIN + 1 + int64_t(lane) * Q;
+    T val[Q];
+    uint32_t kind[Q];
+#pragma unroll
+    for (int j = 0; j < Q; ++j) {
+      val[j] = id;
+      kind[j] = kStatusPrefix;  // before tile 0: identity prefix
+      if (first + j >= 0) kind[j] = st.read(first + j, &val[j]);
+    }
     uint32_t spins = 0;
-    while (__any(kind == kStatusInvalid)) {
+    while (true) {
+      bool missing = false;
+#pragma unroll
+      for (int j = 0; j < Q; ++j) missing |= kind[j] == kStatusInvalid;
+      if (!__any(missing)) break;
       __builtin_amdgcn_s_sleep(1);
-      if (kind == kStatusInvalid) kind = st.read(idx, &val);
+#pragma unroll
+      for (int j = 0; j < Q; ++j)
+        if (kind[j] == kStatusInvalid) kind[j] = st.read(first + j, &val[j]);
       if (++spins > kSpinLimit) {
-        if (kind == kStatusInvalid) {
-          atomicOr(err, 1u);
-          kind = kStatusPrefix;
-          val = id;
+#pragma unroll
+        for (int j = 0; j < Q; ++j) {
+          if (kind[j] == kStatusInvalid) {
+            atomicOr(err, 1u);
+            kind[j] = kStatusPrefix;
+            val[j] = id;
+          }
         }
       }
     }
-    const uint64_t pmask = __ballot(kind == kStatusPrefix);
+    // Newest prefix among this lane's Q tiles, and the fold from it onwards.
+    int jp = -1;
+#pragma unroll
+    for (int j = 0; j < Q; ++j)
+      if (kind[j] == kStatusPrefix) jp = j;
+    T c = id;
+#pragma unroll
+    for (int j = 0; j < Q; ++j)
+      if (j >= jp) c = Op::apply(c, val[j]);
+    const uint64_t pmask = __ballot(jp >= 0);
     if (pmask) {
-      const int p = __builtin_ctzll(pmask);
-      T c = lane <= p ? val : id;
-      return Op::apply(wave_reduce<Op>(c), acc);
+      const int p = 63 - __builtin_clzll(pmask);  // lane holding the newest prefix
+      return Op::apply(wave_reduce<Op>(lane >= p ? c : id), acc);
     }
-    acc = Op::apply(wave_reduce<Op>(val), acc);
-    pos -= kWave;
+    acc = Op::apply(wave_reduce<Op>(c), acc);
+    pos -= WIN;
   }
 }
 
@@ -385,6 +413,99 @@ __device__ __forceinline__ void store_blocked(T* __restrict__ p, int64_t valid, 
 
 __device__ __forceinline__ bool aligned16(const void* p) {
   return (reinterpret_cast<uintptr_t>(p) & 15) == 0;
+}
+
+// Orders a wavefront's own LDS accesses (all LDS traffic of the wave has
+// completed before any later access is issued).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront-striped global access with an LDS transpose.
+//
+// A wavefront owns a contiguous chunk of 64*ITEMS elements; lane l works on
+// the ITEMS consecutive elements [l*ITEMS, (l+1)*ITEMS) of it ("blocked"
+// registers, what sequential per-thread scans and merges need).  Global
+// memory is touched "striped": instruction j of lane l moves the 16-byte
+// vector j*64 + l, so every instruction covers 1 KB of consecutive bytes.
+// The per-wave LDS region holds one pad element after every 128 bytes, which
+// makes the blocked reads and writes bank-conflict free for 4-byte types.
+// Only the owning wavefront touches its region, so no workgroup barrier is
+// needed.
+// ---------------------------------------------------------------------------
+template <class T, int ITEMS>
+struct WaveTile {
+  static constexpr int N = kWave * ITEMS;          // elements per wave chunk
+  static constexpr int VE = 16 / int(sizeof(T));   // elements per 16-byte vector
+  static constexpr int NV = ITEMS / VE;            // vectors per lane
+  static constexpr int PAD = 128 / int(sizeof(T));  // one pad element per 128 bytes
+  static constexpr int LDS_ELEMS = N + N / PAD;
+  static_assert(ITEMS % VE == 0, "ITEMS must fill whole 16-byte vectors");
+  __device__ __forceinline__ static int pos(int e) { return e + e / PAD; }
+};
+
+// Load chunk[0, count) (count <= N) into blocked registers; missing elements
+// get `fill`.
+template <class T, int ITEMS>
+__device__ __forceinline__ void wave_load_blocked(const T* chunk, int count, T (&v)[ITEMS], T fill,
+                                                  T* lds) {
+  using W = WaveTile<T, ITEMS>;
+  const int lane = lane_id();
+  if (count >= W::N && aligned16(chunk)) {
+    const u32x4* q = reinterpret_cast<const u32x4*>(chunk);
+    u32x4 t[W::NV];
+#pragma unroll
+    for (int j = 0; j < W::NV; ++j) t[j] = q[j * kWave + lane];
+#pragma unroll
+    for (int j = 0; j < W::NV; ++j) {
+      const T* e = reinterpret_cast<const T*>(&t[j]);
+      const int base = W::pos((j * kWave + lane) * W::VE);
+#pragma unroll
+      for (int k = 0; k < W::VE; ++k) lds[base + k] = e[k];
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const int e = j * kWave + lane;
+      lds[W::pos(e)] = e < count ? chunk[e] : fill;
+    }
+  }
+  wave_lds_sync();
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) v[i] = lds[W::pos(lane * ITEMS + i)];
+  wave_lds_sync();
+}
+
+// Store blocked registers to chunk[0, count) (count <= N).
+template <class T, int ITEMS>
+__device__ __forceinline__ void wave_store_blocked(T* chunk, int count, const T (&v)[ITEMS],
+                                                   T* lds) {
+  using W = WaveTile<T, ITEMS>;
+  const int lane = lane_id();
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) lds[W::pos(lane * ITEMS + i)] = v[i];
+  wave_lds_sync();
+  if (count >= W::N && aligned16(chunk)) {
+    u32x4* q = reinterpret_cast<u32x4*>(chunk);
+#pragma unroll
+    for (int j = 0; j < W::NV; ++j) {
+      u32x4 t;
+      T* e = reinterpret_cast<T*>(&t);
+      const int base = W::pos((j * kWave + lane) * W::VE);
+#pragma unroll
+      for (int k = 0; k < W::VE; ++k) e[k] = lds[base + k];
+      q[j * kWave + lane] = t;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const int e = j * kWave + lane;
+      if (e < count) chunk[e] = lds[W::pos(e)];
+    }
+  }
+  wave_lds_sync();
 }
 
 }  // namespace bulkx

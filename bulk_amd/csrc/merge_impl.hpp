@@ -1,0 +1,162 @@
+// Merge-path machinery shared by merge, merge_by_key and the merge sort.
+//
+// bulk::merge_path (merge.hpp:35-60) finds, for a diagonal d of the merge
+// matrix, how many of the first d outputs come from range A; ties go to A
+// because the probe is comp(B[d-1-m], A[m]).  Here the global search of a
+// tile boundary is done by one wavefront probing 64 positions per step
+// (about 5 steps for 2^27 elements instead of 27 dependent loads), so no
+// separate partition launch is needed (the reference's thrust::tabulate,
+// merge.cu:152-154).  The tile itself is staged through LDS like the
+// reference's staged_merge (merge.cu:12-36): each thread finds its own
+// diagonal inside the tile and runs the sequential agent merge
+// (merge.hpp:63-170) over its grain.
+#pragma once
+
+#include "device_common.hpp"
+#include "host_common.hpp"
+
+namespace bulkx {
+
+template <class K>
+struct MergeConfig {
+  static constexpr int BLOCK = 256;
+  static constexpr int ITEMS = sizeof(K) == 4 ? 16 : 8;
+  static constexpr int TILE = BLOCK * ITEMS;
+};
+
+// Wave-cooperative merge_path over global memory.  Every lane of the
+// calling wavefront must participate; every lane returns the same result.
+template <class CMP, class K>
+__device__ int64_t wave_merge_path(const K* a, int64_t n1, const K* b, int64_t n2, int64_t diag) {
+  const int lane = lane_id();
+  int64_t lo = diag > n2 ? diag - n2 : 0;
+  int64_t hi = diag < n1 ? diag : n1;
+  while (lo < hi) {
+    const int64_t len = hi - lo;
+    int64_t m;
+    bool inside;
+    if (len <= kWave) {
+      m = lo + lane;
+      inside = m < hi;
+    } else {
+      m = lo + ((len * lane) >> 6);
+      inside = true;
+    }
+    // predicate is monotone in m: false below the answer, true from it on
+    bool p = true;
+    if (inside) p = CMP::apply(b[diag - 1 - m], a[m]);
+    const uint64_t mask = __ballot(p);
+    if (len <= kWave) {
+      lo = mask ? readlane(m, __builtin_ctzll(mask)) : hi;
+      break;
+    }
+    if (mask == 0) {
+      lo = readlane(m, kWave - 1) + 1;
+    } else {
+      const int f = __builtin_ctzll(mask);
+      const int64_t mf = readlane(m, f);
+      if (f > 0) lo = readlane(m, f - 1) + 1;
+      hi = mf;
+    }
+  }
+  return lo;
+}
+
+// Sequential merge_path over LDS: A = s[0, n1), B = s[b_off, b_off + n2).
+template <class CMP, class K>
+__device__ __forceinline__ int lds_merge_path(const K* s, int n1, int b_off, int n2, int diag) {
+  int lo = diag > n2 ? diag - n2 : 0;
+  int hi = diag < n1 ? diag : n1;
+  while (lo < hi) {
+    int m = (lo + hi) >> 1;
+    if (CMP::apply(s[b_off + diag - 1 - m], s[m])) hi = m;
+    else lo = m + 1;
+  }
+  return lo;
+}
+
+// Sequential agent merge of up to ITEMS outputs from LDS runs
+// A = [i1, e1), B = [i2, e2) (the merge.hpp:109-129 body: take A unless B's
+// head compares strictly less).  Produces keys and the LDS index they came from.
+template <class CMP, class K, int ITEMS>
+__device__ __forceinline__ void lds_serial_merge(const K* s, int i1, int e1, int i2, int e2,
+                                                 int count, K (&rk)[ITEMS], int (&ri)[ITEMS]) {
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    if (i < count) {
+      const bool has1 = i1 < e1, has2 = i2 < e2;
+      const K ka = has1 ? s[i1] : K();
+      const K kb = has2 ? s[i2] : K();
+      const bool p = !has2 || (has1 && !CMP::apply(kb, ka));
+      rk[i] = p ? ka : kb;
+      ri[i] = p ? i1 : i2;
+      if (p) ++i1;
+      else ++i2;
+    }
+  }
+}
+
+// LDS elements the staging arrays of merge_tile need: the tile itself, and
+// the padded per-wave transpose regions used for the output stores.
+template <class T, int BLOCK, int ITEMS>
+struct MergeLds {
+  static constexpr int ELEMS = (BLOCK / kWave) * WaveTile<T, ITEMS>::LDS_ELEMS;
+};
+
+// Merge of one tile: A[0, na) and B[0, nb) (na + nb <= TILE) into
+// out[0, na + nb).  s_k / s_v hold MergeLds<...>::ELEMS elements.  Ends with
+// a workgroup barrier, so it can be called again on the same LDS.
+template <class CMP, class K, class V, bool HAS_V, int BLOCK, int ITEMS>
+__device__ __forceinline__ void merge_tile(const K* a, const V* va, int na, const K* b,
+                                           const V* vb, int nb, K* out, V* vout, K* s_k,
+                                           V* s_v) {
+  // Bounds guard (only reachable with a comparator that is not a strict weak
+  // order, e.g. NaN keys): never stage or write more than one tile.
+  constexpr int TILE = BLOCK * ITEMS;
+  na = na < 0 ? 0 : (na > TILE ? TILE : na);
+  nb = nb < 0 ? 0 : (nb > TILE - na ? TILE - na : nb);
+  const int cnt = na + nb;
+  for (int j = threadIdx.x; j < cnt; j += BLOCK) {
+    const bool fa = j < na;
+    s_k[j] = fa ? a[j] : b[j - na];
+    if constexpr (HAS_V) s_v[j] = fa ? va[j] : vb[j - na];
+  }
+  __syncthreads();
+  const int d = threadIdx.x * ITEMS < cnt ? threadIdx.x * ITEMS : cnt;
+  const int ai = lds_merge_path<CMP>(s_k, na, na, nb, d);
+  const int bi = na + (d - ai);
+  K rk[ITEMS];
+  V rv[ITEMS];
+  int ri[ITEMS];
+  const int count = cnt - d < ITEMS ? cnt - d : ITEMS;
+  lds_serial_merge<CMP, K, ITEMS>(s_k, ai, na, bi, cnt, count, rk, ri);
+  if constexpr (HAS_V) {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) rv[i] = i < count ? s_v[ri[i]] : V();
+  }
+  __syncthreads();  // every read of the stage is done; reuse it for the transposes
+  const int wid = threadIdx.x / kWave;
+  const int wb = wid * WaveTile<K, ITEMS>::N;
+  const int wcount = cnt - wb <= 0 ? 0 : (cnt - wb >= WaveTile<K, ITEMS>::N ? WaveTile<K, ITEMS>::N : cnt - wb);
+  if (wcount > 0) {
+    wave_store_blocked<K, ITEMS>(out + wb, wcount, rk, s_k + wid * WaveTile<K, ITEMS>::LDS_ELEMS);
+    if constexpr (HAS_V)
+      wave_store_blocked<V, ITEMS>(vout + wb, wcount, rv, s_v + wid * WaveTile<V, ITEMS>::LDS_ELEMS);
+  }
+  __syncthreads();
+}
+
+// Tile boundaries of a two-range merge: wave 0 searches the tile's first
+// diagonal, wave 1 its last; results land in s_split[0..1].
+template <class CMP, class K>
+__device__ __forceinline__ void tile_splits(const K* a, int64_t n1, const K* b, int64_t n2,
+                                            int64_t d0, int64_t d1, int64_t* s_split) {
+  const int wid = threadIdx.x / kWave;
+  if (wid < 2) {
+    const int64_t r = wave_merge_path<CMP>(a, n1, b, n2, wid == 0 ? d0 : d1);
+    if (lane_id() == 0) s_split[wid] = r;
+  }
+  __syncthreads();
+}
+
+}  // namespace bulkx

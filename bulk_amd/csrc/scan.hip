@@ -36,9 +36,11 @@ __global__ __launch_bounds__(BLOCK) void scan_kernel(const T* in, T* out, int64_
   using A = typename AccP::type;
   using AOp = typename AccP::op;
   constexpr int TILE = BLOCK * ITEMS;
+  using WT = WaveTile<T, ITEMS>;
   __shared__ A s_wave[BLOCK / kWave];
   __shared__ A s_prefix;
   __shared__ uint32_t s_tile;
+  __shared__ T s_tr[BLOCK / kWave][WT::LDS_ELEMS];
   const TileStatus<A> st{status_words};
 
   int64_t tile;
@@ -49,15 +51,15 @@ __global__ __launch_bounds__(BLOCK) void scan_kernel(const T* in, T* out, int64_
     __syncthreads();
     tile = s_tile;
   }
-  const int64_t tile_base = tile * TILE;
-  const int64_t my_base = tile_base + int64_t(threadIdx.x) * ITEMS;
-  const int64_t valid = n - my_base;  // may be <= 0 for trailing threads
-  const bool full = (tile_base + TILE <= n);
+  const int wid = threadIdx.x / kWave;
+  const int64_t chunk_base = tile * TILE + int64_t(wid) * WT::N;
+  const int64_t rem = n - chunk_base;
+  const int count = rem <= 0 ? 0 : (rem >= WT::N ? WT::N : int(rem));
   const T id = Op::template identity<T>();
   const A aid = AOp::template identity<A>();
 
   T v[ITEMS];
-  load_blocked<T, ITEMS>(in + (valid > 0 ? my_base : 0), valid, v, id, full && aligned16(in));
+  wave_load_blocked<T, ITEMS>(in + (count > 0 ? chunk_base : 0), count, v, id, s_tr[wid]);
 
   // Thread-sequential inclusive scan of the owned elements (the bounded-agent
   // body of bulk::inclusive_scan, scan.hpp:35-55).
@@ -95,7 +97,7 @@ __global__ __launch_bounds__(BLOCK) void scan_kernel(const T* in, T* out, int64_
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) v[i] = AccP::to(AOp::apply(p, AccP::from(v[i])));
   }
-  if (valid > 0) store_blocked<T, ITEMS>(out + my_base, valid, v, full && aligned16(out));
+  if (count > 0) wave_store_blocked<T, ITEMS>(out + chunk_base, count, v, s_tr[wid]);
 }
 
 template <class T>

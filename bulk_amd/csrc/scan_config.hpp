@@ -7,7 +7,7 @@ namespace bulkx {
 template <class T>
 struct ScanConfig {
   static constexpr int BLOCK = 256;
-  static constexpr int ITEMS = sizeof(T) == 4 ? 16 : 8;
+  static constexpr int ITEMS = sizeof(T) == 4 ? 32 : 16;
 };
 
 }  // namespace bulkx

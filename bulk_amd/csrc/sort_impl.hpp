@@ -1,0 +1,210 @@
+// Stable merge sort by key: replaces stable_merge_sort_by_key
+// (merge_sort_by_key.cu:150-209).
+//
+// Phase 1 (block sort, as stable_sort_each_kernel merge_sort_by_key.cu:12-25
+// -> stable_merge_sort.hpp:125-206): each workgroup loads a tile of
+// BLOCK*ITEMS keys, every thread sorts its grain by odd-even transposition
+// (sort.hpp:33-75, stable with a strict comparator), then log2(BLOCK)
+// rounds of LDS merge-path merges double the sorted runs.  Only (key, tile
+// index) pairs move between rounds; values are gathered once at the end.
+// Keys and indices live in separate LDS arrays with a barrier between the
+// reads of one round and the writes of the next, so the reference's
+// shared key/value stage hazard (stable_merge_sort.hpp:97-115) cannot occur.
+//
+// Phase 2 (merge passes, as merge_by_key_kernel merge_sort_by_key.cu:61-87
+// and locate_merge_path :90-131): runs of width W = TILE, 2*TILE, ... are
+// merged pairwise, one output tile per workgroup, ping-ponging between the
+// caller's arrays and the workspace; the block sort writes to whichever
+// buffer makes the last pass land in the caller's arrays, so no copy-back
+// (the reference's K12, merge_sort_by_key.cu:204-208) is needed.
+#pragma once
+
+#include "merge_impl.hpp"
+
+namespace bulkx {
+
+template <class K>
+using SortConfig = MergeConfig<K>;
+
+template <class CMP, class K, class V, bool HAS_V, int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void block_sort_kernel(const K* keys, const V* vals, int64_t n,
+                                                           K* kout, V* vout) {
+  constexpr int TILE = BLOCK * ITEMS;
+  using WK = WaveTile<K, ITEMS>;
+  using WV = WaveTile<V, ITEMS>;
+  __shared__ K s_k[MergeLds<K, BLOCK, ITEMS>::ELEMS];
+  __shared__ uint16_t s_i[TILE];
+  __shared__ V s_v[HAS_V ? MergeLds<V, BLOCK, ITEMS>::ELEMS : 1];
+
+  const int64_t base = int64_t(blockIdx.x) * TILE;
+  const int cnt = int(n - base < TILE ? n - base : TILE);
+  const int mine = threadIdx.x * ITEMS;
+  int vt = cnt - mine;  // valid items of this thread
+  vt = vt < 0 ? 0 : (vt > ITEMS ? ITEMS : vt);
+  const int wid = threadIdx.x / kWave;
+  const int wb = wid * WK::N;
+  const int wcount = cnt - wb <= 0 ? 0 : (cnt - wb >= WK::N ? WK::N : cnt - wb);
+
+  K k[ITEMS];
+  int idx[ITEMS];
+  // Coalesced load through the wave's transpose region (inside s_k, which
+  // the merge rounds only reuse after their first barrier).
+  wave_load_blocked<K, ITEMS>(keys + base + (wcount > 0 ? wb : 0), wcount, k, K(),
+                              s_k + wid * WK::LDS_ELEMS);
+  if constexpr (HAS_V) {
+    for (int j = threadIdx.x; j < cnt; j += BLOCK) s_v[j] = vals[base + j];
+  }
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) idx[i] = mine + i;
+
+  // Odd-even transposition sort of the grain; items at or past vt never move.
+#pragma unroll
+  for (int r = 0; r < ITEMS; ++r) {
+#pragma unroll
+    for (int j = r & 1; j < ITEMS - 1; j += 2) {
+      if (j + 1 < vt && CMP::apply(k[j + 1], k[j])) {
+        K tk = k[j];
+        k[j] = k[j + 1];
+        k[j + 1] = tk;
+        int ti = idx[j];
+        idx[j] = idx[j + 1];
+        idx[j + 1] = ti;
+      }
+    }
+  }
+
+  // Merge rounds inside the tile: runs of w items -> runs of 2w.
+  for (int w = ITEMS; w < TILE; w *= 2) {
+    __syncthreads();  // previous round's LDS reads are done
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      if (i < vt) {
+        s_k[mine + i] = k[i];
+        s_i[mine + i] = uint16_t(idx[i]);
+      }
+    }
+    __syncthreads();
+    if (vt > 0) {
+      const int ps = (mine / (2 * w)) * (2 * w);
+      int na = cnt - ps;
+      na = na < 0 ? 0 : (na > w ? w : na);
+      int nb = cnt - ps - w;
+      nb = nb < 0 ? 0 : (nb > w ? w : nb);
+      const int diag = mine - ps;
+      const int ai = lds_merge_path<CMP>(s_k + ps, na, w, nb, diag);
+      const int bi = diag - ai;
+      K rk[ITEMS];
+      int ri[ITEMS];
+      lds_serial_merge<CMP, K, ITEMS>(s_k, ps + ai, ps + na, ps + w + bi, ps + w + nb, vt, rk, ri);
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) {
+        if (i < vt) {
+          k[i] = rk[i];
+          idx[i] = s_i[ri[i]];
+        }
+      }
+    }
+  }
+
+  V rv[ITEMS];
+  if constexpr (HAS_V) {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) rv[i] = i < vt ? s_v[idx[i]] : V();
+  }
+  __syncthreads();  // all LDS reads done; reuse s_k / s_v as transpose regions
+  if (wcount > 0) {
+    wave_store_blocked<K, ITEMS>(kout + base + wb, wcount, k, s_k + wid * WK::LDS_ELEMS);
+    if constexpr (HAS_V)
+      wave_store_blocked<V, ITEMS>(vout + base + wb, wcount, rv, s_v + wid * WV::LDS_ELEMS);
+  }
+}
+
+// One merge pass: pairs of sorted runs of width `width` (a multiple of TILE)
+// from (src_k, src_v) into (dst_k, dst_v).
+template <class CMP, class K, class V, bool HAS_V, int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void merge_pass_kernel(const K* __restrict__ src_k,
+                                                           const V* __restrict__ src_v, int64_t n,
+                                                           int64_t width, K* __restrict__ dst_k,
+                                                           V* __restrict__ dst_v) {
+  constexpr int TILE = BLOCK * ITEMS;
+  __shared__ K s_k[MergeLds<K, BLOCK, ITEMS>::ELEMS];
+  __shared__ V s_v[HAS_V ? MergeLds<V, BLOCK, ITEMS>::ELEMS : 1];
+  __shared__ int64_t s_split[2];
+  const int64_t out0 = int64_t(blockIdx.x) * TILE;
+  const int64_t ps = (out0 / (2 * width)) * (2 * width);
+  int64_t na = n - ps;
+  na = na < 0 ? 0 : (na > width ? width : na);
+  int64_t nb = n - ps - width;
+  nb = nb < 0 ? 0 : (nb > width ? width : nb);
+  const K* a = src_k + ps;
+  const K* b = src_k + ps + width;
+  const int64_t d0 = out0 - ps;
+  const int64_t d1 = d0 + TILE < na + nb ? d0 + TILE : na + nb;
+  tile_splits<CMP>(a, na, b, nb, d0, d1, s_split);
+  const int64_t a0 = s_split[0], a1 = s_split[1];
+  const int64_t b0 = d0 - a0, b1 = d1 - a1;
+  merge_tile<CMP, K, V, HAS_V, BLOCK, ITEMS>(
+      a + a0, HAS_V ? src_v + ps + a0 : nullptr, int(a1 - a0), b + b0,
+      HAS_V ? src_v + ps + width + b0 : nullptr, int(b1 - b0), dst_k + out0,
+      HAS_V ? dst_v + out0 : nullptr, s_k, s_v);
+}
+
+inline int sort_num_passes(int64_t tiles) {
+  int p = 0;
+  while ((int64_t(1) << p) < tiles) ++p;
+  return p;
+}
+
+template <class CMP, class K, class V, bool HAS_V>
+int sort_launch(K* keys, V* vals, int64_t n, char* ws, hipStream_t stream) {
+  using C = SortConfig<K>;
+  const int64_t tiles = (n + C::TILE - 1) / C::TILE;
+  const int passes = sort_num_passes(tiles);
+  K* tk = reinterpret_cast<K*>(ws);
+  V* tv = HAS_V ? reinterpret_cast<V*>(ws + align_up(size_t(n) * sizeof(K), 256)) : nullptr;
+  // An odd pass count starts from the workspace so the last pass ends in place.
+  K* ck = (passes & 1) ? tk : keys;
+  V* cv = (passes & 1) ? tv : vals;
+  K* ok = (passes & 1) ? keys : tk;
+  V* ov = (passes & 1) ? vals : tv;
+  block_sort_kernel<CMP, K, V, HAS_V, C::BLOCK, C::ITEMS><<<dim3(uint32_t(tiles)), C::BLOCK, 0, stream>>>(
+      keys, vals, n, ck, cv);
+  BULKX_LAUNCHED();
+  int64_t width = C::TILE;
+  for (int p = 0; p < passes; ++p, width *= 2) {
+    merge_pass_kernel<CMP, K, V, HAS_V, C::BLOCK, C::ITEMS>
+        <<<dim3(uint32_t(tiles)), C::BLOCK, 0, stream>>>(ck, cv, n, width, ok, ov);
+    BULKX_LAUNCHED();
+    K* t1 = ck;
+    ck = ok;
+    ok = t1;
+    V* t2 = cv;
+    cv = ov;
+    ov = t2;
+  }
+  return BULKX_SUCCESS;
+}
+
+template <class K>
+int sort_dispatch_key(int vdtype, int comp, void* keys, void* vals, int64_t n, char* ws,
+                      hipStream_t stream) {
+  int st = BULKX_SUCCESS;
+  BULKX_DISPATCH_CMP(comp, CMP, {
+    if (vdtype < 0)
+      st = sort_launch<CMP, K, uint32_t, false>(static_cast<K*>(keys), nullptr, n, ws, stream);
+    else if (dtype_size(vdtype) == 4)
+      st = sort_launch<CMP, K, uint32_t, true>(static_cast<K*>(keys), static_cast<uint32_t*>(vals),
+                                               n, ws, stream);
+    else
+      st = sort_launch<CMP, K, uint64_t, true>(static_cast<K*>(keys), static_cast<uint64_t*>(vals),
+                                               n, ws, stream);
+  });
+  return st;
+}
+
+int sort_dispatch_k4(int kdtype, int vdtype, int comp, void* keys, void* vals, int64_t n,
+                     char* ws, hipStream_t stream);
+int sort_dispatch_k8(int kdtype, int vdtype, int comp, void* keys, void* vals, int64_t n,
+                     char* ws, hipStream_t stream);
+
+}  // namespace bulkx
