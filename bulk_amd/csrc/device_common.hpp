@@ -90,27 +90,52 @@ struct OpPlusDD {
 // Accumulator policy: which type and operator carry partial results between
 // threads and tiles.  float sums accumulate in double, double sums in
 // double-double; everything else (integers, min, max) stays in T and is exact.
+//
+// Two levels: `row` (type row_t, operator row_op) is used for the short
+// wavefront-level scans (depth ~ log2 of a wave chunk), `type` / `op` for
+// the carries from row to row, across waves, tiles and the look-back chain,
+// whose depth grows with n.  lift() moves a row value to the carry level and
+// finish(p, r) returns the element value p op r in T.
 template <class T, class Op>
 struct Acc {
   using type = T;
   using op = Op;
+  using row_t = T;
+  using row_op = Op;
   __host__ __device__ __forceinline__ static type from(T x) { return x; }
   __host__ __device__ __forceinline__ static T to(type a) { return a; }
+  __host__ __device__ __forceinline__ static row_t row(T x) { return x; }
+  __host__ __device__ __forceinline__ static type lift(row_t r) { return r; }
+  __host__ __device__ __forceinline__ static T finish(type p, row_t r) { return Op::apply(p, r); }
 };
 template <>
 struct Acc<float, OpPlus> {
   using type = double;
   using op = OpPlus;
+  using row_t = double;
+  using row_op = OpPlus;
   __host__ __device__ __forceinline__ static double from(float x) { return double(x); }
   __host__ __device__ __forceinline__ static float to(double a) { return float(a); }
+  __host__ __device__ __forceinline__ static double row(float x) { return double(x); }
+  __host__ __device__ __forceinline__ static double lift(double r) { return r; }
+  __host__ __device__ __forceinline__ static float finish(double p, double r) { return float(p + r); }
 };
 template <>
 struct Acc<double, OpPlus> {
   using type = dd;
   using op = OpPlusDD;
+  using row_t = double;
+  using row_op = OpPlus;
   __host__ __device__ __forceinline__ static dd from(double x) { return dd{x, 0.0}; }
   __host__ __device__ __forceinline__ static double to(dd a) {
     return __builtin_isfinite(a.hi) ? a.hi + a.lo : a.hi;
+  }
+  __host__ __device__ __forceinline__ static double row(double x) { return x; }
+  __host__ __device__ __forceinline__ static dd lift(double r) { return dd{r, 0.0}; }
+  // p.hi + (p.lo + r): one rounding of the low part and one of the result.
+  __host__ __device__ __forceinline__ static double finish(dd p, double r) {
+#pragma clang fp contract(off)
+    return __builtin_isfinite(p.hi) ? p.hi + (p.lo + r) : p.hi + r;
   }
 };
 
@@ -304,7 +329,7 @@ struct TileStatus {
 // One window costs one memory round trip, so the inclusive-prefix front
 // advances 64*Q tiles per round trip.  Returns the exclusive prefix of
 // `tile` (tile >= 1).
-template <class Op, int Q = 4, class T>
+template <class Op, int Q = 1, class T>
 __device__ T lookback_exclusive_prefix(const TileStatus<T>& st, int64_t tile, uint32_t* err) {
   constexpr int WIN = kWave * Q;
   const int lane = lane_id();
@@ -420,6 +445,91 @@ __device__ __forceinline__ bool aligned16(const void* p) {
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
+}
+
+// ---------------------------------------------------------------------------
+// Striped register tiles.  A wavefront owns a contiguous chunk of 64*ITEMS
+// elements viewed as NV rows of 64 16-byte vectors; lane l holds vector l of
+// every row (x[j*VE + q] = chunk[(j*64 + l)*VE + q]).  Each load or store
+// instruction of a wave therefore covers 1 KB of consecutive bytes, and no
+// LDS is involved.  Scans over this layout combine a lane-local scan of each
+// vector, a DPP scan across the row and a carry from row to row.
+// ---------------------------------------------------------------------------
+template <class T, int ITEMS, int VE_ = 16 / int(sizeof(T))>
+struct Striped {
+  static constexpr int VE = VE_;  // elements per lane per row
+  static constexpr int NV = ITEMS / VE;
+  static constexpr int N = kWave * ITEMS;
+  static_assert(ITEMS % VE == 0, "ITEMS must fill whole rows");
+  static_assert(VE * sizeof(T) == 4 || VE * sizeof(T) == 8 || VE * sizeof(T) == 16,
+                "a lane's row slice must be 4, 8 or 16 bytes");
+  // chunk offset of x[i] for lane l
+  __device__ __forceinline__ static int offset(int i, int lane) {
+    return ((i / VE) * kWave + lane) * VE + (i % VE);
+  }
+};
+
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+template <int BYTES>
+struct RawVec;
+template <>
+struct RawVec<4> { using type = uint32_t; };
+template <>
+struct RawVec<8> { using type = u32x2; };
+template <>
+struct RawVec<16> { using type = u32x4; };
+
+// NT = nontemporal loads (streams read exactly once).
+template <class T, int ITEMS, int VE = 16 / int(sizeof(T)), bool NT = false>
+__device__ __forceinline__ void striped_load(const T* chunk, int count, T (&x)[ITEMS], T fill) {
+  using S = Striped<T, ITEMS, VE>;
+  using RV = typename RawVec<VE * sizeof(T)>::type;
+  const int lane = lane_id();
+  if (count >= S::N && (reinterpret_cast<uintptr_t>(chunk) % (VE * sizeof(T))) == 0) {
+    const RV* q = reinterpret_cast<const RV*>(chunk);
+    RV t[S::NV];
+#pragma unroll
+    for (int j = 0; j < S::NV; ++j) {
+      if constexpr (NT) t[j] = __builtin_nontemporal_load(q + j * kWave + lane);
+      else t[j] = q[j * kWave + lane];
+    }
+#pragma unroll
+    for (int j = 0; j < S::NV; ++j) {
+      const T* e = reinterpret_cast<const T*>(&t[j]);
+#pragma unroll
+      for (int k = 0; k < S::VE; ++k) x[j * S::VE + k] = e[k];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const int o = S::offset(i, lane);
+      x[i] = o < count ? chunk[o] : fill;
+    }
+  }
+}
+
+template <class T, int ITEMS, int VE = 16 / int(sizeof(T))>
+__device__ __forceinline__ void striped_store(T* chunk, int count, const T (&x)[ITEMS]) {
+  using S = Striped<T, ITEMS, VE>;
+  using RV = typename RawVec<VE * sizeof(T)>::type;
+  const int lane = lane_id();
+  if (count >= S::N && (reinterpret_cast<uintptr_t>(chunk) % (VE * sizeof(T))) == 0) {
+    RV* q = reinterpret_cast<RV*>(chunk);
+#pragma unroll
+    for (int j = 0; j < S::NV; ++j) {
+      RV t;
+      T* e = reinterpret_cast<T*>(&t);
+#pragma unroll
+      for (int k = 0; k < S::VE; ++k) e[k] = x[j * S::VE + k];
+      q[j * kWave + lane] = t;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const int o = S::offset(i, lane);
+      if (o < count) chunk[o] = x[i];
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------

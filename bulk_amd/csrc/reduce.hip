@@ -47,8 +47,10 @@ __global__ __launch_bounds__(BLOCK) void reduce_partials_kernel(
   int64_t i = int64_t(blockIdx.x) * BLOCK + threadIdx.x;
   for (; i + (UNROLL - 1) * stride < nvec; i += UNROLL * stride) {
     u32x4 t[UNROLL];
+    // Nontemporal loads: the stream is read once, so it should not displace
+    // anything in L2 / MALL (read-only streams reach ~87% of HBM peak this way).
 #pragma unroll
-    for (int u = 0; u < UNROLL; ++u) t[u] = vin[i + u * stride];
+    for (int u = 0; u < UNROLL; ++u) t[u] = __builtin_nontemporal_load(vin + i + u * stride);
 #pragma unroll
     for (int u = 0; u < UNROLL; ++u) {
       const T* e = reinterpret_cast<const T*>(&t[u]);
@@ -60,7 +62,7 @@ __global__ __launch_bounds__(BLOCK) void reduce_partials_kernel(
     }
   }
   for (; i < nvec; i += stride) {
-    u32x4 t = vin[i];
+    u32x4 t = __builtin_nontemporal_load(vin + i);
     const T* e = reinterpret_cast<const T*>(&t);
     T s = e[0];
 #pragma unroll

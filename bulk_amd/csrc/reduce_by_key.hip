@@ -20,8 +20,7 @@ __global__ void rbk_prepare_kernel(uint32_t* header, uint64_t* words, int64_t nu
 size_t reduce_by_key_workspace_bytes(int kdtype, int vdtype, int64_t n) {
   size_t ks = dtype_size(kdtype), vs = dtype_size(vdtype);
   if (ks == 0 || vs == 0 || n <= 0) return 0;
-  const int items = (ks + vs <= 8) ? 16 : 8;
-  const int64_t tile = 256 * items;
+  const int64_t tile = 512 * 16;
   const int64_t tiles = (n + tile - 1) / tile;
   if (tiles <= 1) return 0;
   return kRbkHeaderBytes + size_t(tiles) * kRbkMaxStatusWords * sizeof(uint64_t);

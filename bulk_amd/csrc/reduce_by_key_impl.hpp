@@ -45,16 +45,61 @@ struct SegOp {
   }
 };
 
+// Row-level pair for the short wavefront scans: 32-bit head count (a row
+// holds at most 64*VE elements) and the value in its own type.  It is
+// lifted to Seg<A> (64-bit count, carry accumulator) for row, wave, tile and
+// look-back carries.
+template <class V>
+struct RowSeg {
+  uint32_t c;
+  V v;
+};
+
+template <class Op, class V>
+struct RowSegOp {
+  __device__ __forceinline__ static RowSeg<V> apply(RowSeg<V> a, RowSeg<V> b) {
+    RowSeg<V> r;
+    r.c = a.c + b.c;
+    r.v = b.c ? b.v : Op::apply(a.v, b.v);
+    return r;
+  }
+  template <class X>
+  __device__ __forceinline__ static RowSeg<V> identity() {
+    RowSeg<V> r;
+    r.c = 0;
+    r.v = Op::template identity<V>();
+    return r;
+  }
+};
+
 template <class K, class V>
 struct RbkConfig {
-  static constexpr int BLOCK = 256;
-  static constexpr int ITEMS = (sizeof(K) + sizeof(V) <= 8) ? 16 : 8;
+  static constexpr int BLOCK = 512;
+  static constexpr int ITEMS = 16;
   static constexpr int TILE = BLOCK * ITEMS;
 };
 
 constexpr size_t kRbkHeaderBytes = 256;
 constexpr int kRbkMaxStatusWords = 6;  // Seg<dd> = 24 bytes
 
+// Per-wave LDS region: the padded transposes of keys and values, reused as
+// the stage of the wave's outputs.
+template <class K, class V, int ITEMS>
+struct RbkLds {
+  static constexpr int KBYTES = (WaveTile<K, ITEMS>::LDS_ELEMS * int(sizeof(K)) + 15) / 16 * 16;
+  static constexpr int VBYTES = WaveTile<V, ITEMS>::LDS_ELEMS * int(sizeof(V));
+  static constexpr int BYTES = KBYTES + VBYTES;
+  static_assert(kWave * ITEMS * int(sizeof(K)) <= KBYTES, "key stage fits");
+};
+
+// Layout: a wavefront owns a contiguous chunk of 64*ITEMS elements, loaded
+// with coalesced 16-byte accesses and transposed through LDS so that lane l
+// holds the ITEMS consecutive elements [l*ITEMS, (l+1)*ITEMS).  Flags, the
+// lane's run aggregate and the final walk are sequential per lane (the
+// bounded-agent body of reduce_by_key.hpp:159-208); only one DPP scan per
+// wave chunk crosses lanes.  The chunk's tails occupy consecutive output
+// slots, so they are staged in the wave's LDS region and written with
+// coalesced stores.
 template <class K, class V, class Op, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void reduce_by_key_kernel(
     const K* __restrict__ keys, const V* __restrict__ vals, int64_t n, K* __restrict__ kout,
@@ -65,23 +110,19 @@ __global__ __launch_bounds__(BLOCK) void reduce_by_key_kernel(
   using AOp = typename AccP::op;
   using S = Seg<A>;
   using SOp = SegOp<AOp, A>;
-  constexpr int TILE = BLOCK * ITEMS;
-  constexpr int NW = BLOCK / kWave;
+  using RS = RowSeg<V>;
+  using RSOp = RowSegOp<Op, V>;
   using WK = WaveTile<K, ITEMS>;
   using WV = WaveTile<V, ITEMS>;
-  static_assert(NW * WK::LDS_ELEMS >= TILE && NW * WV::LDS_ELEMS >= TILE, "staging fits");
+  using L = RbkLds<K, V, ITEMS>;
+  constexpr int TILE = BLOCK * ITEMS;
+  constexpr int NW = BLOCK / kWave;
+  constexpr int N = WK::N;
 
   __shared__ S s_wave[NW];
   __shared__ S s_prefix;
   __shared__ uint32_t s_tile;
-  __shared__ uint32_t s_count;
-  __shared__ uint32_t s_first_head;
-  // Per-wave transpose regions; after the block scan they are reused as the
-  // tile's compaction stage (s_k / s_v).
-  __shared__ K s_trk[NW][WK::LDS_ELEMS];
-  __shared__ V s_trv[NW][WV::LDS_ELEMS];
-  K* s_k = &s_trk[0][0];
-  V* s_v = &s_trv[0][0];
+  __shared__ __align__(16) char s_region[NW][L::BYTES];
   const TileStatus<S> st{words};
 
   int64_t tile;
@@ -94,59 +135,108 @@ __global__ __launch_bounds__(BLOCK) void reduce_by_key_kernel(
   }
   const int wid = threadIdx.x / kWave;
   const int lane = lane_id();
-  const int64_t chunk_base = tile * TILE + int64_t(wid) * WK::N;
+  const int64_t chunk_base = tile * TILE + int64_t(wid) * N;
   const int64_t rem = n - chunk_base;
-  const int count = rem <= 0 ? 0 : (rem >= WK::N ? WK::N : int(rem));
-  const int64_t my_base = chunk_base + int64_t(lane) * ITEMS;
-  const int64_t valid = count - lane * ITEMS;
+  const int count = rem <= 0 ? 0 : (rem >= N ? N : int(rem));
+  const int64_t cb = count > 0 ? chunk_base : 0;
+  K* lk = reinterpret_cast<K*>(s_region[wid]);
+  V* lv = reinterpret_cast<V*>(s_region[wid] + L::KBYTES);
 
+  // Coalesced loads, transposed to ITEMS consecutive elements per lane.
   K k[ITEMS];
   V v[ITEMS];
-  const int64_t cb = count > 0 ? chunk_base : 0;
-  wave_load_blocked<K, ITEMS>(keys + cb, count, k, K(0), s_trk[wid]);
-  wave_load_blocked<V, ITEMS>(vals + cb, count, v, V(0), s_trv[wid]);
-  // Neighbouring keys: inside the wave by DPP lane shifts, across wave
-  // chunks from global memory (one load per wave edge).
-  K gprev = K(0), gnext = K(0);
-  if (lane == 0 && count > 0 && chunk_base > 0) gprev = keys[chunk_base - 1];
-  if (lane == kWave - 1 && chunk_base + WK::N < n) gnext = keys[chunk_base + WK::N];
-  const K kprev = dpp_move<kWaveShr1>(gprev, k[ITEMS - 1]);
-  const K knext = dpp_move<kWaveShl1>(gnext, k[0]);
-
-  // Head / tail flags of the owned elements (head_flags.hpp, tail_flags.hpp).
-  uint32_t heads = 0, tails = 0;
+  {
+    const u32x4* q = reinterpret_cast<const u32x4*>(keys + cb);
+    if (count >= N && aligned16(keys + cb)) {
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i) {
-    if (i < valid) {
-      const int64_t g = my_base + i;
-      const K before = i ? k[i - 1] : kprev;
-      const K after = (i + 1 < ITEMS) ? k[i + 1] : knext;
-      if (g == 0 || !(k[i] == before)) heads |= 1u << i;
-      if (g == n - 1 || !(k[i] == after)) tails |= 1u << i;
-    }
-  }
-  if (threadIdx.x == 0) {
-    s_count = 0;
-    s_first_head = heads & 1u;
-  }
-
-  // Thread aggregate in the (count, value) monoid.
-  S agg = SOp::template identity<S>();
+      for (int j = 0; j < WK::NV; ++j) {
+        const u32x4 t = __builtin_nontemporal_load(q + j * kWave + lane);
+        const K* e = reinterpret_cast<const K*>(&t);
+        const int base = WK::pos((j * kWave + lane) * WK::VE);
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i) {
-    if (i < valid) {
-      const A x = AccP::from(v[i]);
-      if (heads & (1u << i)) {
-        agg.c += 1;
-        agg.v = x;
-      } else {
-        agg.v = AOp::apply(agg.v, x);
+        for (int c = 0; c < WK::VE; ++c) lk[base + c] = e[c];
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < ITEMS; ++j) {
+        const int e = j * kWave + lane;
+        lk[WK::pos(e)] = e < count ? keys[cb + e] : K(0);
       }
     }
   }
+  {
+    const u32x4* q = reinterpret_cast<const u32x4*>(vals + cb);
+    if (count >= N && aligned16(vals + cb)) {
+#pragma unroll
+      for (int j = 0; j < WV::NV; ++j) {
+        const u32x4 t = __builtin_nontemporal_load(q + j * kWave + lane);
+        const V* e = reinterpret_cast<const V*>(&t);
+        const int base = WV::pos((j * kWave + lane) * WV::VE);
+#pragma unroll
+        for (int c = 0; c < WV::VE; ++c) lv[base + c] = e[c];
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < ITEMS; ++j) {
+        const int e = j * kWave + lane;
+        lv[WV::pos(e)] = e < count ? vals[cb + e] : V(0);
+      }
+    }
+  }
+  K gprev = K(0), gnext = K(0);
+  if (lane == 0 && count > 0 && chunk_base > 0) gprev = keys[chunk_base - 1];
+  if (lane == kWave - 1 && count >= N && chunk_base + N < n) gnext = keys[chunk_base + N];
+  wave_lds_sync();
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) k[i] = lk[WK::pos(lane * ITEMS + i)];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) v[i] = lv[WV::pos(lane * ITEMS + i)];
+  const K kprev = lane > 0 ? lk[WK::pos(lane * ITEMS - 1)] : gprev;
+  const K knext = lane < kWave - 1 ? lk[WK::pos(lane * ITEMS + ITEMS)] : gnext;
+  wave_lds_sync();  // the region is free again (used as the output stage below)
 
-  S tile_agg;
-  const S thread_prefix = block_exclusive_scan<BLOCK, SOp>(agg, s_wave, &tile_agg);
+  // Head / tail flags (head_flags.hpp, tail_flags.hpp): one key comparison
+  // per element; a tail is the element before a head.
+  int valid = count - lane * ITEMS;
+  valid = valid < 0 ? 0 : (valid > ITEMS ? ITEMS : valid);
+  const uint32_t vmask = valid >= 32 ? ~0u : ((1u << valid) - 1);
+  uint32_t h = 0;
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) h |= uint32_t(!(k[i] == (i ? k[i - 1] : kprev))) << i;
+  uint32_t t = (h >> 1) | (uint32_t(!(k[ITEMS - 1] == knext)) << (ITEMS - 1));
+  if (chunk_base == 0 && lane == 0) h |= 1u;  // the array's first element
+  if (valid > 0 && valid < ITEMS) t |= 1u << (valid - 1);  // last element of a partial chunk
+  if (valid == ITEMS && count == N && chunk_base + N == n && lane == kWave - 1)
+    t |= 1u << (ITEMS - 1);  // last element of a full last chunk
+  h &= vmask;
+  t &= vmask;
+
+  // Lane aggregate (count of heads, value since the last head), DPP scan
+  // across the wave, LDS combine across waves.  Values past the end of the
+  // array only extend the last run's aggregate, which nothing reads.
+  RS a;
+  a.c = uint32_t(__builtin_popcount(h));
+  a.v = Op::template identity<V>();
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) a.v = (h >> i) & 1u ? v[i] : Op::apply(a.v, v[i]);
+  const RS inc = wave_inclusive_scan<RSOp>(a);
+  const RS lane_prefix = wave_shift_right<RSOp>(inc);
+  const RS wave_total = readlane(inc, kWave - 1);
+  auto lift = [](RS r) {
+    S x;
+    x.c = r.c;
+    x.v = AccP::from(r.v);
+    return x;
+  };
+  if (lane == 0) s_wave[wid] = lift(wave_total);
+  __syncthreads();
+  S wave_prefix = SOp::template identity<S>(), tile_agg = SOp::template identity<S>();
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    const S x = s_wave[w];
+    if (w < wid) wave_prefix = SOp::apply(wave_prefix, x);
+    tile_agg = SOp::apply(tile_agg, x);
+  }
 
   S tile_prefix = SOp::template identity<S>();
   if (!single_tile) {
@@ -165,44 +255,43 @@ __global__ __launch_bounds__(BLOCK) void reduce_by_key_kernel(
       tile_prefix = s_prefix;
     }
   }
+  // The tile holding element n-1 reports the run count.
+  if (threadIdx.x == 0 && tile * TILE + TILE >= n)
+    *num_runs = int64_t(tile_prefix.c + tile_agg.c);
 
-  // Output slots of this tile start at `base`: the run open at the tile's
-  // first element (its head may lie in an earlier tile) gets slot base.
-  const uint64_t base = tile_prefix.c - (s_first_head ? 0 : 1);
-  S p = SOp::apply(tile_prefix, thread_prefix);
-  uint64_t c = p.c;
-  A s = p.v;
-  uint32_t my_count = 0;
+  // Walk: a tail's run has slot (heads so far) - 1.  The chunk's tails hold
+  // consecutive slots from base (the run open at the chunk's first element),
+  // so slot - base < 64*ITEMS is a 32-bit stage index.
+  const S p = SOp::apply(SOp::apply(tile_prefix, wave_prefix), lift(lane_prefix));
+  const uint64_t base = readlane(p.c + (h & 1u) - 1, 0);
+  const uint32_t ntails = wave_reduce<OpPlus>(uint32_t(__builtin_popcount(t)));
+  K* sk = lk;
+  V* sv = lv;
+  uint32_t e = uint32_t(p.c - base);
+  V local = Op::template identity<V>();
+  bool started = false;  // a run began inside this lane's slice
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
-    if (i < valid) {
-      const A x = AccP::from(v[i]);
-      if (heads & (1u << i)) {
-        c += 1;
-        s = x;
-      } else {
-        s = AOp::apply(s, x);
-      }
-      if (tails & (1u << i)) {
-        const uint32_t slot = uint32_t(c - 1 - base);
-        if (slot < uint32_t(TILE)) {
-          s_k[slot] = k[i];
-          s_v[slot] = AccP::to(s);
-          my_count = slot + 1;
-        }
+    const bool hi = (h >> i) & 1u;
+    e += hi;
+    local = hi ? v[i] : Op::apply(local, v[i]);
+    started |= hi;
+    if ((t >> i) & 1u) {
+      const uint32_t slot = e - 1;
+      if (slot < uint32_t(N)) {
+        sk[slot] = k[i];
+        sv[slot] = started ? local : AccP::to(AOp::apply(p.v, AccP::from(local)));
       }
     }
   }
-  if (my_count) atomicMax(&s_count, my_count);
-  if (my_base < n && n - 1 < my_base + ITEMS) *num_runs = int64_t(c);  // owner of element n-1
-  __syncthreads();
-  // Bounds guard: a tile never writes past TILE slots nor past n outputs.
-  uint64_t cnt = s_count < uint32_t(TILE) ? s_count : uint32_t(TILE);
+  wave_lds_sync();
+  // bounds guard: slots stay below n when the scan is consistent
+  uint32_t cnt = ntails < uint32_t(N) ? ntails : uint32_t(N);
   if (base >= uint64_t(n)) cnt = 0;
-  else if (base + cnt > uint64_t(n)) cnt = uint64_t(n) - base;
-  for (uint32_t j = threadIdx.x; j < cnt; j += BLOCK) {
-    kout[base + j] = s_k[j];
-    vout[base + j] = s_v[j];
+  else if (base + cnt > uint64_t(n)) cnt = uint32_t(uint64_t(n) - base);
+  for (uint32_t x = lane; x < cnt; x += kWave) {
+    kout[base + x] = sk[x];
+    vout[base + x] = sv[x];
   }
 }
 

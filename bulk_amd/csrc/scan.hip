@@ -35,12 +35,12 @@ __global__ __launch_bounds__(BLOCK) void scan_kernel(const T* in, T* out, int64_
   using AccP = Acc<T, Op>;
   using A = typename AccP::type;
   using AOp = typename AccP::op;
+  using SL = Striped<T, ITEMS>;
   constexpr int TILE = BLOCK * ITEMS;
-  using WT = WaveTile<T, ITEMS>;
-  __shared__ A s_wave[BLOCK / kWave];
+  constexpr int NW = BLOCK / kWave;
+  __shared__ A s_wave[NW];
   __shared__ A s_prefix;
   __shared__ uint32_t s_tile;
-  __shared__ T s_tr[BLOCK / kWave][WT::LDS_ELEMS];
   const TileStatus<A> st{status_words};
 
   int64_t tile;
@@ -52,23 +52,48 @@ __global__ __launch_bounds__(BLOCK) void scan_kernel(const T* in, T* out, int64_
     tile = s_tile;
   }
   const int wid = threadIdx.x / kWave;
-  const int64_t chunk_base = tile * TILE + int64_t(wid) * WT::N;
+  const int64_t chunk_base = tile * TILE + int64_t(wid) * SL::N;
   const int64_t rem = n - chunk_base;
-  const int count = rem <= 0 ? 0 : (rem >= WT::N ? WT::N : int(rem));
+  const int count = rem <= 0 ? 0 : (rem >= SL::N ? SL::N : int(rem));
   const T id = Op::template identity<T>();
   const A aid = AOp::template identity<A>();
 
-  T v[ITEMS];
-  wave_load_blocked<T, ITEMS>(in + (count > 0 ? chunk_base : 0), count, v, id, s_tr[wid]);
+  T x[ITEMS];
+  striped_load<T, ITEMS>(in + (count > 0 ? chunk_base : 0), count, x, id);
 
-  // Thread-sequential inclusive scan of the owned elements (the bounded-agent
-  // body of bulk::inclusive_scan, scan.hpp:35-55).
+  // Lane-local inclusive scan of each 16-byte vector (the bounded-agent
+  // body of bulk::inclusive_scan, scan.hpp:35-55, over VE elements).
 #pragma unroll
-  for (int i = 1; i < ITEMS; ++i) v[i] = Op::apply(v[i - 1], v[i]);
+  for (int j = 0; j < SL::NV; ++j)
+#pragma unroll
+    for (int q = 1; q < SL::VE; ++q)
+      x[j * SL::VE + q] = Op::apply(x[j * SL::VE + q - 1], x[j * SL::VE + q]);
 
-  A tile_agg;
-  const A thread_prefix =
-      block_exclusive_scan<BLOCK, AOp>(AccP::from(v[ITEMS - 1]), s_wave, &tile_agg);
+  // DPP scan across each row (row accumulator), carried from row to row
+  // (carry accumulator).
+  using R = typename AccP::row_t;
+  using ROp = typename AccP::row_op;
+  R rowp[SL::NV];
+  R rowt[SL::NV];  // row totals (wave-uniform)
+  A carry = aid;
+#pragma unroll
+  for (int j = 0; j < SL::NV; ++j) {
+    const R inc = wave_inclusive_scan<ROp>(AccP::row(x[j * SL::VE + SL::VE - 1]));
+    rowp[j] = wave_shift_right<ROp>(inc);
+    rowt[j] = readlane(inc, kWave - 1);
+    carry = AOp::apply(carry, AccP::lift(rowt[j]));
+  }
+
+  // Combine the waves of the tile.
+  if (lane_id() == 0) s_wave[wid] = carry;
+  __syncthreads();
+  A wave_prefix = aid, tile_agg = aid;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    const A t = s_wave[w];
+    if (w < wid) wave_prefix = AOp::apply(wave_prefix, t);
+    tile_agg = AOp::apply(tile_agg, t);
+  }
 
   A tile_prefix;  // everything before this tile, init included
   if (tile == 0) {
@@ -88,16 +113,24 @@ __global__ __launch_bounds__(BLOCK) void scan_kernel(const T* in, T* out, int64_
     tile_prefix = s_prefix;
   }
 
-  const A p = AOp::apply(tile_prefix, thread_prefix);
-  if constexpr (EXCLUSIVE) {
+  A rc = AOp::apply(tile_prefix, wave_prefix);  // carry into row j
 #pragma unroll
-    for (int i = ITEMS - 1; i > 0; --i) v[i] = AccP::to(AOp::apply(p, AccP::from(v[i - 1])));
-    v[0] = AccP::to(p);
-  } else {
+  for (int j = 0; j < SL::NV; ++j) {
+    // prefix of this lane's vector in row j: carry level + row level
+    const A p = AOp::apply(rc, AccP::lift(rowp[j]));
+    rc = AOp::apply(rc, AccP::lift(rowt[j]));
+    if constexpr (EXCLUSIVE) {
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) v[i] = AccP::to(AOp::apply(p, AccP::from(v[i])));
+      for (int q = SL::VE - 1; q > 0; --q)
+        x[j * SL::VE + q] = AccP::finish(p, AccP::row(x[j * SL::VE + q - 1]));
+      x[j * SL::VE] = AccP::to(p);
+    } else {
+#pragma unroll
+      for (int q = 0; q < SL::VE; ++q)
+        x[j * SL::VE + q] = AccP::finish(p, AccP::row(x[j * SL::VE + q]));
+    }
   }
-  if (count > 0) wave_store_blocked<T, ITEMS>(out + chunk_base, count, v, s_tr[wid]);
+  if (count > 0) striped_store<T, ITEMS>(out + chunk_base, count, x);
 }
 
 template <class T>

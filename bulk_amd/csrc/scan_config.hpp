@@ -4,9 +4,11 @@
 
 namespace bulkx {
 
+// 1024-thread workgroups amortise one look-back over a 128 KB tile; the
+// shapes come from a block x items sweep on MI355X at 2^24 and 2^28 elements.
 template <class T>
 struct ScanConfig {
-  static constexpr int BLOCK = 256;
+  static constexpr int BLOCK = 1024;
   static constexpr int ITEMS = sizeof(T) == 4 ? 32 : 16;
 };
 
