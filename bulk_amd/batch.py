@@ -1,0 +1,97 @@
+"""Python front end of the batch executor (bulkx_batch_* in include/bulkx.h).
+
+A job list is a sequence of ``Job`` records over device tensors; ``Batch``
+runs the whole list as one persistent launch (plus one per sort merge
+round), recorded into a hipGraph on first use, or as one plain call per job
+(the per-call baseline).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import torch
+
+from . import _capi as C
+from .api import dtype_code
+
+
+@dataclass
+class Job:
+    kind: int
+    dtype: int
+    value_dtype: int = -1
+    n0: int = 0
+    n1: int = 0
+    tensors: dict = field(default_factory=dict)  # name -> tensor (kept alive)
+
+    def to_c(self) -> C.Job:
+        t = self.tensors
+
+        def p(name):
+            x = t.get(name)
+            return ctypes.c_void_p(x.data_ptr()) if x is not None else ctypes.c_void_p(0)
+
+        return C.Job(self.kind, self.dtype, self.value_dtype, 0, self.n0, self.n1,
+                     p("in0"), p("in1"), p("in2"), p("in3"), p("out0"), p("out1"), p("out2"))
+
+    @property
+    def elements(self) -> int:
+        return self.n0 + (self.n1 if self.kind == C.JOB_MERGE_BY_KEY else 0)
+
+
+def scan_job(x: torch.Tensor, out: torch.Tensor) -> Job:
+    return Job(C.JOB_INCLUSIVE_SCAN, dtype_code(x), -1, x.numel(), 0, {"in0": x, "out0": out})
+
+
+def reduce_job(x: torch.Tensor, out: torch.Tensor) -> Job:
+    return Job(C.JOB_REDUCE, dtype_code(x), -1, x.numel(), 0, {"in0": x, "out0": out})
+
+
+def reduce_by_key_job(keys, vals, kout, vout, count) -> Job:
+    return Job(C.JOB_REDUCE_BY_KEY, dtype_code(keys), dtype_code(vals), keys.numel(), 0,
+               {"in0": keys, "in1": vals, "out0": kout, "out1": vout, "out2": count})
+
+
+def merge_by_key_job(k1, v1, k2, v2, kout, vout) -> Job:
+    return Job(C.JOB_MERGE_BY_KEY, dtype_code(k1), dtype_code(v1), k1.numel(), k2.numel(),
+               {"in0": k1, "in1": v1, "in2": k2, "in3": v2, "out0": kout, "out1": vout})
+
+
+def sort_by_key_job(keys, vals) -> Job:
+    return Job(C.JOB_SORT_BY_KEY, dtype_code(keys), dtype_code(vals), keys.numel(), 0,
+               {"out0": keys, "out1": vals})
+
+
+class Batch:
+    """bulkx_batch: create once with capacities, set a job list, launch."""
+
+    def __init__(self, max_jobs: int, max_elements: int, stream: torch.cuda.Stream | None = None):
+        self.stream = stream if stream is not None else torch.cuda.current_stream()
+        self._h = ctypes.c_void_p()
+        C.check(C.lib().bulkx_batch_create(int(max_jobs), int(max_elements),
+                                           ctypes.c_void_p(self.stream.cuda_stream),
+                                           ctypes.byref(self._h)), "bulkx_batch_create")
+        self._jobs: list[Job] = []
+
+    def set_jobs(self, jobs: list[Job]) -> None:
+        arr = (C.Job * max(len(jobs), 1))(*[j.to_c() for j in jobs])
+        C.check(C.lib().bulkx_batch_set_jobs(self._h, arr, len(jobs)), "bulkx_batch_set_jobs")
+        self._jobs = list(jobs)
+
+    def launch(self, use_graph: bool = True) -> None:
+        C.check(C.lib().bulkx_batch_launch(self._h, 1 if use_graph else 0), "bulkx_batch_launch")
+
+    def launch_per_call(self) -> None:
+        C.check(C.lib().bulkx_batch_launch_per_call(self._h), "bulkx_batch_launch_per_call")
+
+    def close(self) -> None:
+        if self._h:
+            C.lib().bulkx_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):  # pragma: no cover - best effort
+        try:
+            self.close()
+        except Exception:
+            pass

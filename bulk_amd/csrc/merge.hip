@@ -12,19 +12,8 @@ __global__ __launch_bounds__(BLOCK) void merge_kernel(const K* __restrict__ a,
                                                       const K* __restrict__ b,
                                                       const V* __restrict__ vb, int64_t n2,
                                                       K* __restrict__ out, V* __restrict__ vout) {
-  constexpr int TILE = BLOCK * ITEMS;
-  __shared__ K s_k[MergeLds<K, BLOCK, ITEMS>::ELEMS];
-  __shared__ V s_v[HAS_V ? MergeLds<V, BLOCK, ITEMS>::ELEMS : 1];
-  __shared__ int64_t s_split[2];
-  const int64_t n = n1 + n2;
-  const int64_t d0 = int64_t(blockIdx.x) * TILE;
-  const int64_t d1 = d0 + TILE < n ? d0 + TILE : n;
-  tile_splits<CMP>(a, n1, b, n2, d0, d1, s_split);
-  const int64_t a0 = s_split[0], a1 = s_split[1];
-  const int64_t b0 = d0 - a0, b1 = d1 - a1;
-  merge_tile<CMP, K, V, HAS_V, BLOCK, ITEMS>(a + a0, HAS_V ? va + a0 : nullptr, int(a1 - a0),
-                                             b + b0, HAS_V ? vb + b0 : nullptr, int(b1 - b0),
-                                             out + d0, HAS_V ? vout + d0 : nullptr, s_k, s_v);
+  __shared__ MergeSmem<K, V, HAS_V, BLOCK, ITEMS> sm;
+  merge_two_tile<CMP, K, V, HAS_V, BLOCK, ITEMS>(sm, a, va, n1, b, vb, n2, out, vout, blockIdx.x);
 }
 
 template <class CMP, class K, class V, bool HAS_V>

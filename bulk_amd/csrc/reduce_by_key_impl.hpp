@@ -101,10 +101,20 @@ struct RbkLds {
 // slots, so they are staged in the wave's LDS region and written with
 // coalesced stores.
 template <class K, class V, class Op, int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void reduce_by_key_kernel(
-    const K* __restrict__ keys, const V* __restrict__ vals, int64_t n, K* __restrict__ kout,
-    V* __restrict__ vout, int64_t* __restrict__ num_runs, uint32_t* header, uint64_t* words,
-    bool single_tile) {
+struct RbkSmem {
+  using S = Seg<typename Acc<V, Op>::type>;
+  S wave[BLOCK / kWave];
+  S prefix;
+  __align__(16) char region[BLOCK / kWave][RbkLds<K, V, ITEMS>::BYTES];
+};
+
+// One tile of reduce_by_key (shared by the device-wide kernel and the batch
+// executor).
+template <class K, class V, class Op, int BLOCK, int ITEMS>
+__device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, const K* keys,
+                                         const V* vals, int64_t n, K* kout, V* vout,
+                                         int64_t* num_runs, int64_t tile, uint64_t* words,
+                                         uint32_t* err, bool single_tile) {
   using AccP = Acc<V, Op>;
   using A = typename AccP::type;
   using AOp = typename AccP::op;
@@ -118,29 +128,15 @@ __global__ __launch_bounds__(BLOCK) void reduce_by_key_kernel(
   constexpr int TILE = BLOCK * ITEMS;
   constexpr int NW = BLOCK / kWave;
   constexpr int N = WK::N;
-
-  __shared__ S s_wave[NW];
-  __shared__ S s_prefix;
-  __shared__ uint32_t s_tile;
-  __shared__ __align__(16) char s_region[NW][L::BYTES];
   const TileStatus<S> st{words};
-
-  int64_t tile;
-  if (single_tile) {
-    tile = 0;
-  } else {
-    if (threadIdx.x == 0) s_tile = atomicAdd(header, 1u);
-    __syncthreads();
-    tile = s_tile;
-  }
   const int wid = threadIdx.x / kWave;
   const int lane = lane_id();
   const int64_t chunk_base = tile * TILE + int64_t(wid) * N;
   const int64_t rem = n - chunk_base;
   const int count = rem <= 0 ? 0 : (rem >= N ? N : int(rem));
   const int64_t cb = count > 0 ? chunk_base : 0;
-  K* lk = reinterpret_cast<K*>(s_region[wid]);
-  V* lv = reinterpret_cast<V*>(s_region[wid] + L::KBYTES);
+  K* lk = reinterpret_cast<K*>(sm.region[wid]);
+  V* lv = reinterpret_cast<V*>(sm.region[wid] + L::KBYTES);
 
   // Coalesced loads, transposed to ITEMS consecutive elements per lane.
   K k[ITEMS];
@@ -228,12 +224,12 @@ __global__ __launch_bounds__(BLOCK) void reduce_by_key_kernel(
     x.v = AccP::from(r.v);
     return x;
   };
-  if (lane == 0) s_wave[wid] = lift(wave_total);
+  if (lane == 0) sm.wave[wid] = lift(wave_total);
   __syncthreads();
   S wave_prefix = SOp::template identity<S>(), tile_agg = SOp::template identity<S>();
 #pragma unroll
   for (int w = 0; w < NW; ++w) {
-    const S x = s_wave[w];
+    const S x = sm.wave[w];
     if (w < wid) wave_prefix = SOp::apply(wave_prefix, x);
     tile_agg = SOp::apply(tile_agg, x);
   }
@@ -245,14 +241,14 @@ __global__ __launch_bounds__(BLOCK) void reduce_by_key_kernel(
     } else {
       if (threadIdx.x == 0) st.publish(tile, kStatusAggregate, tile_agg);
       if (threadIdx.x < kWave) {
-        S excl = lookback_exclusive_prefix<SOp>(st, tile, header + 1);
+        S excl = lookback_exclusive_prefix<SOp>(st, tile, err);
         if (threadIdx.x == 0) {
           st.publish(tile, kStatusPrefix, SOp::apply(excl, tile_agg));
-          s_prefix = excl;
+          sm.prefix = excl;
         }
       }
       __syncthreads();
-      tile_prefix = s_prefix;
+      tile_prefix = sm.prefix;
     }
   }
   // The tile holding element n-1 reports the run count.
@@ -293,6 +289,24 @@ __global__ __launch_bounds__(BLOCK) void reduce_by_key_kernel(
     kout[base + x] = sk[x];
     vout[base + x] = sv[x];
   }
+}
+
+
+template <class K, class V, class Op, int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void reduce_by_key_kernel(
+    const K* __restrict__ keys, const V* __restrict__ vals, int64_t n, K* __restrict__ kout,
+    V* __restrict__ vout, int64_t* __restrict__ num_runs, uint32_t* header, uint64_t* words,
+    bool single_tile) {
+  __shared__ RbkSmem<K, V, Op, BLOCK, ITEMS> sm;
+  __shared__ uint32_t s_tile;
+  int64_t tile = 0;
+  if (!single_tile) {
+    if (threadIdx.x == 0) s_tile = atomicAdd(header, 1u);
+    __syncthreads();
+    tile = s_tile;
+  }
+  rbk_tile<K, V, Op, BLOCK, ITEMS>(sm, keys, vals, n, kout, vout, num_runs, tile, words,
+                                   header + 1, single_tile);
 }
 
 __global__ void rbk_empty_kernel(int64_t* num_runs);

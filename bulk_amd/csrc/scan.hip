@@ -15,6 +15,7 @@
 #include "device_common.hpp"
 #include "host_common.hpp"
 #include "scan_config.hpp"
+#include "scan_impl.hpp"
 
 namespace bulkx {
 
@@ -32,105 +33,16 @@ template <class T, class Op, int BLOCK, int ITEMS, bool EXCLUSIVE>
 __global__ __launch_bounds__(BLOCK) void scan_kernel(const T* in, T* out, int64_t n, T init,
                                                      bool has_init, uint32_t* header,
                                                      uint64_t* status_words, bool single_tile) {
-  using AccP = Acc<T, Op>;
-  using A = typename AccP::type;
-  using AOp = typename AccP::op;
-  using SL = Striped<T, ITEMS>;
-  constexpr int TILE = BLOCK * ITEMS;
-  constexpr int NW = BLOCK / kWave;
-  __shared__ A s_wave[NW];
-  __shared__ A s_prefix;
+  __shared__ ScanSmem<T, Op, BLOCK> sm;
   __shared__ uint32_t s_tile;
-  const TileStatus<A> st{status_words};
-
-  int64_t tile;
-  if (single_tile) {
-    tile = 0;
-  } else {
+  int64_t tile = 0;
+  if (!single_tile) {
     if (threadIdx.x == 0) s_tile = atomicAdd(header, 1u);
     __syncthreads();
     tile = s_tile;
   }
-  const int wid = threadIdx.x / kWave;
-  const int64_t chunk_base = tile * TILE + int64_t(wid) * SL::N;
-  const int64_t rem = n - chunk_base;
-  const int count = rem <= 0 ? 0 : (rem >= SL::N ? SL::N : int(rem));
-  const T id = Op::template identity<T>();
-  const A aid = AOp::template identity<A>();
-
-  T x[ITEMS];
-  striped_load<T, ITEMS>(in + (count > 0 ? chunk_base : 0), count, x, id);
-
-  // Lane-local inclusive scan of each 16-byte vector (the bounded-agent
-  // body of bulk::inclusive_scan, scan.hpp:35-55, over VE elements).
-#pragma unroll
-  for (int j = 0; j < SL::NV; ++j)
-#pragma unroll
-    for (int q = 1; q < SL::VE; ++q)
-      x[j * SL::VE + q] = Op::apply(x[j * SL::VE + q - 1], x[j * SL::VE + q]);
-
-  // DPP scan across each row (row accumulator), carried from row to row
-  // (carry accumulator).
-  using R = typename AccP::row_t;
-  using ROp = typename AccP::row_op;
-  R rowp[SL::NV];
-  R rowt[SL::NV];  // row totals (wave-uniform)
-  A carry = aid;
-#pragma unroll
-  for (int j = 0; j < SL::NV; ++j) {
-    const R inc = wave_inclusive_scan<ROp>(AccP::row(x[j * SL::VE + SL::VE - 1]));
-    rowp[j] = wave_shift_right<ROp>(inc);
-    rowt[j] = readlane(inc, kWave - 1);
-    carry = AOp::apply(carry, AccP::lift(rowt[j]));
-  }
-
-  // Combine the waves of the tile.
-  if (lane_id() == 0) s_wave[wid] = carry;
-  __syncthreads();
-  A wave_prefix = aid, tile_agg = aid;
-#pragma unroll
-  for (int w = 0; w < NW; ++w) {
-    const A t = s_wave[w];
-    if (w < wid) wave_prefix = AOp::apply(wave_prefix, t);
-    tile_agg = AOp::apply(tile_agg, t);
-  }
-
-  A tile_prefix;  // everything before this tile, init included
-  if (tile == 0) {
-    tile_prefix = has_init ? AccP::from(init) : aid;
-    if (!single_tile && threadIdx.x == 0)
-      st.publish(0, kStatusPrefix, AOp::apply(tile_prefix, tile_agg));
-  } else {
-    if (threadIdx.x == 0) st.publish(tile, kStatusAggregate, tile_agg);
-    if (threadIdx.x < kWave) {
-      A excl = lookback_exclusive_prefix<AOp>(st, tile, header + 1);
-      if (threadIdx.x == 0) {
-        st.publish(tile, kStatusPrefix, AOp::apply(excl, tile_agg));
-        s_prefix = excl;
-      }
-    }
-    __syncthreads();
-    tile_prefix = s_prefix;
-  }
-
-  A rc = AOp::apply(tile_prefix, wave_prefix);  // carry into row j
-#pragma unroll
-  for (int j = 0; j < SL::NV; ++j) {
-    // prefix of this lane's vector in row j: carry level + row level
-    const A p = AOp::apply(rc, AccP::lift(rowp[j]));
-    rc = AOp::apply(rc, AccP::lift(rowt[j]));
-    if constexpr (EXCLUSIVE) {
-#pragma unroll
-      for (int q = SL::VE - 1; q > 0; --q)
-        x[j * SL::VE + q] = AccP::finish(p, AccP::row(x[j * SL::VE + q - 1]));
-      x[j * SL::VE] = AccP::to(p);
-    } else {
-#pragma unroll
-      for (int q = 0; q < SL::VE; ++q)
-        x[j * SL::VE + q] = AccP::finish(p, AccP::row(x[j * SL::VE + q]));
-    }
-  }
-  if (count > 0) striped_store<T, ITEMS>(out + chunk_base, count, x);
+  scan_tile<T, Op, BLOCK, ITEMS, EXCLUSIVE>(sm, in, out, n, init, has_init, tile, status_words,
+                                            header + 1, single_tile, nullptr);
 }
 
 template <class T>

@@ -26,17 +26,26 @@ namespace bulkx {
 template <class K>
 using SortConfig = MergeConfig<K>;
 
+template <class K, class V, bool HAS_V, int BLOCK, int ITEMS>
+struct SortSmem {
+  K k[MergeLds<K, BLOCK, ITEMS>::ELEMS];
+  uint16_t i[BLOCK * ITEMS];
+  V v[HAS_V ? MergeLds<V, BLOCK, ITEMS>::ELEMS : 1];
+};
+
+// Block sort of tile `tile` of keys/vals[0, n) into kout/vout (may alias).
 template <class CMP, class K, class V, bool HAS_V, int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void block_sort_kernel(const K* keys, const V* vals, int64_t n,
-                                                           K* kout, V* vout) {
+__device__ __forceinline__ void block_sort_tile(SortSmem<K, V, HAS_V, BLOCK, ITEMS>& sm,
+                                                const K* keys, const V* vals, int64_t n, K* kout,
+                                                V* vout, int64_t tile) {
   constexpr int TILE = BLOCK * ITEMS;
   using WK = WaveTile<K, ITEMS>;
   using WV = WaveTile<V, ITEMS>;
-  __shared__ K s_k[MergeLds<K, BLOCK, ITEMS>::ELEMS];
-  __shared__ uint16_t s_i[TILE];
-  __shared__ V s_v[HAS_V ? MergeLds<V, BLOCK, ITEMS>::ELEMS : 1];
+  K* s_k = sm.k;
+  uint16_t* s_i = sm.i;
+  V* s_v = sm.v;
 
-  const int64_t base = int64_t(blockIdx.x) * TILE;
+  const int64_t base = tile * TILE;
   const int cnt = int(n - base < TILE ? n - base : TILE);
   const int mine = threadIdx.x * ITEMS;
   int vt = cnt - mine;  // valid items of this thread
@@ -119,34 +128,40 @@ __global__ __launch_bounds__(BLOCK) void block_sort_kernel(const K* keys, const 
   }
 }
 
-// One merge pass: pairs of sorted runs of width `width` (a multiple of TILE)
-// from (src_k, src_v) into (dst_k, dst_v).
 template <class CMP, class K, class V, bool HAS_V, int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void merge_pass_kernel(const K* __restrict__ src_k,
-                                                           const V* __restrict__ src_v, int64_t n,
-                                                           int64_t width, K* __restrict__ dst_k,
-                                                           V* __restrict__ dst_v) {
+__global__ __launch_bounds__(BLOCK) void block_sort_kernel(const K* keys, const V* vals, int64_t n,
+                                                           K* kout, V* vout) {
+  __shared__ SortSmem<K, V, HAS_V, BLOCK, ITEMS> sm;
+  block_sort_tile<CMP, K, V, HAS_V, BLOCK, ITEMS>(sm, keys, vals, n, kout, vout, blockIdx.x);
+}
+
+// One tile of a merge pass: pairs of sorted runs of width `width` (a
+// multiple of TILE) from (src_k, src_v) into (dst_k, dst_v).
+template <class CMP, class K, class V, bool HAS_V, int BLOCK, int ITEMS>
+__device__ __forceinline__ void merge_pass_tile(MergeSmem<K, V, HAS_V, BLOCK, ITEMS>& sm,
+                                                const K* src_k, const V* src_v, int64_t n,
+                                                int64_t width, K* dst_k, V* dst_v, int64_t tile) {
   constexpr int TILE = BLOCK * ITEMS;
-  __shared__ K s_k[MergeLds<K, BLOCK, ITEMS>::ELEMS];
-  __shared__ V s_v[HAS_V ? MergeLds<V, BLOCK, ITEMS>::ELEMS : 1];
-  __shared__ int64_t s_split[2];
-  const int64_t out0 = int64_t(blockIdx.x) * TILE;
+  const int64_t out0 = tile * TILE;
   const int64_t ps = (out0 / (2 * width)) * (2 * width);
   int64_t na = n - ps;
   na = na < 0 ? 0 : (na > width ? width : na);
   int64_t nb = n - ps - width;
   nb = nb < 0 ? 0 : (nb > width ? width : nb);
-  const K* a = src_k + ps;
-  const K* b = src_k + ps + width;
-  const int64_t d0 = out0 - ps;
-  const int64_t d1 = d0 + TILE < na + nb ? d0 + TILE : na + nb;
-  tile_splits<CMP>(a, na, b, nb, d0, d1, s_split);
-  const int64_t a0 = s_split[0], a1 = s_split[1];
-  const int64_t b0 = d0 - a0, b1 = d1 - a1;
-  merge_tile<CMP, K, V, HAS_V, BLOCK, ITEMS>(
-      a + a0, HAS_V ? src_v + ps + a0 : nullptr, int(a1 - a0), b + b0,
-      HAS_V ? src_v + ps + width + b0 : nullptr, int(b1 - b0), dst_k + out0,
-      HAS_V ? dst_v + out0 : nullptr, s_k, s_v);
+  merge_two_tile<CMP, K, V, HAS_V, BLOCK, ITEMS>(
+      sm, src_k + ps, HAS_V ? src_v + ps : nullptr, na, src_k + ps + width,
+      HAS_V ? src_v + ps + width : nullptr, nb, dst_k + ps, HAS_V ? dst_v + ps : nullptr,
+      (out0 - ps) / TILE);
+}
+
+template <class CMP, class K, class V, bool HAS_V, int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void merge_pass_kernel(const K* __restrict__ src_k,
+                                                           const V* __restrict__ src_v, int64_t n,
+                                                           int64_t width, K* __restrict__ dst_k,
+                                                           V* __restrict__ dst_v) {
+  __shared__ MergeSmem<K, V, HAS_V, BLOCK, ITEMS> sm;
+  merge_pass_tile<CMP, K, V, HAS_V, BLOCK, ITEMS>(sm, src_k, src_v, n, width, dst_k, dst_v,
+                                                  blockIdx.x);
 }
 
 inline int sort_num_passes(int64_t tiles) {

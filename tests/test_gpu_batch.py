@@ -1,0 +1,154 @@
+"""GPU parity of the batch executor: a mixed job list run as one persistent
+launch (eagerly and as a replayed hipGraph) and as one plain call per job,
+each job's result checked against the oracle."""
+import numpy as np
+import pytest
+import torch
+
+from bulk_amd import _capi as C
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _make_jobs(orc, sizes, seed=0):
+    """Jobs over every supported kind; returns (jobs, checks) where each check
+    compares a job's outputs with the oracle."""
+    from bulk_amd import batch as B
+    jobs, checks = [], []
+    rng = np.random.default_rng(seed)
+    for i, n in enumerate(sizes):
+        kind = i % 6
+        if kind == 0:  # int32 scan
+            x = orc.gen_uniform_i32(seed + i, -1000, 1000, n)
+            dx, out = _dev(x), torch.empty(n, dtype=torch.int32, device="cuda")
+            jobs.append(B.scan_job(dx, out))
+            ref = orc.inclusive_scan(x)
+            checks.append(lambda out=out, ref=ref: np.array_equal(out.cpu().numpy(), ref))
+        elif kind == 1:  # float32 scan, within the stated bound
+            x = orc.gen_uniform_f32(seed + i, n)
+            dx, out = _dev(x), torch.empty(n, dtype=torch.float32, device="cuda")
+            jobs.append(B.scan_job(dx, out))
+            hi, lo, ab = orc.prefix_sum_exact(x)
+            bound = orc.float_tolerance(n, 2.0 ** -24, ab)
+            checks.append(lambda out=out, hi=hi, lo=lo, b=bound:
+                          bool(np.all(np.abs(out.cpu().numpy().astype(np.float64) - hi - lo) <= b)))
+        elif kind == 2:  # reduce_by_key, int32 keys, int64 or float32 values
+            keys, runs = orc.gen_runs_geometric(seed + i, n, 8.0)
+            if i % 12 == 2:
+                vals = orc.gen_uniform_i64(seed + i, -(1 << 20), 1 << 20, n)
+            else:
+                vals = orc.gen_uniform_i32(seed + i, -50, 50, n)
+            dk, dv = _dev(keys), _dev(vals)
+            ko = torch.empty(n, dtype=torch.int32, device="cuda")
+            vo = torch.empty(n, dtype=dv.dtype, device="cuda")
+            cnt = torch.full((1,), -1, dtype=torch.int64, device="cuda")
+            jobs.append(B.reduce_by_key_job(dk, dv, ko, vo, cnt))
+            rk, rv = orc.reduce_by_key(keys, vals, textbook=True)
+            checks.append(lambda ko=ko, vo=vo, cnt=cnt, rk=rk, rv=rv:
+                          int(cnt.item()) == rk.size
+                          and np.array_equal(ko.cpu().numpy()[:rk.size], rk)
+                          and np.array_equal(vo.cpu().numpy()[:rk.size], rv))
+        elif kind == 3:  # merge_by_key, float32 keys with ties, int32 payload
+            pool = orc.gen_uniform_f32(seed + 99, 64)
+            n1 = n // 2 + 1
+            a = orc.gen_merge_keys(seed + i, 0, n1, pool, 0.1)
+            b = orc.gen_merge_keys(seed + i, 1, n - n1 + 3, pool, 0.1)
+            va = np.arange(a.size, dtype=np.int32)
+            vb = (1 << 28) + np.arange(b.size, dtype=np.int32)
+            ko = torch.empty(a.size + b.size, dtype=torch.float32, device="cuda")
+            vo = torch.empty(a.size + b.size, dtype=torch.int32, device="cuda")
+            jobs.append(B.merge_by_key_job(_dev(a), _dev(va), _dev(b), _dev(vb), ko, vo))
+            rk, rv = orc.merge_by_key(a, va, b, vb)
+            checks.append(lambda ko=ko, vo=vo, rk=rk, rv=rv:
+                          np.array_equal(ko.cpu().numpy(), rk) and np.array_equal(vo.cpu().numpy(), rv))
+        elif kind == 4:  # stable sort by key, int32 keys with ties, index payload
+            keys = orc.gen_uniform_i32(seed + i, 0, max(2, n // 16), n)
+            vals = np.arange(n, dtype=np.int32)
+            dk, dv = _dev(keys), _dev(vals)
+            jobs.append(B.sort_by_key_job(dk, dv))
+            order = np.argsort(keys, kind="stable")
+            rk, rv = keys[order], vals[order]
+            keys0, vals0 = dk.clone(), dv.clone()
+            checks.append(lambda dk=dk, dv=dv, rk=rk, rv=rv:
+                          np.array_equal(dk.cpu().numpy(), rk) and np.array_equal(dv.cpu().numpy(), rv))
+            # sort is in place: keep the input to restore between runs
+            jobs[-1].restore = (dk, dv, keys0, vals0)
+        else:  # reduce, int32
+            x = orc.gen_uniform_i32(seed + i, -1000, 1000, n)
+            out = torch.zeros(1, dtype=torch.int32, device="cuda")
+            jobs.append(B.reduce_job(_dev(x), out))
+            ref = int(orc.reduce(x, 0))
+            checks.append(lambda out=out, ref=ref: int(out.item()) == ref)
+    return jobs, checks
+
+
+def _restore(jobs):
+    for j in jobs:
+        r = getattr(j, "restore", None)
+        if r is not None:
+            r[0].copy_(r[2])
+            r[1].copy_(r[3])
+
+
+SIZES = [1, 1000, 4095, 4096, 4097, 12345, 16384, 16385, 65537, 200001, 3, 1 << 18,
+         77777, 8191, 8193, 1 << 20, 50000, 30000, 2, 4096 * 3 + 1, 123457, 9999, 1 << 16, 7]
+
+
+@pytest.mark.parametrize("mode", ["graph", "eager", "per_call"])
+def test_batch_matches_oracle(bx, orc, mode):
+    from bulk_amd.batch import Batch
+    jobs, checks = _make_jobs(orc, SIZES, seed=1)
+    b = Batch(max_jobs=64, max_elements=1 << 23)
+    b.set_jobs(jobs)
+    print(f"batch mode={mode} jobs={len(jobs)}", flush=True)
+    if mode == "per_call":
+        b.launch_per_call()
+    else:
+        b.launch(use_graph=(mode == "graph"))
+    torch.cuda.synchronize()
+    bad = [i for i, c in enumerate(checks) if not c()]
+    assert bad == [], f"jobs {bad} differ (sizes {[SIZES[i] for i in bad]})"
+    b.close()
+
+
+def test_batch_graph_replays_and_new_job_lists(bx, orc):
+    """Replays of one captured graph, with eager work between them and a new
+    job list uploaded before the last replays, all match the oracle."""
+    from bulk_amd.batch import Batch
+    b = Batch(max_jobs=64, max_elements=1 << 23)
+    jobs, checks = _make_jobs(orc, SIZES, seed=2)
+    b.set_jobs(jobs)
+    for rep in range(3):
+        _restore(jobs)
+        for j in jobs:  # scribble over the outputs eagerly
+            for name in ("out0", "out1"):
+                t = j.tensors.get(name)
+                if t is not None and j.kind != C.JOB_SORT_BY_KEY:
+                    t.fill_(7)
+        b.launch(use_graph=True)
+        torch.cuda.synchronize()
+        assert all(c() for c in checks), f"replay {rep}"
+    jobs2, checks2 = _make_jobs(orc, list(reversed(SIZES))[:17], seed=3)
+    b.set_jobs(jobs2)
+    for rep in range(2):
+        _restore(jobs2)
+        b.launch(use_graph=True)
+        torch.cuda.synchronize()
+        assert all(c() for c in checks2), f"new list replay {rep}"
+    b.close()
+
+
+def test_batch_rejects_over_capacity(bx, orc):
+    from bulk_amd import batch as B
+    from bulk_amd._capi import BulkError
+    b = B.Batch(max_jobs=2, max_elements=1000)
+    x = torch.zeros(2000, dtype=torch.int32, device="cuda")
+    with pytest.raises(BulkError):
+        b.set_jobs([B.scan_job(x, x)])
+    with pytest.raises(BulkError):
+        b.set_jobs([B.scan_job(x[:10], x[:10])] * 3)
+    b.close()
