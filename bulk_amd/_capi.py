@@ -87,9 +87,12 @@ SIGNATURES = {
     "bulkx_batch_set_jobs": (_I, [_P, ctypes.POINTER(Job), _I]),
     "bulkx_batch_launch": (_I, [_P, _I]),
     "bulkx_batch_launch_per_call": (_I, [_P]),
-    "bulkx_server_create": (_I, [_L, _I, _I, ctypes.POINTER(_P)]),
+    "bulkx_batch_last_timing": (_I, [_P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
+    "bulkx_server_create": (_I, [_L, _I, _I, _L, _I, ctypes.POINTER(_P)]),
     "bulkx_server_destroy": (_I, [_P]),
     "bulkx_server_submit": (_I, [_P, ctypes.POINTER(Job), _L, ctypes.POINTER(_L)]),
+    "bulkx_server_submit_at": (_I, [_P, ctypes.POINTER(Job), _L, ctypes.c_double, ctypes.POINTER(_L)]),
+    "bulkx_server_now_us": (ctypes.c_double, [_P]),
     "bulkx_server_drain": (_I, [_P]),
     "bulkx_server_request_times": (_I, [_P, _L, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_double),

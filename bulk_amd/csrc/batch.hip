@@ -325,20 +325,26 @@ struct bulkx_batch {
   int num_jobs = 0;
   int64_t total_tiles0 = 0, total_sort_tiles = 0;
   bool pending_upload = false;
+  // phase timing of eager launches
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  bool timed = false;
   // graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
 
-static int batch_enqueue_kernels(bulkx_batch* b, hipStream_t s) {
+static int batch_enqueue_kernels(bulkx_batch* b, hipStream_t s, bool timed) {
   batch_clear_kernel<<<256, 256, 0, s>>>(b->d_desc, b->d_status);
   BULKX_LAUNCHED();
+  if (timed) BULKX_HIP(hipEventRecord(b->ev[0], s));
   batch_phase0_kernel<<<b->grid, kBatchBlock, 0, s>>>(b->d_desc, b->d_jobs, b->d_tile_job);
   BULKX_LAUNCHED();
+  if (timed) BULKX_HIP(hipEventRecord(b->ev[1], s));
   for (int p = 0; p < b->max_passes; ++p) {
     batch_pass_kernel<<<b->grid, kBatchBlock, 0, s>>>(b->d_desc, b->d_jobs, b->d_sort_tile_job, p);
     BULKX_LAUNCHED();
   }
+  if (timed) BULKX_HIP(hipEventRecord(b->ev[2], s));
   return BULKX_SUCCESS;
 }
 
@@ -364,7 +370,7 @@ extern "C" int bulkx_batch_create(int max_jobs, int64_t max_elements, void* stre
   if (e != hipSuccess || per_cu < 1) per_cu = 1;
   b->grid = per_cu * props.num_cus;
   b->status_words_cap = size_t(b->max_tiles) * kStatusWordsPerTile;
-  b->tmp_bytes_cap = size_t(max_elements) * 8 + size_t(max_jobs) * 512;
+  b->tmp_bytes_cap = 0;  // sort ping-pong space grows on demand in set_jobs
   // largest per-call workspace: sort of max_elements 4-byte keys + 4-byte values
   b->ws_bytes = std::max({sort_workspace_bytes(BULKX_INT32, BULKX_INT32, max_elements),
                           scan_workspace_bytes(BULKX_FLOAT32, max_elements),
@@ -393,7 +399,6 @@ extern "C" int bulkx_batch_create(int max_jobs, int64_t max_elements, void* stre
   dmalloc(reinterpret_cast<void**>(&b->d_tile_job), sizeof(uint32_t) * b->max_tiles);
   dmalloc(reinterpret_cast<void**>(&b->d_sort_tile_job), sizeof(uint32_t) * b->max_tiles);
   dmalloc(reinterpret_cast<void**>(&b->d_status), sizeof(uint64_t) * b->status_words_cap);
-  dmalloc(reinterpret_cast<void**>(&b->d_tmp), b->tmp_bytes_cap);
   dmalloc(reinterpret_cast<void**>(&b->d_ws), b->ws_bytes);
   hmalloc(reinterpret_cast<void**>(&b->h_desc), sizeof(BatchDesc));
   hmalloc(reinterpret_cast<void**>(&b->h_jobs), sizeof(BatchJobDev) * max_jobs);
@@ -405,6 +410,7 @@ extern "C" int bulkx_batch_create(int max_jobs, int64_t max_elements, void* stre
   }
   memset(b->h_desc, 0, sizeof(BatchDesc));
   hipError_t r = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking);
+  for (int i = 0; i < 3 && r == hipSuccess; ++i) r = hipEventCreate(&b->ev[i]);
   if (r != hipSuccess) {
     set_last_error("hipStreamCreateWithFlags", r);
     bulkx_batch_destroy(b);
@@ -425,6 +431,8 @@ extern "C" int bulkx_batch_destroy(bulkx_batch* b) {
   (void)hipStreamSynchronize(b->stream);
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
+  for (hipEvent_t e : b->ev)
+    if (e) (void)hipEventDestroy(e);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   for (void* p : {static_cast<void*>(b->d_desc), static_cast<void*>(b->d_jobs),
                   static_cast<void*>(b->d_tile_job), static_cast<void*>(b->d_sort_tile_job),
@@ -478,11 +486,19 @@ extern "C" int bulkx_batch_set_jobs(bulkx_batch* b, const bulkx_job* jobs, int n
     tiles0 += t;
   }
   if (elements > b->max_elements || tiles0 > b->max_tiles || sort_tiles > b->max_tiles ||
-      status_words > b->status_words_cap || tmp_bytes > b->tmp_bytes_cap)
+      status_words > b->status_words_cap)
     return BULKX_ERR_BUDGET;
 
   // The staging buffers may still feed the previous upload.
   BULKX_HIP(hipStreamSynchronize(b->stream));
+  if (tmp_bytes > b->tmp_bytes_cap) {  // grow the sort ping-pong space (never inside a launch)
+    if (b->d_tmp) BULKX_HIP(hipFree(b->d_tmp));
+    b->d_tmp = nullptr;
+    b->tmp_bytes_cap = 0;
+    const size_t want = tmp_bytes + tmp_bytes / 4;
+    BULKX_HIP(hipMalloc(reinterpret_cast<void**>(&b->d_tmp), want));
+    b->tmp_bytes_cap = want;
+  }
   b->jobs.assign(jobs, jobs + num_jobs);
   b->num_jobs = num_jobs;
   int64_t t0 = 0, ts = 0;
@@ -556,10 +572,13 @@ extern "C" int bulkx_batch_set_jobs(bulkx_batch* b, const bulkx_job* jobs, int n
 
 extern "C" int bulkx_batch_launch(bulkx_batch* b, int use_graph) {
   if (!b) return BULKX_ERR_INVALID_ARGUMENT;
-  if (!use_graph) return batch_enqueue_kernels(b, b->stream);
+  if (!use_graph) {
+    b->timed = true;
+    return batch_enqueue_kernels(b, b->stream, true);
+  }
   if (!b->exec) {
     BULKX_HIP(hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal));
-    int st = batch_enqueue_kernels(b, b->capture_stream);
+    int st = batch_enqueue_kernels(b, b->capture_stream, false);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(b->capture_stream, &g);
     if (st != BULKX_SUCCESS) return st;
@@ -604,5 +623,17 @@ extern "C" int bulkx_batch_launch_per_call(bulkx_batch* b) {
     }
     if (st != BULKX_SUCCESS) return st;
   }
+  return BULKX_SUCCESS;
+}
+
+extern "C" int bulkx_batch_last_timing(bulkx_batch* b, float* phase0_ms, float* passes_ms) {
+  if (!b) return BULKX_ERR_INVALID_ARGUMENT;
+  if (!b->timed) return BULKX_ERR_NOT_READY;
+  BULKX_HIP(hipEventSynchronize(b->ev[2]));
+  float a = 0.f, c = 0.f;
+  BULKX_HIP(hipEventElapsedTime(&a, b->ev[0], b->ev[1]));
+  BULKX_HIP(hipEventElapsedTime(&c, b->ev[1], b->ev[2]));
+  if (phase0_ms) *phase0_ms = a;
+  if (passes_ms) *passes_ms = c;
   return BULKX_SUCCESS;
 }

@@ -3,6 +3,8 @@
 #include <stdlib.h>
 
 #include "host_common.hpp"
+#include "merge_impl.hpp"
+#include "reduce_by_key_impl.hpp"
 #include "scan_config.hpp"
 #include "tuning.hpp"
 
@@ -17,7 +19,7 @@ int choose_shape(int algo, int dtype, int64_t n, LaunchShape* out) {
   switch (algo) {
     case BULKX_ALGO_INCLUSIVE_SCAN:
     case BULKX_ALGO_EXCLUSIVE_SCAN: {
-      s.group_size = 256;
+      s.group_size = es == 4 ? ScanConfig<float>::BLOCK : ScanConfig<double>::BLOCK;
       s.grain_size = es == 4 ? ScanConfig<float>::ITEMS : ScanConfig<double>::ITEMS;
       int64_t tile = int64_t(s.group_size) * s.grain_size;
       s.num_groups = (n + tile - 1) / tile;
@@ -29,6 +31,28 @@ int choose_shape(int algo, int dtype, int64_t n, LaunchShape* out) {
       int64_t tile = int64_t(s.group_size) * s.grain_size;
       int64_t want = (n + tile - 1) / tile;
       s.num_groups = want < int64_t(cus) * 8 ? want : int64_t(cus) * 8;
+      break;
+    }
+    case BULKX_ALGO_REDUCE_BY_KEY: {
+      s.group_size = RbkConfig<int, int>::BLOCK;
+      s.grain_size = RbkConfig<int, int>::ITEMS;
+      int64_t tile = int64_t(s.group_size) * s.grain_size;
+      s.num_groups = (n + tile - 1) / tile;
+      break;
+    }
+    case BULKX_ALGO_MERGE:
+    case BULKX_ALGO_MERGE_BY_KEY:
+    case BULKX_ALGO_SORT_BY_KEY: {
+      s.group_size = es == 4 ? MergeConfig<float>::BLOCK : MergeConfig<double>::BLOCK;
+      s.grain_size = es == 4 ? MergeConfig<float>::ITEMS : MergeConfig<double>::ITEMS;
+      int64_t tile = int64_t(s.group_size) * s.grain_size;
+      s.num_groups = (n + tile - 1) / tile;
+      break;
+    }
+    case BULKX_ALGO_REDUCE_INTERVALS: {
+      s.group_size = 256;
+      s.grain_size = 1;
+      s.num_groups = n;  // one workgroup per interval (capped at 65536 and looped)
       break;
     }
     default:

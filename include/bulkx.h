@@ -260,6 +260,10 @@ int bulkx_batch_set_jobs(bulkx_batch* batch, const bulkx_job* jobs, int num_jobs
 int bulkx_batch_launch(bulkx_batch* batch, int use_graph);
 /* Run each job as its own plain call on the batch's stream (the per-call baseline). */
 int bulkx_batch_launch_per_call(bulkx_batch* batch);
+/* Device time in ms of the phases of the last eager launch (use_graph = 0):
+ * the persistent first-phase kernel and all sort merge rounds together.
+ * Waits for that launch to finish.  BULKX_ERR_NOT_READY if there is none. */
+int bulkx_batch_last_timing(bulkx_batch* batch, float* phase0_ms, float* passes_ms);
 
 /* ------------------------------------------------------------------------
  * Serving: a request loop that admits jobs under an HBM budget, keeps their
@@ -268,16 +272,31 @@ int bulkx_batch_launch_per_call(bulkx_batch* batch);
  * ------------------------------------------------------------------------ */
 typedef struct bulkx_server bulkx_server;
 
+/* Server flags. */
+enum bulkx_serve_flags {
+  BULKX_SERVE_PER_CALL = 1, /* launch every request alone through the plain entry points */
+  BULKX_SERVE_NO_GRAPH = 2  /* launch batches without hipGraph replay */
+};
+
 /* hbm_budget_bytes bounds the device memory of admitted-but-unfinished jobs;
- * max_inflight bounds the batches on the device at once (one stream each). */
+ * max_inflight bounds the batches on the device at once (one stream each);
+ * a batch holds at most max_batch_jobs jobs and max_batch_elements elements. */
 int bulkx_server_create(int64_t hbm_budget_bytes, int max_inflight, int max_batch_jobs,
-                        bulkx_server** server);
+                        int64_t max_batch_elements, int flags, bulkx_server** server);
 int bulkx_server_destroy(bulkx_server* server);
 /* Submit a job whose data already resides in device memory.  `bytes` is its
  * device footprint charged to the budget.  Returns an id through *request_id;
  * BULKX_ERR_BUDGET if the job alone exceeds the budget. */
 int bulkx_server_submit(bulkx_server* server, const bulkx_job* job, int64_t bytes,
                         int64_t* request_id);
+/* Like bulkx_server_submit, but the request arrives at `arrival_us` on the
+ * server clock (microseconds since creation; requests must be submitted in
+ * arrival order).  The dispatcher does not see it before that time, so an
+ * open-loop arrival process can be replayed exactly. */
+int bulkx_server_submit_at(bulkx_server* server, const bulkx_job* job, int64_t bytes,
+                           double arrival_us, int64_t* request_id);
+/* Microseconds elapsed on the server clock. */
+double bulkx_server_now_us(bulkx_server* server);
 /* Block until every submitted request has completed. */
 int bulkx_server_drain(bulkx_server* server);
 /* Per-request timing in microseconds since server creation: arrival (submit),

@@ -58,13 +58,20 @@ def test_zero_length_calls_are_noops():
 def test_workspace_size_query_is_host_only():
     lib = C.lib()
     nb = ctypes.c_size_t()
-    assert lib.bulkx_workspace_size(C.ALGO_INCLUSIVE_SCAN, C.INT32, -1, 1 << 24, 0,
-                                    ctypes.byref(nb)) == 0
-    # one status word per 4096-element tile plus a header
-    assert nb.value == 256 + 8 * ((1 << 24) // 4096)
-    assert lib.bulkx_workspace_size(C.ALGO_INCLUSIVE_SCAN, C.FLOAT64, -1, 1 << 20, 0,
-                                    ctypes.byref(nb)) == 0
-    assert nb.value == 256 + 32 * ((1 << 20) // 2048)
+    ng, gs, gr = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()
+    # one look-back record per tile plus a 256-byte header; records are one
+    # 64-bit word per 32 bits of the accumulator (int32: 1, float64 as
+    # double-double: 4)
+    for dt, n, words in [(C.INT32, 1 << 24, 1), (C.FLOAT64, 1 << 20, 4), (C.FLOAT32, 12345679, 2)]:
+        assert lib.bulkx_workspace_size(C.ALGO_INCLUSIVE_SCAN, dt, -1, n, 0, ctypes.byref(nb)) == 0
+        assert lib.bulkx_choose_sizes(C.ALGO_INCLUSIVE_SCAN, dt, n, ctypes.byref(ng), ctypes.byref(gs),
+                                      ctypes.byref(gr)) == 0
+        tile = gs.value * gr.value
+        assert ng.value == -(-n // tile)
+        assert nb.value == 256 + 8 * words * ng.value
+    # a single tile needs no scratch
+    assert lib.bulkx_workspace_size(C.ALGO_INCLUSIVE_SCAN, C.INT32, -1, 1000, 0, ctypes.byref(nb)) == 0
+    assert nb.value == 0
     assert lib.bulkx_workspace_size(C.ALGO_INCLUSIVE_SCAN, 9, -1, 10, 0, ctypes.byref(nb)) == 2
 
 
