@@ -1,0 +1,358 @@
+"""Mixed-job stream benchmark on one MI355X.
+
+Workload: 4096 device-resident jobs (scan 40%, reduce_by_key 30%, merge 20%,
+sort_by_key 10%), sizes log-uniform on [2^10, 2^22] elements, int32 or
+float32 with probability 1/2 (bulk_amd.workload).
+
+One step = the whole stream through the batch executor: one hipGraph replay
+of the persistent segmented launch (clear + phase-0 + sort merge rounds).
+The same job list is also run as one plain launch per job (the per-call
+baseline) and through the serving loop (continuous batching under a 32 GB
+HBM budget, saturated and with Poisson arrivals) for p50/p99 latency.
+
+Prints ONE JSON line (rank 0, N=1).  `--gpus N` with N > 1 exits: this
+build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _parse():
+    p = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--jobs", type=int, default=4096)
+    p.add_argument("--seed", type=int, default=5)
+    p.add_argument("--budget-gb", type=float, default=32.0)
+    p.add_argument("--no-serve", action="store_true",
+                   help="skip the serving runs (batched and per-call steps only)")
+    p.add_argument("--load", type=float, default=0.5,
+                   help="Poisson arrival rate as a fraction of the saturated served job rate")
+    p.add_argument("--cpu-seconds", type=float, default=10.0,
+                   help="wall-clock budget of the CPU baseline sample")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-per-call", action="store_true",
+                   help="skip the per-call leg (and the batched == per-call check)")
+    p.add_argument("--traffic-file", default=os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                          "profiles", "batch_phase0_traffic.txt"))
+    return p.parse_args()
+
+
+def _log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _pct(xs, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if len(xs) else None
+
+
+def _sort_passes(n, tile=4096):
+    t = (n + tile - 1) // tile
+    p = 0
+    while (1 << p) < t:
+        p += 1
+    return p
+
+
+def _read_traffic(path):
+    """HBM bytes per phase-0 launch from a committed rocprofv3 --pmc summary
+    (lines `fetch_bytes_per_launch: X` and `write_bytes_per_launch: Y`)."""
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if ":" in line:
+            k, v = line.split(":", 1)
+            try:
+                vals[k.strip()] = float(v.split()[0])
+            except (ValueError, IndexError):
+                pass
+    if "fetch_bytes_per_launch" in vals and "write_bytes_per_launch" in vals:
+        return vals["fetch_bytes_per_launch"] + vals["write_bytes_per_launch"], vals.get("jobs"), \
+            vals.get("seed")
+    return None
+
+
+def main():
+    args = _parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(jobs would shard by job across GPUs with no exchange).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from bulk_amd import _capi as C
+    from bulk_amd import workload as W
+    from bulk_amd.batch import Batch
+
+    C.lib()  # fails loudly when the native library is not built
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+
+    # ---------------------------------------------------------------- data
+    t0 = time.perf_counter()
+    specs = W.make_specs(args.jobs, seed=args.seed)
+    jobs = W.build_jobs(specs, seed=args.seed + 2)
+    torch.cuda.synchronize()
+    total = W.total_elements(jobs)
+    _log(f"generated {len(jobs)} jobs, {total} elements in {time.perf_counter() - t0:.1f}s")
+
+    # device copy bandwidth (read 1 GiB + write 1 GiB)
+    src = torch.empty(1 << 28, dtype=torch.int32, device=dev).fill_(1)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 2 * src.numel() * 4 * 10 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del src, dst
+    _log(f"device copy {copy_gbs:.0f} GB/s")
+
+    # ----------------------------------------------------- per-call baseline
+    def timed(fn, steps):
+        ms = []
+        for _ in range(steps):
+            W.reset_all(jobs)
+            torch.cuda.synchronize()
+            a = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ms.append((time.perf_counter() - a) * 1e3)
+        return ms
+
+    ref_out, pc_ms = None, None
+    if not args.no_per_call:
+        pc = Batch(max_jobs=len(jobs), max_elements=total)
+        pc.set_jobs([j.job for j in jobs])
+        W.reset_all(jobs)
+        pc.launch_per_call()
+        torch.cuda.synchronize()
+        ref_out = {}
+        for i, j in enumerate(jobs):
+            t = j.job.tensors
+            if j.spec.kind == C.JOB_REDUCE_BY_KEY:
+                m = int(t["out2"].item())
+                ref_out[i] = (t["out0"][:m].clone(), t["out1"][:m].clone(), m)
+            elif j.spec.kind == C.JOB_SORT_BY_KEY:
+                ref_out[i] = (t["out0"].clone(), t["out1"].clone())
+            else:
+                ref_out[i] = tuple(t[k].clone() for k in ("out0", "out1") if k in t)
+        for _ in range(args.warmup):
+            W.reset_all(jobs)
+            pc.launch_per_call()
+        torch.cuda.synchronize()
+        pc_ms = timed(pc.launch_per_call, args.steps)
+        pc.close()
+        _log(f"per-call: {np.mean(pc_ms):.2f} ms/step")
+
+    # --------------------------------------------------------- batched (graph)
+    bt = Batch(max_jobs=len(jobs), max_elements=total)
+    bt.set_jobs([j.job for j in jobs])
+    for _ in range(args.warmup):
+        W.reset_all(jobs)
+        bt.launch(use_graph=True)
+    torch.cuda.synchronize()
+
+    # equal accuracy: every batched result equals the per-call one (integer
+    # work bit-exact, float scans / sums within the stated bound of FP64)
+    W.reset_all(jobs)
+    bt.launch(use_graph=True)
+    torch.cuda.synchronize()
+    runs = {i: int(j.job.tensors["out2"].item()) for i, j in enumerate(jobs)
+            if j.spec.kind == C.JOB_REDUCE_BY_KEY}
+    mismatches, worst_ratio = 0, 0.0
+    for i, j in enumerate(jobs):
+        if ref_out is None:
+            break
+        t = j.job.tensors
+        k, dt, n = j.spec.kind, j.spec.dtype, j.spec.n
+        if k == C.JOB_REDUCE_BY_KEY:
+            m = runs[i]
+            ok = m == ref_out[i][2] and torch.equal(t["out0"][:m], ref_out[i][0])
+            if dt == "int32":
+                ok = ok and torch.equal(t["out1"][:m], ref_out[i][1])
+            else:  # FP64 run sums (keys are run indices 0..m-1)
+                kk = t["in0"].long()
+                v64 = torch.zeros(m, dtype=torch.float64, device=dev).index_add_(0, kk, t["in1"].double())
+                a64 = torch.zeros(m, dtype=torch.float64, device=dev).index_add_(
+                    0, kk, t["in1"].double().abs())
+                bound = 4 * max(1, int(np.ceil(np.log2(max(n, 2))))) * 2.0 ** -24 * a64
+                err = (t["out1"][:m].double() - v64).abs()
+                ok = ok and bool((err <= bound).all())
+                worst_ratio = max(worst_ratio, float((err / bound.clamp_min(1e-300)).max()))
+        elif k == C.JOB_INCLUSIVE_SCAN and dt == "float32":
+            x64 = t["in0"].double()
+            p64 = torch.cumsum(x64, 0)
+            a64 = torch.cumsum(x64.abs(), 0)
+            bound = 4 * max(1, int(np.ceil(np.log2(max(n, 2))))) * 2.0 ** -24 * a64
+            err = (t["out0"].double() - p64).abs()
+            ok = bool((err <= bound).all())
+            worst_ratio = max(worst_ratio, float((err / bound.clamp_min(1e-300)).max()))
+        else:
+            ok = all(torch.equal(t[name], r) for name, r in zip(("out0", "out1"), ref_out[i]))
+        mismatches += 0 if ok else 1
+    checked = ref_out is not None
+    ref_out = None  # free the per-call copies
+    _log(f"batched vs per-call: {mismatches} mismatching jobs; worst float err/bound {worst_ratio:.3g}")
+
+    bt_ms = timed(lambda: bt.launch(use_graph=True), args.steps)
+    _log(f"batched: {np.mean(bt_ms):.2f} ms/step")
+
+    # roofline of the phase-0 kernel: eager launches timed with HIP events on
+    # the batch's stream (phase-0 only; the sort merge rounds separately)
+    p0, pp = [], []
+    for _ in range(max(3, args.steps)):
+        W.reset_all(jobs)
+        bt.launch(use_graph=False)
+        a, b = bt.last_timing()
+        p0.append(a)
+        pp.append(b)
+    torch.cuda.synchronize()
+    bt.close()
+    phase0_bytes = sum(j.algorithmic_bytes(runs.get(i)) for i, j in enumerate(jobs))
+    pass_bytes = sum(_sort_passes(j.spec.n) * 16 * j.spec.n for j in jobs
+                     if j.spec.kind == C.JOB_SORT_BY_KEY)
+    p0_ms, pp_ms = float(np.mean(p0)), float(np.mean(pp))
+    p0_gbs = phase0_bytes / (p0_ms * 1e-3) / 1e9
+    pp_gbs = pass_bytes / (pp_ms * 1e-3) / 1e9 if pp_ms > 0 else None
+    _log(f"phase0 {p0_ms:.3f} ms {p0_gbs:.0f} GB/s; sort rounds {pp_ms:.3f} ms")
+
+    # ---------------------------------------------------------------- serving
+    serve = None
+    if not args.no_serve:
+        from bulk_amd.serve import Server, SERVE_PER_CALL
+        budget = int(args.budget_gb * 1e9)
+
+        def serve_run(flags, arrivals_us):
+            W.reset_all(jobs)
+            torch.cuda.synchronize()
+            s = Server(budget, max_inflight=3, max_batch_jobs=512, max_batch_elements=1 << 27,
+                       flags=flags)
+            t_start = s.now_us() + 2000.0
+            ids = []
+            for j, off in zip(jobs, arrivals_us):
+                ids.append(s.submit(j.job, j.footprint_bytes(), arrival_us=t_start + off))
+            s.drain()
+            times = [s.times(r) for r in ids]
+            st = s.stats()
+            s.close()
+            lat = [(d - a) * 1e-3 for a, _, d in times]  # ms
+            span_ms = (max(d for _, _, d in times) - t_start) * 1e-3
+            return {"p50_ms": _pct(lat, 50), "p99_ms": _pct(lat, 99), "span_ms": span_ms,
+                    "batches": st["batches"], "peak_admitted_gb": st["peak_bytes"] / 1e9}
+
+        zero = np.zeros(len(jobs))
+        serve_run(0, zero)  # warm the slots' graphs
+        sat = serve_run(0, zero)
+        sat_melem = total / (sat["span_ms"] * 1e-3) / 1e6
+        rate = args.load * len(jobs) / (sat["span_ms"] * 1e-3)
+        arr = W.poisson_arrivals(len(jobs), rate, seed=args.seed + 1)
+        pois = serve_run(0, arr)
+        pois_pc = serve_run(SERVE_PER_CALL, arr)
+        sat_pc = serve_run(SERVE_PER_CALL, zero)
+        serve = {
+            "saturated": {**sat, "melem_per_s": sat_melem},
+            "saturated_per_call": {**sat_pc, "melem_per_s": total / (sat_pc["span_ms"] * 1e-3) / 1e6},
+            "poisson_rate_jobs_per_s": rate,
+            "poisson": pois,
+            "poisson_per_call": pois_pc,
+        }
+        _log(f"served saturated {sat_melem:.0f} Melem/s p99 {sat['p99_ms']:.2f} ms; "
+             f"poisson p50 {pois['p50_ms']:.3f} p99 {pois['p99_ms']:.3f} ms; "
+             f"per-call poisson p99 {pois_pc['p99_ms']:.3f} ms")
+
+    # ------------------------------------------------------------ CPU baseline
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        done_elems, cpu_s, sample = 0, 0.0, 0
+        order = np.random.default_rng(args.seed + 3).permutation(len(jobs))
+        for i in order:
+            if cpu_s >= args.cpu_seconds:
+                break
+            j = jobs[int(i)]
+            t = j.job.tensors
+            j.reset()
+            host = {k: v.cpu().numpy() for k, v in t.items() if k.startswith("in") or
+                    (j.spec.kind == C.JOB_SORT_BY_KEY and k in ("out0", "out1"))}
+            a = time.perf_counter()
+            if j.spec.kind == C.JOB_INCLUSIVE_SCAN:
+                O.inclusive_scan(host["in0"])
+            elif j.spec.kind == C.JOB_REDUCE_BY_KEY:
+                O.reduce_by_key(host["in0"], host["in1"])
+            elif j.spec.kind == C.JOB_MERGE_BY_KEY:
+                O.merge_by_key(host["in0"], host["in1"], host["in2"], host["in3"])
+            else:
+                O.stable_sort_by_key(host["out0"], host["out1"])
+            cpu_s += time.perf_counter() - a
+            done_elems += j.elements
+            sample += 1
+        cpu = {"value": done_elems / cpu_s / 1e6, "unit": "Melem/s", "cores": 1, "kind": "port",
+               "sample": f"{sample} of the {len(jobs)} jobs ({done_elems} elements, {cpu_s:.1f} s), "
+                         "random order, sequential restatement of the reference algorithm bodies"}
+        _log(f"cpu baseline {cpu['value']:.1f} Melem/s over {sample} jobs")
+
+    # ------------------------------------------------------------------ report
+    bt_mean = float(np.mean(bt_ms))
+    value = total / (bt_mean * 1e-3) / 1e6
+    pc_mean = float(np.mean(pc_ms)) if pc_ms else None
+    pc_value = total / (pc_mean * 1e-3) / 1e6 if pc_ms else None
+    tr = _read_traffic(args.traffic_file)
+    traffic = None
+    if tr is not None and tr[1] == len(jobs) and tr[2] == args.seed:
+        traffic = tr[0]
+    line = {
+        "metric": "batched vs per-call Melem/s + p99 latency, 4k mixed jobs; GB/s %HBM peak",
+        "value": value,
+        "unit": "Melem/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": bt_mean,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32+float32",
+        "data": "synthetic",
+        "config": {"workload": f"mixed-job stream: {len(jobs)} jobs (scan 40%, reduce_by_key 30%, "
+                               "merge 20%, sort_by_key 10%), log-uniform 2^10..2^22 elements, "
+                               "int32/float32", "jobs": len(jobs), "elements": total,
+                   "seed": args.seed, "hbm_budget_gb": args.budget_gb},
+        "per_call_melem_per_s": pc_value,
+        "per_call_ms_per_step": pc_mean,
+        "speedup_vs_per_call": value / pc_value if pc_value else None,
+        "batched_equals_per_call": (mismatches == 0) if checked else None,
+        "float_err_over_bound_max": worst_ratio,
+        "serve": serve,
+        "p99_ms": serve["poisson"]["p99_ms"] if serve else None,
+        "p50_ms": serve["poisson"]["p50_ms"] if serve else None,
+        "copy_gbs": copy_gbs,
+        "roofline": {"bound": "hbm", "kernel": "batch_phase0_kernel", "achieved": p0_gbs,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": p0_gbs / HBM_PEAK_GBS,
+                     "frac_of_copy": p0_gbs / copy_gbs, "traffic": traffic,
+                     "algorithmic_bytes": phase0_bytes, "avg_launch_ms": p0_ms,
+                     "launches": len(p0)},
+        "sort_rounds": {"kernel": "batch_pass_kernel", "ms": pp_ms, "bytes": pass_bytes,
+                        "gbs": pp_gbs},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -85,6 +85,14 @@ class Batch:
     def launch_per_call(self) -> None:
         C.check(C.lib().bulkx_batch_launch_per_call(self._h), "bulkx_batch_launch_per_call")
 
+    def last_timing(self) -> tuple[float, float]:
+        """(phase-0 ms, sort-merge-rounds ms) of the last eager launch, from
+        HIP events recorded on the batch's stream."""
+        a, b = ctypes.c_float(), ctypes.c_float()
+        C.check(C.lib().bulkx_batch_last_timing(self._h, ctypes.byref(a), ctypes.byref(b)),
+                "bulkx_batch_last_timing")
+        return a.value, b.value
+
     def close(self) -> None:
         if self._h:
             C.lib().bulkx_batch_destroy(self._h)

@@ -291,6 +291,17 @@ static int batch_op(const bulkx_job& j) {
   }
 }
 
+// Workspace one plain entry point needs for job j.
+static size_t per_call_workspace(const bulkx_job& j) {
+  switch (j.kind) {
+    case BULKX_JOB_INCLUSIVE_SCAN: return scan_workspace_bytes(j.dtype, j.n0);
+    case BULKX_JOB_REDUCE: return reduce_workspace_bytes(j.dtype, j.n0);
+    case BULKX_JOB_REDUCE_BY_KEY: return reduce_by_key_workspace_bytes(j.dtype, j.value_dtype, j.n0);
+    case BULKX_JOB_SORT_BY_KEY: return sort_workspace_bytes(j.dtype, j.value_dtype, j.n0);
+    default: return 0;
+  }
+}
+
 }  // namespace bulkx
 
 using namespace bulkx;
@@ -371,12 +382,7 @@ extern "C" int bulkx_batch_create(int max_jobs, int64_t max_elements, void* stre
   b->grid = per_cu * props.num_cus;
   b->status_words_cap = size_t(b->max_tiles) * kStatusWordsPerTile;
   b->tmp_bytes_cap = 0;  // sort ping-pong space grows on demand in set_jobs
-  // largest per-call workspace: sort of max_elements 4-byte keys + 4-byte values
-  b->ws_bytes = std::max({sort_workspace_bytes(BULKX_INT32, BULKX_INT32, max_elements),
-                          scan_workspace_bytes(BULKX_FLOAT32, max_elements),
-                          reduce_workspace_bytes(BULKX_FLOAT32, max_elements),
-                          reduce_by_key_workspace_bytes(BULKX_INT32, BULKX_INT64, max_elements),
-                          size_t(256)});
+  b->ws_bytes = 256;  // the per-call workspace grows on demand in set_jobs
   int st = BULKX_SUCCESS;
   auto dmalloc = [&](void** p, size_t bytes) {
     if (st != BULKX_SUCCESS) return;
@@ -451,7 +457,7 @@ extern "C" int bulkx_batch_set_jobs(bulkx_batch* b, const bulkx_job* jobs, int n
     return BULKX_ERR_INVALID_ARGUMENT;
   // Validate and lay out on the host first; nothing is touched on failure.
   int64_t elements = 0, tiles0 = 0, sort_tiles = 0;
-  size_t status_words = 0, tmp_bytes = 0;
+  size_t status_words = 0, tmp_bytes = 0, ws_need = 256;
   for (int i = 0; i < num_jobs; ++i) {
     const bulkx_job& j = jobs[i];
     const int op = batch_op(j);
@@ -459,6 +465,7 @@ extern "C" int bulkx_batch_set_jobs(bulkx_batch* b, const bulkx_job* jobs, int n
     if (j.n0 < 0 || j.n1 < 0) return BULKX_ERR_INVALID_ARGUMENT;
     const int64_t n = j.n0 + (op == OP_MERGE_I32 || op == OP_MERGE_F32 ? j.n1 : 0);
     elements += n;
+    ws_need = std::max(ws_need, per_call_workspace(j));
     if (n == 0 && j.kind != BULKX_JOB_REDUCE && j.kind != BULKX_JOB_REDUCE_BY_KEY) continue;
     const bool needs_out2 = j.kind == BULKX_JOB_REDUCE_BY_KEY;
     if (!j.out0 || (needs_out2 && (!j.out1 || !j.out2)) ||
@@ -498,6 +505,13 @@ extern "C" int bulkx_batch_set_jobs(bulkx_batch* b, const bulkx_job* jobs, int n
     const size_t want = tmp_bytes + tmp_bytes / 4;
     BULKX_HIP(hipMalloc(reinterpret_cast<void**>(&b->d_tmp), want));
     b->tmp_bytes_cap = want;
+  }
+  if (ws_need > b->ws_bytes) {  // grow the workspace of the per-call path
+    if (b->d_ws) BULKX_HIP(hipFree(b->d_ws));
+    b->d_ws = nullptr;
+    b->ws_bytes = 0;
+    BULKX_HIP(hipMalloc(reinterpret_cast<void**>(&b->d_ws), ws_need));
+    b->ws_bytes = ws_need;
   }
   b->jobs.assign(jobs, jobs + num_jobs);
   b->num_jobs = num_jobs;

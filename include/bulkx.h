@@ -286,7 +286,9 @@ int bulkx_server_create(int64_t hbm_budget_bytes, int max_inflight, int max_batc
 int bulkx_server_destroy(bulkx_server* server);
 /* Submit a job whose data already resides in device memory.  `bytes` is its
  * device footprint charged to the budget.  Returns an id through *request_id;
- * BULKX_ERR_BUDGET if the job alone exceeds the budget. */
+ * BULKX_ERR_BUDGET if the job alone exceeds the budget.  The server's streams
+ * are non-blocking: the job's inputs must be complete on the device when it
+ * is submitted (the caller synchronizes the stream that produced them). */
 int bulkx_server_submit(bulkx_server* server, const bulkx_job* job, int64_t bytes,
                         int64_t* request_id);
 /* Like bulkx_server_submit, but the request arrives at `arrival_us` on the
