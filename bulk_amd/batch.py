@@ -93,6 +93,16 @@ class Batch:
                 "bulkx_batch_last_timing")
         return a.value, b.value
 
+    STAGES = ("scan", "reduce_by_key_4b", "reduce_by_key_8b", "merge_by_key", "sort_block",
+              "sort_rounds")
+
+    def stage_timing(self) -> dict[str, float]:
+        """Per-stage ms of the last eager launch (HIP events on the batch's stream)."""
+        arr = (ctypes.c_float * len(self.STAGES))()
+        C.check(C.lib().bulkx_batch_stage_timing(self._h, arr, len(self.STAGES)),
+                "bulkx_batch_stage_timing")
+        return dict(zip(self.STAGES, list(arr)))
+
     def close(self) -> None:
         if self._h:
             C.lib().bulkx_batch_destroy(self._h)

@@ -65,27 +65,61 @@ struct BatchJobDev {
   int64_t sort_tile_begin;
 };
 
+// Phase-0 work is split by op family; each family is one persistent kernel
+// compiled for its own register and LDS footprint (a single kernel over all
+// ops would run at the occupancy of the heaviest body).
+enum BatchFamily : int32_t {
+  FAM_SCAN_I32 = 0, FAM_SCAN_F32, FAM_RBK_I32, FAM_RBK_F32, FAM_RBK_I64, FAM_MERGE, FAM_SORT,
+  FAM_COUNT
+};
+constexpr int kBatchStages = FAM_COUNT + 1;  // the family kernels, then the sort merge rounds
+constexpr int kGridPartition = FAM_COUNT, kGridPass = FAM_COUNT + 1, kGridKinds = FAM_COUNT + 2;
+
 struct BatchDesc {
-  int64_t total_tiles0;
+  int64_t fam_begin[FAM_COUNT + 1];  // family f owns tiles [fam_begin[f], fam_begin[f+1])
   int64_t total_sort_tiles;
   int64_t num_status_words;
-  uint32_t counter0;
+  uint32_t counter[FAM_COUNT];
   uint32_t err;
-  uint32_t pass_counter[kBatchMaxPasses];
 };
 
-union BatchSmem {
-  ScanSmem<int32_t, OpPlus, kBatchBlock> scan_i;
-  ScanSmem<float, OpPlus, kBatchBlock> scan_f;
-  RbkSmem<int32_t, int32_t, OpPlus, kBatchBlock, kBatchRbkItems> rbk_i;
-  RbkSmem<int32_t, float, OpPlus, kBatchBlock, kBatchRbkItems> rbk_f;
-  RbkSmem<int32_t, int64_t, OpPlus, kBatchBlock, kBatchRbkItems> rbk_l;
-  MergeSmem<int32_t, uint32_t, true, kBatchBlock, kBatchMergeItems> merge_i;
-  MergeSmem<float, uint32_t, true, kBatchBlock, kBatchMergeItems> merge_f;
-  MergeSmem<uint32_t, uint32_t, true, kBatchBlock, kBatchMergeItems> merge_u;
-  SortSmem<int32_t, uint32_t, true, kBatchBlock, kBatchMergeItems> sort_i;
-  SortSmem<uint32_t, uint32_t, true, kBatchBlock, kBatchMergeItems> sort_u;
-  SortSmem<float, uint32_t, true, kBatchBlock, kBatchMergeItems> sort_f;
+template <int FAM>
+struct FamilySmem;
+template <>
+struct FamilySmem<FAM_SCAN_I32> {
+  ScanSmem<int32_t, OpPlus, kBatchBlock> s;
+};
+template <>
+struct FamilySmem<FAM_SCAN_F32> {
+  ScanSmem<float, OpPlus, kBatchBlock> s;
+};
+template <>
+struct FamilySmem<FAM_RBK_I32> {
+  RbkSmem<int32_t, int32_t, OpPlus, kBatchBlock, kBatchRbkItems> s;
+};
+template <>
+struct FamilySmem<FAM_RBK_F32> {
+  RbkSmem<int32_t, float, OpPlus, kBatchBlock, kBatchRbkItems> s;
+};
+template <>
+struct FamilySmem<FAM_RBK_I64> {
+  RbkSmem<int32_t, int64_t, OpPlus, kBatchBlock, kBatchRbkItems> s;
+};
+template <>
+struct FamilySmem<FAM_MERGE> {
+  union {
+    MergeSmem<int32_t, uint32_t, true, kBatchBlock, kBatchMergeItems> i;
+    MergeSmem<float, uint32_t, true, kBatchBlock, kBatchMergeItems> f;
+    MergeSmem<uint32_t, uint32_t, true, kBatchBlock, kBatchMergeItems> u;
+  };
+};
+template <>
+struct FamilySmem<FAM_SORT> {
+  union {
+    SortSmem<int32_t, uint32_t, true, kBatchBlock, kBatchMergeItems> i;
+    SortSmem<uint32_t, uint32_t, true, kBatchBlock, kBatchMergeItems> u;
+    SortSmem<float, uint32_t, true, kBatchBlock, kBatchMergeItems> f;
+  };
 };
 
 __global__ void batch_clear_kernel(BatchDesc* d, uint64_t* status) {
@@ -93,167 +127,189 @@ __global__ void batch_clear_kernel(BatchDesc* d, uint64_t* status) {
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nw;
        i += int64_t(gridDim.x) * blockDim.x)
     status[i] = 0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    d->counter0 = 0;
-    d->err = 0;
-  }
-  if (blockIdx.x == 0 && threadIdx.x < kBatchMaxPasses) d->pass_counter[threadIdx.x] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < FAM_COUNT) d->counter[threadIdx.x] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) d->err = 0;
 }
 
-template <class K>
-__device__ __forceinline__ void sort_block_job(BatchSmem& sm, const BatchJobDev& job, int64_t lt);
-
-template <class K>
-__device__ __forceinline__ SortSmem<K, uint32_t, true, kBatchBlock, kBatchMergeItems>& sort_smem(
-    BatchSmem& sm);
-template <>
-__device__ __forceinline__ SortSmem<int32_t, uint32_t, true, kBatchBlock, kBatchMergeItems>&
-sort_smem<int32_t>(BatchSmem& sm) { return sm.sort_i; }
-template <>
-__device__ __forceinline__ SortSmem<uint32_t, uint32_t, true, kBatchBlock, kBatchMergeItems>&
-sort_smem<uint32_t>(BatchSmem& sm) { return sm.sort_u; }
-template <>
-__device__ __forceinline__ SortSmem<float, uint32_t, true, kBatchBlock, kBatchMergeItems>&
-sort_smem<float>(BatchSmem& sm) { return sm.sort_f; }
-
-template <class K>
-__device__ __forceinline__ void sort_block_job(BatchSmem& sm, const BatchJobDev& job, int64_t lt) {
+template <class K, class SM>
+__device__ __forceinline__ void sort_block_job(SM& sm, const BatchJobDev& job, int64_t lt) {
   K* keys = static_cast<K*>(job.out0);
   uint32_t* vals = static_cast<uint32_t*>(job.out1);
   // an odd number of rounds starts in the temporary so the last lands in place
   K* dk = (job.passes & 1) ? static_cast<K*>(job.tmp_k) : keys;
   uint32_t* dv = (job.passes & 1) ? static_cast<uint32_t*>(job.tmp_v) : vals;
-  block_sort_tile<CmpLess, K, uint32_t, true, kBatchBlock, kBatchMergeItems>(
-      sort_smem<K>(sm), keys, vals, job.n0, dk, dv, lt);
+  block_sort_tile<CmpLess, K, uint32_t, true, kBatchBlock, kBatchMergeItems>(sm, keys, vals, job.n0,
+                                                                             dk, dv, lt);
 }
 
-__global__ __launch_bounds__(kBatchBlock) void batch_phase0_kernel(BatchDesc* d,
-                                                                   const BatchJobDev* jobs,
-                                                                   const uint32_t* tile_job) {
-  __shared__ BatchSmem sm;
-  __shared__ uint32_t s_t;
-  const int64_t total = d->total_tiles0;
-  while (true) {
-    __syncthreads();  // the previous tile's LDS traffic is over
-    if (threadIdx.x == 0) s_t = atomicAdd(&d->counter0, 1u);
-    __syncthreads();
-    const int64_t t = s_t;
-    if (t >= total) break;
-    const BatchJobDev& job = jobs[tile_job[t]];
-    const int64_t lt = t - job.tile_begin0;
-    const bool single = job.tiles0 == 1;
+// One phase-0 tile of a job of family FAM.
+template <int FAM>
+__device__ __forceinline__ void run_tile(FamilySmem<FAM>& sm, BatchDesc* d, const BatchJobDev& job,
+                                         int64_t lt) {
+  const bool single = job.tiles0 == 1;
+  if constexpr (FAM == FAM_SCAN_I32) {
+    if (job.op == OP_SCAN_I32)
+      scan_tile<int32_t, OpPlus, kBatchBlock, kBatchScanItems, false>(
+          sm.s, static_cast<const int32_t*>(job.in0), static_cast<int32_t*>(job.out0), job.n0, 0,
+          false, lt, job.status, &d->err, single, nullptr);
+    else
+      scan_tile<int32_t, OpPlus, kBatchBlock, kBatchScanItems, false, false>(
+          sm.s, static_cast<const int32_t*>(job.in0), nullptr, job.n0, 0, false, lt, job.status,
+          &d->err, single, static_cast<int32_t*>(job.out0));
+  } else if constexpr (FAM == FAM_SCAN_F32) {
+    if (job.op == OP_SCAN_F32)
+      scan_tile<float, OpPlus, kBatchBlock, kBatchScanItems, false>(
+          sm.s, static_cast<const float*>(job.in0), static_cast<float*>(job.out0), job.n0, 0.f,
+          false, lt, job.status, &d->err, single, nullptr);
+    else
+      scan_tile<float, OpPlus, kBatchBlock, kBatchScanItems, false, false>(
+          sm.s, static_cast<const float*>(job.in0), nullptr, job.n0, 0.f, false, lt, job.status,
+          &d->err, single, static_cast<float*>(job.out0));
+  } else if constexpr (FAM == FAM_RBK_I32) {
+    rbk_tile<int32_t, int32_t, OpPlus, kBatchBlock, kBatchRbkItems>(
+        sm.s, static_cast<const int32_t*>(job.in0), static_cast<const int32_t*>(job.in1), job.n0,
+        static_cast<int32_t*>(job.out0), static_cast<int32_t*>(job.out1),
+        static_cast<int64_t*>(job.out2), lt, job.status, &d->err, single);
+  } else if constexpr (FAM == FAM_RBK_F32) {
+    rbk_tile<int32_t, float, OpPlus, kBatchBlock, kBatchRbkItems>(
+        sm.s, static_cast<const int32_t*>(job.in0), static_cast<const float*>(job.in1), job.n0,
+        static_cast<int32_t*>(job.out0), static_cast<float*>(job.out1),
+        static_cast<int64_t*>(job.out2), lt, job.status, &d->err, single);
+  } else if constexpr (FAM == FAM_RBK_I64) {
+    rbk_tile<int32_t, int64_t, OpPlus, kBatchBlock, kBatchRbkItems>(
+        sm.s, static_cast<const int32_t*>(job.in0), static_cast<const int64_t*>(job.in1), job.n0,
+        static_cast<int32_t*>(job.out0), static_cast<int64_t*>(job.out1),
+        static_cast<int64_t*>(job.out2), lt, job.status, &d->err, single);
+  } else if constexpr (FAM == FAM_MERGE) {
+    if (job.op == OP_MERGE_I32)
+      merge_two_tile<CmpLess, int32_t, uint32_t, true, kBatchBlock, kBatchMergeItems>(
+          sm.i, static_cast<const int32_t*>(job.in0), static_cast<const uint32_t*>(job.in1),
+          job.n0, static_cast<const int32_t*>(job.in2), static_cast<const uint32_t*>(job.in3),
+          job.n1, static_cast<int32_t*>(job.out0), static_cast<uint32_t*>(job.out1), lt);
+    else
+      merge_two_tile<CmpLess, float, uint32_t, true, kBatchBlock, kBatchMergeItems>(
+          sm.f, static_cast<const float*>(job.in0), static_cast<const uint32_t*>(job.in1),
+          job.n0, static_cast<const float*>(job.in2), static_cast<const uint32_t*>(job.in3),
+          job.n1, static_cast<float*>(job.out0), static_cast<uint32_t*>(job.out1), lt);
+  } else {
     switch (job.op) {
-      case OP_SCAN_I32:
-        scan_tile<int32_t, OpPlus, kBatchBlock, kBatchScanItems, false>(
-            sm.scan_i, static_cast<const int32_t*>(job.in0), static_cast<int32_t*>(job.out0), job.n0,
-            0, false, lt, job.status, &d->err, single, nullptr);
-        break;
-      case OP_SCAN_F32:
-        scan_tile<float, OpPlus, kBatchBlock, kBatchScanItems, false>(
-            sm.scan_f, static_cast<const float*>(job.in0), static_cast<float*>(job.out0), job.n0,
-            0.f, false, lt, job.status, &d->err, single, nullptr);
-        break;
-      case OP_REDUCE_I32:
-        scan_tile<int32_t, OpPlus, kBatchBlock, kBatchScanItems, false, false>(
-            sm.scan_i, static_cast<const int32_t*>(job.in0), nullptr, job.n0, 0, false, lt,
-            job.status, &d->err, single, static_cast<int32_t*>(job.out0));
-        break;
-      case OP_REDUCE_F32:
-        scan_tile<float, OpPlus, kBatchBlock, kBatchScanItems, false, false>(
-            sm.scan_f, static_cast<const float*>(job.in0), nullptr, job.n0, 0.f, false, lt,
-            job.status, &d->err, single, static_cast<float*>(job.out0));
-        break;
-      case OP_RBK_I32_I32:
-        rbk_tile<int32_t, int32_t, OpPlus, kBatchBlock, kBatchRbkItems>(
-            sm.rbk_i, static_cast<const int32_t*>(job.in0), static_cast<const int32_t*>(job.in1),
-            job.n0, static_cast<int32_t*>(job.out0), static_cast<int32_t*>(job.out1),
-            static_cast<int64_t*>(job.out2), lt, job.status, &d->err, single);
-        break;
-      case OP_RBK_I32_F32:
-        rbk_tile<int32_t, float, OpPlus, kBatchBlock, kBatchRbkItems>(
-            sm.rbk_f, static_cast<const int32_t*>(job.in0), static_cast<const float*>(job.in1),
-            job.n0, static_cast<int32_t*>(job.out0), static_cast<float*>(job.out1),
-            static_cast<int64_t*>(job.out2), lt, job.status, &d->err, single);
-        break;
-      case OP_RBK_I32_I64:
-        rbk_tile<int32_t, int64_t, OpPlus, kBatchBlock, kBatchRbkItems>(
-            sm.rbk_l, static_cast<const int32_t*>(job.in0), static_cast<const int64_t*>(job.in1),
-            job.n0, static_cast<int32_t*>(job.out0), static_cast<int64_t*>(job.out1),
-            static_cast<int64_t*>(job.out2), lt, job.status, &d->err, single);
-        break;
-      case OP_MERGE_I32:
-        merge_two_tile<CmpLess, int32_t, uint32_t, true, kBatchBlock, kBatchMergeItems>(
-            sm.merge_i, static_cast<const int32_t*>(job.in0), static_cast<const uint32_t*>(job.in1),
-            job.n0, static_cast<const int32_t*>(job.in2), static_cast<const uint32_t*>(job.in3),
-            job.n1, static_cast<int32_t*>(job.out0), static_cast<uint32_t*>(job.out1), lt);
-        break;
-      case OP_MERGE_F32:
-        merge_two_tile<CmpLess, float, uint32_t, true, kBatchBlock, kBatchMergeItems>(
-            sm.merge_f, static_cast<const float*>(job.in0), static_cast<const uint32_t*>(job.in1),
-            job.n0, static_cast<const float*>(job.in2), static_cast<const uint32_t*>(job.in3),
-            job.n1, static_cast<float*>(job.out0), static_cast<uint32_t*>(job.out1), lt);
-        break;
-      case OP_SORT_I32: sort_block_job<int32_t>(sm, job, lt); break;
-      case OP_SORT_U32: sort_block_job<uint32_t>(sm, job, lt); break;
-      case OP_SORT_F32: sort_block_job<float>(sm, job, lt); break;
-      default: break;
+      case OP_SORT_I32: sort_block_job<int32_t>(sm.i, job, lt); break;
+      case OP_SORT_U32: sort_block_job<uint32_t>(sm.u, job, lt); break;
+      default: sort_block_job<float>(sm.f, job, lt); break;
     }
   }
 }
 
-template <class K>
-__device__ __forceinline__ MergeSmem<K, uint32_t, true, kBatchBlock, kBatchMergeItems>& pass_smem(
-    BatchSmem& sm);
-template <>
-__device__ __forceinline__ MergeSmem<int32_t, uint32_t, true, kBatchBlock, kBatchMergeItems>&
-pass_smem<int32_t>(BatchSmem& sm) { return sm.merge_i; }
-template <>
-__device__ __forceinline__ MergeSmem<float, uint32_t, true, kBatchBlock, kBatchMergeItems>&
-pass_smem<float>(BatchSmem& sm) { return sm.merge_f; }
-template <>
-__device__ __forceinline__ MergeSmem<uint32_t, uint32_t, true, kBatchBlock, kBatchMergeItems>&
-pass_smem<uint32_t>(BatchSmem& sm) { return sm.merge_u; }
-
-template <class K>
-__device__ __forceinline__ void sort_pass_job(BatchSmem& sm, const BatchJobDev& job, int pass,
-                                              int64_t lt) {
-  K* orig_k = static_cast<K*>(job.out0);
-  uint32_t* orig_v = static_cast<uint32_t*>(job.out1);
-  K* tmp_k = static_cast<K*>(job.tmp_k);
-  uint32_t* tmp_v = static_cast<uint32_t*>(job.tmp_v);
-  // the block sort wrote to "first"; rounds alternate from there
-  const bool first_is_tmp = (job.passes & 1) != 0;
-  const bool src_is_tmp = first_is_tmp ^ ((pass & 1) != 0);
-  const K* sk = src_is_tmp ? tmp_k : orig_k;
-  const uint32_t* sv = src_is_tmp ? tmp_v : orig_v;
-  K* dk = src_is_tmp ? orig_k : tmp_k;
-  uint32_t* dv = src_is_tmp ? orig_v : tmp_v;
-  merge_pass_tile<CmpLess, K, uint32_t, true, kBatchBlock, kBatchMergeItems>(
-      pass_smem<K>(sm), sk, sv, job.n0, kBatchMergeTile << pass, dk, dv, lt);
+// Phase-0 kernel of one family: one workgroup per tile, launched with one
+// workgroup per tile of the family (the host resizes the grid, also inside
+// the captured graph, whenever the job list changes).  Tiles are claimed
+// from an atomic counter in queue order, so a tile's look-back predecessors
+// always belong to workgroups that are already running or done.  (A
+// persistent loop here would keep the tile body's per-lane address math live
+// across iterations and cost ~40 registers per lane.)
+template <int FAM>
+__global__ __launch_bounds__(kBatchBlock) void batch_phase0_kernel(
+    BatchDesc* d, const BatchJobDev* __restrict__ jobs, const uint32_t* __restrict__ tile_job) {
+  __shared__ FamilySmem<FAM> sm;
+  __shared__ uint32_t s_t;
+  const int64_t begin = d->fam_begin[FAM];
+  const int64_t count = d->fam_begin[FAM + 1] - begin;
+  if (threadIdx.x == 0) s_t = atomicAdd(&d->counter[FAM], 1u);
+  __syncthreads();
+  const int64_t t = wave_uniform(s_t);
+  if (t >= count) return;
+  const BatchJobDev& job = jobs[tile_job[begin + t]];
+  run_tile<FAM>(sm, d, job, begin + t - job.tile_begin0);
 }
 
-__global__ __launch_bounds__(kBatchBlock) void batch_pass_kernel(BatchDesc* d,
-                                                                 const BatchJobDev* jobs,
-                                                                 const uint32_t* sort_tile_job,
-                                                                 int pass) {
-  __shared__ BatchSmem sm;
-  __shared__ uint32_t s_t;
-  const int64_t total = d->total_sort_tiles;
-  while (true) {
-    __syncthreads();
-    if (threadIdx.x == 0) s_t = atomicAdd(&d->pass_counter[pass], 1u);
-    __syncthreads();
-    const int64_t t = s_t;
-    if (t >= total) break;
-    const BatchJobDev& job = jobs[sort_tile_job[t]];
-    if (pass >= job.passes) continue;
-    const int64_t lt = t - job.sort_tile_begin;
-    switch (job.op) {
-      case OP_SORT_I32: sort_pass_job<int32_t>(sm, job, pass, lt); break;
-      case OP_SORT_U32: sort_pass_job<uint32_t>(sm, job, pass, lt); break;
-      case OP_SORT_F32: sort_pass_job<float>(sm, job, pass, lt); break;
-      default: break;
-    }
+template <class K>
+__device__ __forceinline__ MergeSmem<K, uint32_t, true, kBatchBlock, kBatchMergeItems>& merge_smem(
+    FamilySmem<FAM_MERGE>& sm);
+template <>
+__device__ __forceinline__ MergeSmem<int32_t, uint32_t, true, kBatchBlock, kBatchMergeItems>&
+merge_smem<int32_t>(FamilySmem<FAM_MERGE>& sm) { return sm.i; }
+template <>
+__device__ __forceinline__ MergeSmem<float, uint32_t, true, kBatchBlock, kBatchMergeItems>&
+merge_smem<float>(FamilySmem<FAM_MERGE>& sm) { return sm.f; }
+template <>
+__device__ __forceinline__ MergeSmem<uint32_t, uint32_t, true, kBatchBlock, kBatchMergeItems>&
+merge_smem<uint32_t>(FamilySmem<FAM_MERGE>& sm) { return sm.u; }
+
+// Source and destination of a sort job's merge round `pass`: the block sort
+// wrote to "first" (the temporary for an odd number of rounds, so the last
+// round lands in place) and rounds alternate from there.
+template <class K>
+struct PassBuffers {
+  const K* sk;
+  const uint32_t* sv;
+  K* dk;
+  uint32_t* dv;
+  __device__ PassBuffers(const BatchJobDev& job, int pass) {
+    const bool src_is_tmp = ((job.passes & 1) != 0) ^ ((pass & 1) != 0);
+    K* ok = static_cast<K*>(job.out0);
+    uint32_t* ov = static_cast<uint32_t*>(job.out1);
+    K* tk = static_cast<K*>(job.tmp_k);
+    uint32_t* tv = static_cast<uint32_t*>(job.tmp_v);
+    sk = src_is_tmp ? tk : ok;
+    sv = src_is_tmp ? tv : ov;
+    dk = src_is_tmp ? ok : tk;
+    dv = src_is_tmp ? ov : tv;
+  }
+};
+
+// Merge paths of the first diagonal of every sort tile of round `pass`,
+// one wavefront per tile (tiles of jobs that need fewer rounds are skipped).
+__global__ __launch_bounds__(256) void batch_partition_kernel(
+    BatchDesc* d, const BatchJobDev* __restrict__ jobs, const uint32_t* __restrict__ sort_tile_job,
+    int64_t* __restrict__ splits, int pass) {
+  const int64_t t = int64_t(blockIdx.x) * (256 / kWave) + threadIdx.x / kWave;
+  if (t >= d->total_sort_tiles) return;  // uniform per wavefront
+  const BatchJobDev& job = jobs[sort_tile_job[t]];
+  if (pass >= job.passes) return;
+  const int64_t lt = t - job.sort_tile_begin;
+  const int64_t width = kBatchMergeTile << pass;
+  int64_t r;
+  switch (job.op) {
+    case OP_SORT_I32:
+      r = pass_split<CmpLess, int32_t, kBatchMergeTile>(PassBuffers<int32_t>(job, pass).sk,
+                                                        job.n0, width, lt);
+      break;
+    case OP_SORT_U32:
+      r = pass_split<CmpLess, uint32_t, kBatchMergeTile>(PassBuffers<uint32_t>(job, pass).sk,
+                                                         job.n0, width, lt);
+      break;
+    default:
+      r = pass_split<CmpLess, float, kBatchMergeTile>(PassBuffers<float>(job, pass).sk, job.n0,
+                                                      width, lt);
+      break;
+  }
+  if (lane_id() == 0) splits[t] = r;
+}
+
+template <class K>
+__device__ __forceinline__ void sort_pass_job(FamilySmem<FAM_MERGE>& sm, const BatchJobDev& job,
+                                              int pass, int64_t lt, const int64_t* splits) {
+  const PassBuffers<K> pb(job, pass);
+  merge_pass_tile<CmpLess, K, uint32_t, true, kBatchBlock, kBatchMergeItems>(
+      merge_smem<K>(sm), pb.sk, pb.sv, job.n0, kBatchMergeTile << pass, pb.dk, pb.dv, lt,
+      splits + job.sort_tile_begin);
+}
+
+// Merge round `pass` of every sort job: one workgroup per sort tile (the
+// host sizes the grid to the job list's sort tiles).
+__global__ __launch_bounds__(kBatchBlock) void batch_pass_kernel(
+    BatchDesc* d, const BatchJobDev* __restrict__ jobs, const uint32_t* __restrict__ sort_tile_job,
+    const int64_t* __restrict__ splits, int pass) {
+  __shared__ FamilySmem<FAM_MERGE> sm;
+  const int64_t t = blockIdx.x;
+  if (t >= d->total_sort_tiles) return;
+  const BatchJobDev& job = jobs[sort_tile_job[t]];
+  if (pass >= job.passes) return;
+  const int64_t lt = t - job.sort_tile_begin;
+  switch (job.op) {
+    case OP_SORT_I32: sort_pass_job<int32_t>(sm, job, pass, lt, splits); break;
+    case OP_SORT_U32: sort_pass_job<uint32_t>(sm, job, pass, lt, splits); break;
+    default: sort_pass_job<float>(sm, job, pass, lt, splits); break;
   }
 }
 
@@ -291,6 +347,18 @@ static int batch_op(const bulkx_job& j) {
   }
 }
 
+static int host_family(int op) {
+  switch (op) {
+    case OP_SCAN_I32: case OP_REDUCE_I32: return FAM_SCAN_I32;
+    case OP_SCAN_F32: case OP_REDUCE_F32: return FAM_SCAN_F32;
+    case OP_RBK_I32_I32: return FAM_RBK_I32;
+    case OP_RBK_I32_F32: return FAM_RBK_F32;
+    case OP_RBK_I32_I64: return FAM_RBK_I64;
+    case OP_MERGE_I32: case OP_MERGE_F32: return FAM_MERGE;
+    default: return FAM_SORT;
+  }
+}
+
 // Workspace one plain entry point needs for job j.
 static size_t per_call_workspace(const bulkx_job& j) {
   switch (j.kind) {
@@ -314,12 +382,13 @@ struct bulkx_batch {
   int64_t max_elements = 0;
   int64_t max_tiles = 0;
   int max_passes = 0;
-  int grid = 0;
+  int64_t fam_tiles[FAM_COUNT] = {};  // phase-0 grid of each family = its tile count
   // device
   BatchDesc* d_desc = nullptr;
   BatchJobDev* d_jobs = nullptr;
   uint32_t* d_tile_job = nullptr;
   uint32_t* d_sort_tile_job = nullptr;
+  int64_t* d_splits = nullptr;  // merge-path splits of the current merge round
   uint64_t* d_status = nullptr;
   char* d_tmp = nullptr;
   size_t status_words_cap = 0;
@@ -335,29 +404,58 @@ struct bulkx_batch {
   std::vector<bulkx_job> jobs;
   int num_jobs = 0;
   int64_t total_tiles0 = 0, total_sort_tiles = 0;
-  bool pending_upload = false;
-  // phase timing of eager launches
-  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  // stage timing of eager launches: ev[f]..ev[f+1] brackets family f,
+  // ev[FAM_COUNT]..ev[FAM_COUNT+1] the sort merge rounds
+  hipEvent_t ev[kBatchStages + 1] = {};
   bool timed = false;
-  // graph
+  // graph, and its kernel nodes whose grid follows the job list
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  std::vector<std::pair<hipGraphNode_t, int>> sized_nodes;  // (node, grid kind)
+  unsigned graph_grid[kGridKinds] = {};
 };
+
+// Grid of each job-list-sized launch: the phase-0 families (one workgroup per
+// tile), the merge-round partition (one wavefront per sort tile) and the
+// merge rounds (one workgroup per sort tile).  Never 0: an idle launch runs
+// one workgroup that finds no work.
+static unsigned batch_grid(const bulkx_batch* b, int kind) {
+  int64_t g;
+  if (kind < FAM_COUNT) g = b->fam_tiles[kind];
+  else if (kind == kGridPartition) g = (b->total_sort_tiles + 3) / 4;
+  else g = b->total_sort_tiles;
+  return unsigned(g > 0 ? g : 1);
+}
+
+using Phase0Kernel = void (*)(BatchDesc*, const BatchJobDev*, const uint32_t*);
+static const Phase0Kernel kPhase0[FAM_COUNT] = {
+    batch_phase0_kernel<FAM_SCAN_I32>, batch_phase0_kernel<FAM_SCAN_F32>,
+    batch_phase0_kernel<FAM_RBK_I32>,  batch_phase0_kernel<FAM_RBK_F32>,
+    batch_phase0_kernel<FAM_RBK_I64>,  batch_phase0_kernel<FAM_MERGE>,
+    batch_phase0_kernel<FAM_SORT>};
 
 static int batch_enqueue_kernels(bulkx_batch* b, hipStream_t s, bool timed) {
   batch_clear_kernel<<<256, 256, 0, s>>>(b->d_desc, b->d_status);
   BULKX_LAUNCHED();
-  if (timed) BULKX_HIP(hipEventRecord(b->ev[0], s));
-  batch_phase0_kernel<<<b->grid, kBatchBlock, 0, s>>>(b->d_desc, b->d_jobs, b->d_tile_job);
-  BULKX_LAUNCHED();
-  if (timed) BULKX_HIP(hipEventRecord(b->ev[1], s));
-  for (int p = 0; p < b->max_passes; ++p) {
-    batch_pass_kernel<<<b->grid, kBatchBlock, 0, s>>>(b->d_desc, b->d_jobs, b->d_sort_tile_job, p);
+  for (int f = 0; f < FAM_COUNT; ++f) {
+    if (timed) BULKX_HIP(hipEventRecord(b->ev[f], s));
+    kPhase0[f]<<<batch_grid(b, f), kBatchBlock, 0, s>>>(b->d_desc, b->d_jobs, b->d_tile_job);
     BULKX_LAUNCHED();
   }
-  if (timed) BULKX_HIP(hipEventRecord(b->ev[2], s));
+  if (timed) BULKX_HIP(hipEventRecord(b->ev[FAM_COUNT], s));
+  for (int p = 0; p < b->max_passes; ++p) {
+    batch_partition_kernel<<<batch_grid(b, kGridPartition), 256, 0, s>>>(
+        b->d_desc, b->d_jobs, b->d_sort_tile_job, b->d_splits, p);
+    BULKX_LAUNCHED();
+    batch_pass_kernel<<<batch_grid(b, kGridPass), kBatchBlock, 0, s>>>(
+        b->d_desc, b->d_jobs, b->d_sort_tile_job, b->d_splits, p);
+    BULKX_LAUNCHED();
+  }
+  if (timed) BULKX_HIP(hipEventRecord(b->ev[FAM_COUNT + 1], s));
   return BULKX_SUCCESS;
 }
+
+static void batch_drop_graph(bulkx_batch* b);
 
 extern "C" int bulkx_batch_create(int max_jobs, int64_t max_elements, void* stream,
                                   bulkx_batch** out) {
@@ -375,11 +473,6 @@ extern "C" int bulkx_batch_create(int max_jobs, int64_t max_elements, void* stre
     delete b;
     return BULKX_ERR_INVALID_ARGUMENT;
   }
-  int per_cu = 0;
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_phase0_kernel,
-                                                              kBatchBlock, 0);
-  if (e != hipSuccess || per_cu < 1) per_cu = 1;
-  b->grid = per_cu * props.num_cus;
   b->status_words_cap = size_t(b->max_tiles) * kStatusWordsPerTile;
   b->tmp_bytes_cap = 0;  // sort ping-pong space grows on demand in set_jobs
   b->ws_bytes = 256;  // the per-call workspace grows on demand in set_jobs
@@ -404,6 +497,7 @@ extern "C" int bulkx_batch_create(int max_jobs, int64_t max_elements, void* stre
   dmalloc(reinterpret_cast<void**>(&b->d_jobs), sizeof(BatchJobDev) * max_jobs);
   dmalloc(reinterpret_cast<void**>(&b->d_tile_job), sizeof(uint32_t) * b->max_tiles);
   dmalloc(reinterpret_cast<void**>(&b->d_sort_tile_job), sizeof(uint32_t) * b->max_tiles);
+  dmalloc(reinterpret_cast<void**>(&b->d_splits), sizeof(int64_t) * b->max_tiles);
   dmalloc(reinterpret_cast<void**>(&b->d_status), sizeof(uint64_t) * b->status_words_cap);
   dmalloc(reinterpret_cast<void**>(&b->d_ws), b->ws_bytes);
   hmalloc(reinterpret_cast<void**>(&b->h_desc), sizeof(BatchDesc));
@@ -416,7 +510,7 @@ extern "C" int bulkx_batch_create(int max_jobs, int64_t max_elements, void* stre
   }
   memset(b->h_desc, 0, sizeof(BatchDesc));
   hipError_t r = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking);
-  for (int i = 0; i < 3 && r == hipSuccess; ++i) r = hipEventCreate(&b->ev[i]);
+  for (int i = 0; i <= kBatchStages && r == hipSuccess; ++i) r = hipEventCreate(&b->ev[i]);
   if (r != hipSuccess) {
     set_last_error("hipStreamCreateWithFlags", r);
     bulkx_batch_destroy(b);
@@ -435,13 +529,13 @@ extern "C" int bulkx_batch_create(int max_jobs, int64_t max_elements, void* stre
 extern "C" int bulkx_batch_destroy(bulkx_batch* b) {
   if (!b) return BULKX_SUCCESS;
   (void)hipStreamSynchronize(b->stream);
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  batch_drop_graph(b);
   if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
   for (hipEvent_t e : b->ev)
     if (e) (void)hipEventDestroy(e);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
   for (void* p : {static_cast<void*>(b->d_desc), static_cast<void*>(b->d_jobs),
                   static_cast<void*>(b->d_tile_job), static_cast<void*>(b->d_sort_tile_job),
+                  static_cast<void*>(b->d_splits),
                   static_cast<void*>(b->d_status), static_cast<void*>(b->d_tmp),
                   static_cast<void*>(b->d_ws)})
     if (p) (void)hipFree(p);
@@ -515,7 +609,34 @@ extern "C" int bulkx_batch_set_jobs(bulkx_batch* b, const bulkx_job* jobs, int n
   }
   b->jobs.assign(jobs, jobs + num_jobs);
   b->num_jobs = num_jobs;
-  int64_t t0 = 0, ts = 0;
+  // Phase-0 tiles are grouped by family, jobs in list order within a family.
+  auto job_tiles = [](int op, int64_t n) {
+    int64_t t;
+    if (op <= OP_REDUCE_F32) t = (n + kBatchScanTile - 1) / kBatchScanTile;
+    else if (op <= OP_RBK_I32_I64) t = (n + kBatchRbkTile - 1) / kBatchRbkTile;
+    else t = (n + kBatchMergeTile - 1) / kBatchMergeTile;
+    return t < 1 ? int64_t(1) : t;  // an empty reduce / reduce_by_key still writes its result
+  };
+  auto job_n = [](const bulkx_job& j, int op) {
+    return j.n0 + (op == OP_MERGE_I32 || op == OP_MERGE_F32 ? j.n1 : 0);
+  };
+  auto skipped = [](const bulkx_job& j, int64_t n) {
+    return n == 0 && !(j.kind == BULKX_JOB_REDUCE || j.kind == BULKX_JOB_REDUCE_BY_KEY);
+  };
+  int64_t fam_tiles[FAM_COUNT] = {};
+  for (int i = 0; i < num_jobs; ++i) {
+    const int op = batch_op(jobs[i]);
+    const int64_t n = job_n(jobs[i], op);
+    if (!skipped(jobs[i], n)) fam_tiles[host_family(op)] += job_tiles(op, n);
+  }
+  int64_t cursor[FAM_COUNT];
+  memset(b->h_desc, 0, sizeof(BatchDesc));
+  b->h_desc->fam_begin[0] = 0;
+  for (int f = 0; f < FAM_COUNT; ++f) {
+    cursor[f] = b->h_desc->fam_begin[f];
+    b->h_desc->fam_begin[f + 1] = b->h_desc->fam_begin[f] + fam_tiles[f];
+  }
+  int64_t ts = 0;
   size_t sw = 0, tb = 0;
   for (int i = 0; i < num_jobs; ++i) {
     const bulkx_job& j = jobs[i];
@@ -532,20 +653,16 @@ extern "C" int bulkx_batch_set_jobs(bulkx_batch* b, const bulkx_job* jobs, int n
     dj.out0 = j.out0;
     dj.out1 = j.out1;
     dj.out2 = j.out2;
-    const int64_t n = j.n0 + (op == OP_MERGE_I32 || op == OP_MERGE_F32 ? j.n1 : 0);
-    int64_t t;
-    if (op <= OP_REDUCE_F32) t = (n + kBatchScanTile - 1) / kBatchScanTile;
-    else if (op <= OP_RBK_I32_I64) t = (n + kBatchRbkTile - 1) / kBatchRbkTile;
-    else t = (n + kBatchMergeTile - 1) / kBatchMergeTile;
-    const bool chained = op <= OP_RBK_I32_I64;
-    if (n == 0 && !(j.kind == BULKX_JOB_REDUCE || j.kind == BULKX_JOB_REDUCE_BY_KEY)) {
+    const int64_t n = job_n(j, op);
+    if (skipped(j, n)) {
       dj.op = OP_NONE;
-      dj.tiles0 = 0;
       continue;
     }
-    if (t < 1) t = 1;
+    const int64_t t = job_tiles(op, n);
+    const int f = host_family(op);
+    const bool chained = op <= OP_RBK_I32_I64;
     dj.tiles0 = t;
-    dj.tile_begin0 = t0;
+    dj.tile_begin0 = cursor[f];
     if (chained && t > 1) {
       dj.status = b->d_status + sw;
       sw += size_t(t) * kStatusWordsPerTile;
@@ -562,11 +679,11 @@ extern "C" int bulkx_batch_set_jobs(bulkx_batch* b, const bulkx_job* jobs, int n
       for (int64_t k = 0; k < t; ++k) b->h_sort_tile_job[ts + k] = uint32_t(i);
       ts += t;
     }
-    for (int64_t k = 0; k < t; ++k) b->h_tile_job[t0 + k] = uint32_t(i);
-    t0 += t;
+    for (int64_t k = 0; k < t; ++k) b->h_tile_job[cursor[f] + k] = uint32_t(i);
+    cursor[f] += t;
   }
-  memset(b->h_desc, 0, sizeof(BatchDesc));
-  b->h_desc->total_tiles0 = t0;
+  const int64_t t0 = b->h_desc->fam_begin[FAM_COUNT];
+  for (int f = 0; f < FAM_COUNT; ++f) b->fam_tiles[f] = fam_tiles[f];
   b->h_desc->total_sort_tiles = ts;
   b->h_desc->num_status_words = int64_t(sw);
   b->total_tiles0 = t0;
@@ -584,25 +701,84 @@ extern "C" int bulkx_batch_set_jobs(bulkx_batch* b, const bulkx_job* jobs, int n
   return BULKX_SUCCESS;
 }
 
+static void batch_drop_graph(bulkx_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->sized_nodes.clear();
+}
+
+// Records clear + phase-0 + merge rounds once on the private capture stream
+// and finds the phase-0 kernel nodes, whose grids follow the job list.
+static int batch_capture(bulkx_batch* b) {
+  BULKX_HIP(hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal));
+  int st = batch_enqueue_kernels(b, b->capture_stream, false);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(b->capture_stream, &g);
+  if (st != BULKX_SUCCESS) {
+    if (g) (void)hipGraphDestroy(g);
+    return st;
+  }
+  if (e != hipSuccess) {
+    set_last_error("hipStreamEndCapture", e);
+    return BULKX_ERR_HIP;
+  }
+  b->graph = g;
+  size_t num = 0;
+  BULKX_HIP(hipGraphGetNodes(g, nullptr, &num));
+  std::vector<hipGraphNode_t> nodes(num);
+  BULKX_HIP(hipGraphGetNodes(g, nodes.data(), &num));
+  for (hipGraphNode_t node : nodes) {
+    hipGraphNodeType ty;
+    BULKX_HIP(hipGraphNodeGetType(node, &ty));
+    if (ty != hipGraphNodeTypeKernel) continue;
+    hipKernelNodeParams kp;
+    BULKX_HIP(hipGraphKernelNodeGetParams(node, &kp));
+    int kind = -1;
+    for (int f = 0; f < FAM_COUNT; ++f)
+      if (kp.func == reinterpret_cast<void*>(kPhase0[f])) kind = f;
+    if (kp.func == reinterpret_cast<void*>(batch_partition_kernel)) kind = kGridPartition;
+    if (kp.func == reinterpret_cast<void*>(batch_pass_kernel)) kind = kGridPass;
+    if (kind >= 0) b->sized_nodes.emplace_back(node, kind);
+  }
+  BULKX_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  for (int k = 0; k < kGridKinds; ++k) b->graph_grid[k] = batch_grid(b, k);
+  return BULKX_SUCCESS;
+}
+
 extern "C" int bulkx_batch_launch(bulkx_batch* b, int use_graph) {
   if (!b) return BULKX_ERR_INVALID_ARGUMENT;
   if (!use_graph) {
     b->timed = true;
     return batch_enqueue_kernels(b, b->stream, true);
   }
-  if (!b->exec) {
-    BULKX_HIP(hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal));
-    int st = batch_enqueue_kernels(b, b->capture_stream, false);
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(b->capture_stream, &g);
-    if (st != BULKX_SUCCESS) return st;
-    if (e != hipSuccess) {
-      set_last_error("hipStreamEndCapture", e);
-      return BULKX_ERR_HIP;
+  if (b->exec) {  // resize the job-list-sized grids in place
+    bool stale[kGridKinds];
+    bool any = false;
+    for (int k = 0; k < kGridKinds; ++k) {
+      stale[k] = b->graph_grid[k] != batch_grid(b, k);
+      any |= stale[k];
     }
-    b->graph = g;
-    BULKX_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    hipError_t e = hipSuccess;
+    for (const auto& nk : b->sized_nodes) {
+      if (!any || e != hipSuccess) break;
+      if (!stale[nk.second]) continue;
+      hipKernelNodeParams kp;
+      e = hipGraphKernelNodeGetParams(nk.first, &kp);
+      if (e == hipSuccess) {
+        kp.gridDim = dim3(batch_grid(b, nk.second));
+        e = hipGraphExecKernelNodeSetParams(b->exec, nk.first, &kp);
+      }
+    }
+    if (e != hipSuccess) {  // a node could not be updated: record the graph again
+      (void)hipGetLastError();
+      batch_drop_graph(b);
+    } else {
+      for (int k = 0; k < kGridKinds; ++k) b->graph_grid[k] = batch_grid(b, k);
+    }
   }
+  if (!b->exec) BULKX_TRY(batch_capture(b));
   BULKX_HIP(hipGraphLaunch(b->exec, b->stream));
   return BULKX_SUCCESS;
 }
@@ -643,11 +819,23 @@ extern "C" int bulkx_batch_launch_per_call(bulkx_batch* b) {
 extern "C" int bulkx_batch_last_timing(bulkx_batch* b, float* phase0_ms, float* passes_ms) {
   if (!b) return BULKX_ERR_INVALID_ARGUMENT;
   if (!b->timed) return BULKX_ERR_NOT_READY;
-  BULKX_HIP(hipEventSynchronize(b->ev[2]));
+  BULKX_HIP(hipEventSynchronize(b->ev[kBatchStages]));
   float a = 0.f, c = 0.f;
-  BULKX_HIP(hipEventElapsedTime(&a, b->ev[0], b->ev[1]));
-  BULKX_HIP(hipEventElapsedTime(&c, b->ev[1], b->ev[2]));
+  BULKX_HIP(hipEventElapsedTime(&a, b->ev[0], b->ev[FAM_COUNT]));
+  BULKX_HIP(hipEventElapsedTime(&c, b->ev[FAM_COUNT], b->ev[kBatchStages]));
   if (phase0_ms) *phase0_ms = a;
   if (passes_ms) *passes_ms = c;
+  return BULKX_SUCCESS;
+}
+
+extern "C" int bulkx_batch_stage_timing(bulkx_batch* b, float* stage_ms, int num_stages) {
+  if (!b || !stage_ms || num_stages < 0) return BULKX_ERR_INVALID_ARGUMENT;
+  if (!b->timed) return BULKX_ERR_NOT_READY;
+  BULKX_HIP(hipEventSynchronize(b->ev[kBatchStages]));
+  for (int k = 0; k < num_stages; ++k) {
+    float ms = 0.f;
+    if (k < kBatchStages) BULKX_HIP(hipEventElapsedTime(&ms, b->ev[k], b->ev[k + 1]));
+    stage_ms[k] = ms;
+  }
   return BULKX_SUCCESS;
 }

@@ -198,6 +198,13 @@ __device__ __forceinline__ T readlane(T x, int lane) {
   return bitcast<T>(c);
 }
 
+// A value every lane of the wavefront holds (e.g. a tile index broadcast
+// through LDS), moved to a scalar register so that everything derived from it
+// (job records, base addresses) is computed once per wave on the scalar unit.
+__device__ __forceinline__ uint32_t wave_uniform(uint32_t x) {
+  return uint32_t(__builtin_amdgcn_readfirstlane(int(x)));
+}
+
 // Inclusive scan across the 64 lanes of a wavefront: 4 row shifts inside
 // each 16-lane row, then two row broadcasts join the rows.
 template <class Op, class T>

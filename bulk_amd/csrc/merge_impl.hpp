@@ -116,10 +116,28 @@ __device__ __forceinline__ void merge_tile(const K* a, const V* va, int na, cons
   na = na < 0 ? 0 : (na > TILE ? TILE : na);
   nb = nb < 0 ? 0 : (nb > TILE - na ? TILE - na : nb);
   const int cnt = na + nb;
-  for (int j = threadIdx.x; j < cnt; j += BLOCK) {
-    const bool fa = j < na;
-    s_k[j] = fa ? a[j] : b[j - na];
-    if constexpr (HAS_V) s_v[j] = fa ? va[j] : vb[j - na];
+  // Stage A ++ B: all loads of the thread are issued before the first LDS
+  // write, so they are in flight together.
+  {
+    K lk[ITEMS];
+    V lv[ITEMS];
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const int j = threadIdx.x + i * BLOCK;
+      if (j < cnt) {
+        const bool fa = j < na;
+        lk[i] = __builtin_nontemporal_load(fa ? a + j : b + (j - na));
+        if constexpr (HAS_V) lv[i] = __builtin_nontemporal_load(fa ? va + j : vb + (j - na));
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const int j = threadIdx.x + i * BLOCK;
+      if (j < cnt) {
+        s_k[j] = lk[i];
+        if constexpr (HAS_V) s_v[j] = lv[i];
+      }
+    }
   }
   __syncthreads();
   const int d = threadIdx.x * ITEMS < cnt ? threadIdx.x * ITEMS : cnt;

@@ -303,7 +303,7 @@ __global__ __launch_bounds__(BLOCK) void reduce_by_key_kernel(
   if (!single_tile) {
     if (threadIdx.x == 0) s_tile = atomicAdd(header, 1u);
     __syncthreads();
-    tile = s_tile;
+    tile = wave_uniform(s_tile);
   }
   rbk_tile<K, V, Op, BLOCK, ITEMS>(sm, keys, vals, n, kout, vout, num_runs, tile, words,
                                    header + 1, single_tile);

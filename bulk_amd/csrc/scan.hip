@@ -39,7 +39,7 @@ __global__ __launch_bounds__(BLOCK) void scan_kernel(const T* in, T* out, int64_
   if (!single_tile) {
     if (threadIdx.x == 0) s_tile = atomicAdd(header, 1u);
     __syncthreads();
-    tile = s_tile;
+    tile = wave_uniform(s_tile);
   }
   scan_tile<T, Op, BLOCK, ITEMS, EXCLUSIVE>(sm, in, out, n, init, has_init, tile, status_words,
                                             header + 1, single_tile, nullptr);

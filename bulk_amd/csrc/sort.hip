@@ -9,7 +9,7 @@ size_t sort_workspace_bytes(int kdtype, int vdtype, int64_t n) {
   const int64_t tile = ks == 4 ? SortConfig<float>::TILE : SortConfig<double>::TILE;
   if (n <= tile) return 0;
   size_t vs = vdtype >= 0 ? dtype_size(vdtype) : 0;
-  return align_up(size_t(n) * ks, 256) + size_t(n) * vs;
+  return SortWorkspace(ks, vs, n, (n + tile - 1) / tile).total;
 }
 
 }  // namespace bulkx

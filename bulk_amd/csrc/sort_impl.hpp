@@ -60,8 +60,18 @@ __device__ __forceinline__ void block_sort_tile(SortSmem<K, V, HAS_V, BLOCK, ITE
   // the merge rounds only reuse after their first barrier).
   wave_load_blocked<K, ITEMS>(keys + base + (wcount > 0 ? wb : 0), wcount, k, K(),
                               s_k + wid * WK::LDS_ELEMS);
-  if constexpr (HAS_V) {
-    for (int j = threadIdx.x; j < cnt; j += BLOCK) s_v[j] = vals[base + j];
+  if constexpr (HAS_V) {  // striped staging: every load in flight before the LDS writes
+    V lv[ITEMS];
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const int j = threadIdx.x + i * BLOCK;
+      if (j < cnt) lv[i] = vals[base + j];
+    }
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const int j = threadIdx.x + i * BLOCK;
+      if (j < cnt) s_v[j] = lv[i];
+    }
   }
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) idx[i] = mine + i;
@@ -135,33 +145,79 @@ __global__ __launch_bounds__(BLOCK) void block_sort_kernel(const K* keys, const 
   block_sort_tile<CMP, K, V, HAS_V, BLOCK, ITEMS>(sm, keys, vals, n, kout, vout, blockIdx.x);
 }
 
+// Geometry of output tile `tile` of a merge pass over runs of `width`: the
+// pair of runs it belongs to starts at ps, its runs hold na and nb elements,
+// and the tile covers diagonals [d0, d1) of the pair.
+struct PassTile {
+  int64_t ps, na, nb, d0, d1;
+};
+
+template <int TILE>
+__device__ __forceinline__ PassTile pass_tile(int64_t n, int64_t width, int64_t tile) {
+  PassTile g;
+  const int64_t out0 = tile * TILE;
+  g.ps = (out0 / (2 * width)) * (2 * width);
+  g.na = n - g.ps;
+  g.na = g.na < 0 ? 0 : (g.na > width ? width : g.na);
+  g.nb = n - g.ps - width;
+  g.nb = g.nb < 0 ? 0 : (g.nb > width ? width : g.nb);
+  g.d0 = out0 - g.ps;
+  g.d1 = g.d0 + TILE < g.na + g.nb ? g.d0 + TILE : g.na + g.nb;
+  return g;
+}
+
+// Merge path of a tile's first diagonal in a merge pass (the reference's
+// locate_merge_path, merge_sort_by_key.cu:90-131), searched by the calling
+// wavefront; every lane returns the result.
+template <class CMP, class K, int TILE>
+__device__ __forceinline__ int64_t pass_split(const K* src, int64_t n, int64_t width,
+                                              int64_t tile) {
+  const PassTile g = pass_tile<TILE>(n, width, tile);
+  return wave_merge_path<CMP>(src + g.ps, g.na, src + g.ps + width, g.nb, g.d0);
+}
+
 // One tile of a merge pass: pairs of sorted runs of width `width` (a
-// multiple of TILE) from (src_k, src_v) into (dst_k, dst_v).
+// multiple of TILE) from (src_k, src_v) into (dst_k, dst_v).  splits[t] is
+// the merge path of tile t's first diagonal (pass_split); the tile's last
+// diagonal is the pair's end or the next tile's first.
 template <class CMP, class K, class V, bool HAS_V, int BLOCK, int ITEMS>
 __device__ __forceinline__ void merge_pass_tile(MergeSmem<K, V, HAS_V, BLOCK, ITEMS>& sm,
                                                 const K* src_k, const V* src_v, int64_t n,
-                                                int64_t width, K* dst_k, V* dst_v, int64_t tile) {
+                                                int64_t width, K* dst_k, V* dst_v, int64_t tile,
+                                                const int64_t* splits) {
   constexpr int TILE = BLOCK * ITEMS;
-  const int64_t out0 = tile * TILE;
-  const int64_t ps = (out0 / (2 * width)) * (2 * width);
-  int64_t na = n - ps;
-  na = na < 0 ? 0 : (na > width ? width : na);
-  int64_t nb = n - ps - width;
-  nb = nb < 0 ? 0 : (nb > width ? width : nb);
-  merge_two_tile<CMP, K, V, HAS_V, BLOCK, ITEMS>(
-      sm, src_k + ps, HAS_V ? src_v + ps : nullptr, na, src_k + ps + width,
-      HAS_V ? src_v + ps + width : nullptr, nb, dst_k + ps, HAS_V ? dst_v + ps : nullptr,
-      (out0 - ps) / TILE);
+  const PassTile g = pass_tile<TILE>(n, width, tile);
+  const int64_t a0 = splits[tile];
+  const int64_t a1 = g.d1 == g.na + g.nb ? g.na : splits[tile + 1];
+  const int64_t b0 = g.d0 - a0, b1 = g.d1 - a1;
+  const int64_t pb = g.ps + width;
+  merge_tile<CMP, K, V, HAS_V, BLOCK, ITEMS>(
+      src_k + g.ps + a0, HAS_V ? src_v + g.ps + a0 : nullptr, int(a1 - a0), src_k + pb + b0,
+      HAS_V ? src_v + pb + b0 : nullptr, int(b1 - b0), dst_k + g.ps + g.d0,
+      HAS_V ? dst_v + g.ps + g.d0 : nullptr, sm.k, sm.v);
+}
+
+// One wavefront per tile: splits[t] for every tile of a merge pass.
+template <class CMP, class K, int TILE>
+__global__ __launch_bounds__(256) void merge_partition_kernel(const K* __restrict__ src,
+                                                              int64_t n, int64_t width,
+                                                              int64_t tiles,
+                                                              int64_t* __restrict__ splits) {
+  const int64_t t = int64_t(blockIdx.x) * (256 / kWave) + threadIdx.x / kWave;
+  if (t >= tiles) return;  // uniform per wavefront
+  const int64_t r = pass_split<CMP, K, TILE>(src, n, width, t);
+  if (lane_id() == 0) splits[t] = r;
 }
 
 template <class CMP, class K, class V, bool HAS_V, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void merge_pass_kernel(const K* __restrict__ src_k,
                                                            const V* __restrict__ src_v, int64_t n,
                                                            int64_t width, K* __restrict__ dst_k,
-                                                           V* __restrict__ dst_v) {
+                                                           V* __restrict__ dst_v,
+                                                           const int64_t* __restrict__ splits) {
   __shared__ MergeSmem<K, V, HAS_V, BLOCK, ITEMS> sm;
   merge_pass_tile<CMP, K, V, HAS_V, BLOCK, ITEMS>(sm, src_k, src_v, n, width, dst_k, dst_v,
-                                                  blockIdx.x);
+                                                  blockIdx.x, splits);
 }
 
 inline int sort_num_passes(int64_t tiles) {
@@ -170,13 +226,26 @@ inline int sort_num_passes(int64_t tiles) {
   return p;
 }
 
+// Workspace of a sort: ping-pong keys and values, then the pass splits.
+struct SortWorkspace {
+  size_t keys, vals, splits, total;
+  SortWorkspace(size_t ks, size_t vs, int64_t n, int64_t tiles) {
+    keys = 0;
+    vals = align_up(size_t(n) * ks, 256);
+    splits = vals + align_up(size_t(n) * vs, 256);
+    total = splits + size_t(tiles + 1) * sizeof(int64_t);
+  }
+};
+
 template <class CMP, class K, class V, bool HAS_V>
 int sort_launch(K* keys, V* vals, int64_t n, char* ws, hipStream_t stream) {
   using C = SortConfig<K>;
   const int64_t tiles = (n + C::TILE - 1) / C::TILE;
   const int passes = sort_num_passes(tiles);
-  K* tk = reinterpret_cast<K*>(ws);
-  V* tv = HAS_V ? reinterpret_cast<V*>(ws + align_up(size_t(n) * sizeof(K), 256)) : nullptr;
+  const SortWorkspace w(sizeof(K), HAS_V ? sizeof(V) : 0, n, tiles);
+  K* tk = reinterpret_cast<K*>(ws + w.keys);
+  V* tv = HAS_V ? reinterpret_cast<V*>(ws + w.vals) : nullptr;
+  int64_t* splits = reinterpret_cast<int64_t*>(ws + w.splits);
   // An odd pass count starts from the workspace so the last pass ends in place.
   K* ck = (passes & 1) ? tk : keys;
   V* cv = (passes & 1) ? tv : vals;
@@ -186,9 +255,13 @@ int sort_launch(K* keys, V* vals, int64_t n, char* ws, hipStream_t stream) {
       keys, vals, n, ck, cv);
   BULKX_LAUNCHED();
   int64_t width = C::TILE;
+  const uint32_t pgrid = uint32_t((tiles + 3) / 4);
   for (int p = 0; p < passes; ++p, width *= 2) {
+    merge_partition_kernel<CMP, K, C::TILE><<<dim3(pgrid), 256, 0, stream>>>(ck, n, width, tiles,
+                                                                             splits);
+    BULKX_LAUNCHED();
     merge_pass_kernel<CMP, K, V, HAS_V, C::BLOCK, C::ITEMS>
-        <<<dim3(uint32_t(tiles)), C::BLOCK, 0, stream>>>(ck, cv, n, width, ok, ov);
+        <<<dim3(uint32_t(tiles)), C::BLOCK, 0, stream>>>(ck, cv, n, width, ok, ov, splits);
     BULKX_LAUNCHED();
     K* t1 = ck;
     ck = ok;

@@ -261,9 +261,15 @@ int bulkx_batch_launch(bulkx_batch* batch, int use_graph);
 /* Run each job as its own plain call on the batch's stream (the per-call baseline). */
 int bulkx_batch_launch_per_call(bulkx_batch* batch);
 /* Device time in ms of the phases of the last eager launch (use_graph = 0):
- * the persistent first-phase kernel and all sort merge rounds together.
- * Waits for that launch to finish.  BULKX_ERR_NOT_READY if there is none. */
+ * the persistent first-phase kernels (one per op family) and all sort merge
+ * rounds together.  Waits for that launch to finish.  BULKX_ERR_NOT_READY if
+ * there is none. */
 int bulkx_batch_last_timing(bulkx_batch* batch, float* phase0_ms, float* passes_ms);
+/* Per-stage device time in ms of the last eager launch, in launch order:
+ * [0] scan/reduce, [1] reduce_by_key with 4-byte values, [2] reduce_by_key
+ * with 8-byte values, [3] merge_by_key, [4] sort block phase, [5] sort merge
+ * rounds.  Writes min(num_stages, 6) entries (zeros beyond). */
+int bulkx_batch_stage_timing(bulkx_batch* batch, float* stage_ms, int num_stages);
 
 /* ------------------------------------------------------------------------
  * Serving: a request loop that admits jobs under an HBM budget, keeps their
