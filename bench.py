@@ -42,7 +42,7 @@ def _parse():
     p.add_argument("--no-per-call", action="store_true",
                    help="skip the per-call leg (and the batched == per-call check)")
     p.add_argument("--traffic-file", default=os.path.join(os.path.dirname(os.path.abspath(__file__)),
-                                                          "profiles", "batch_phase0_traffic.txt"))
+                                                          "profiles", "batch_phase0_traffic.md"))
     return p.parse_args()
 
 
@@ -63,14 +63,19 @@ def _sort_passes(n, tile=4096):
     return p
 
 
-def _read_traffic(path):
-    """HBM bytes per phase-0 launch from a committed rocprofv3 --pmc summary
-    (lines `fetch_bytes_per_launch: X` and `write_bytes_per_launch: Y`)."""
+def _read_traffic(path, kernel):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3 --pmc
+    summary: the lines after `kernel: <name>` give `fetch_bytes_per_launch: X`
+    and `write_bytes_per_launch: Y` (counter KiB scaled by a copy of known
+    size), plus the `jobs` and `seed` of the workload they were measured on."""
     if not os.path.exists(path):
         return None
-    vals = {}
+    vals, current = {}, None
     for line in open(path):
-        if ":" in line:
+        if line.startswith("kernel:"):
+            current = line.split(":", 1)[1].strip()
+            continue
+        if ":" in line and (current is None or current == kernel):
             k, v = line.split(":", 1)
             try:
                 vals[k.strip()] = float(v.split()[0])
@@ -213,24 +218,41 @@ def main():
     bt_ms = timed(lambda: bt.launch(use_graph=True), args.steps)
     _log(f"batched: {np.mean(bt_ms):.2f} ms/step")
 
-    # roofline of the phase-0 kernel: eager launches timed with HIP events on
-    # the batch's stream (phase-0 only; the sort merge rounds separately)
-    p0, pp = [], []
-    for _ in range(max(3, args.steps)):
+    # per-stage device time: eager launches timed with HIP events on the
+    # batch's stream (one event pair around each phase-0 family kernel and
+    # around all sort merge rounds)
+    reps = max(3, args.steps)
+    acc = {k: 0.0 for k in Batch.STAGES}
+    for _ in range(reps):
         W.reset_all(jobs)
         bt.launch(use_graph=False)
-        a, b = bt.last_timing()
-        p0.append(a)
-        pp.append(b)
+        for k, v in bt.stage_timing().items():
+            acc[k] += v / reps
     torch.cuda.synchronize()
     bt.close()
-    phase0_bytes = sum(j.algorithmic_bytes(runs.get(i)) for i, j in enumerate(jobs))
-    pass_bytes = sum(_sort_passes(j.spec.n) * 16 * j.spec.n for j in jobs
-                     if j.spec.kind == C.JOB_SORT_BY_KEY)
-    p0_ms, pp_ms = float(np.mean(p0)), float(np.mean(pp))
-    p0_gbs = phase0_bytes / (p0_ms * 1e-3) / 1e9
-    pp_gbs = pass_bytes / (pp_ms * 1e-3) / 1e9 if pp_ms > 0 else None
-    _log(f"phase0 {p0_ms:.3f} ms {p0_gbs:.0f} GB/s; sort rounds {pp_ms:.3f} ms")
+    stage_bytes = {k: 0 for k in Batch.STAGES}
+    for i, j in enumerate(jobs):
+        k, dt, n = j.spec.kind, j.spec.dtype, j.spec.n
+        if k == C.JOB_INCLUSIVE_SCAN:
+            stage_bytes["scan_" + ("i32" if dt == "int32" else "f32")] += j.algorithmic_bytes()
+        elif k == C.JOB_REDUCE_BY_KEY:
+            stage_bytes["reduce_by_key_" + ("i32" if dt == "int32" else "f32")] += \
+                j.algorithmic_bytes(runs.get(i))
+        elif k == C.JOB_MERGE_BY_KEY:
+            stage_bytes["merge_by_key"] += j.algorithmic_bytes()
+        else:
+            stage_bytes["sort_block"] += j.algorithmic_bytes()
+            stage_bytes["sort_rounds"] += _sort_passes(n) * 16 * n
+    stages = {k: {"ms": acc[k], "bytes": stage_bytes[k],
+                  "gbs": stage_bytes[k] / (acc[k] * 1e-3) / 1e9 if acc[k] > 0 else None}
+              for k in Batch.STAGES}
+    for k, v in stages.items():
+        if v["bytes"]:
+            _log(f"stage {k:18s} {v['ms']:7.3f} ms {v['gbs'] or 0:7.0f} GB/s")
+    # roofline: the phase-0 family kernel with the largest device time
+    dom = max(Batch.STAGES[:-1], key=lambda k: acc[k])
+    dom_idx = Batch.STAGES.index(dom)
+    dom_gbs = stages[dom]["gbs"]
 
     # ---------------------------------------------------------------- serving
     serve = None
@@ -243,7 +265,9 @@ def main():
             torch.cuda.synchronize()
             s = Server(budget, max_inflight=3, max_batch_jobs=512, max_batch_elements=1 << 27,
                        flags=flags)
-            t_start = s.now_us() + 2000.0
+            # arrivals start after every request has been handed over, so the
+            # client's submission speed does not shape the arrival process
+            t_start = s.now_us() + 300000.0
             ids = []
             for j, off in zip(jobs, arrivals_us):
                 ids.append(s.submit(j.job, j.footprint_bytes(), arrival_us=t_start + off))
@@ -312,7 +336,7 @@ def main():
     value = total / (bt_mean * 1e-3) / 1e6
     pc_mean = float(np.mean(pc_ms)) if pc_ms else None
     pc_value = total / (pc_mean * 1e-3) / 1e6 if pc_ms else None
-    tr = _read_traffic(args.traffic_file)
+    tr = _read_traffic(args.traffic_file, f"batch_phase0_kernel<{dom_idx}>")
     traffic = None
     if tr is not None and tr[1] == len(jobs) and tr[2] == args.seed:
         traffic = tr[0]
@@ -342,13 +366,12 @@ def main():
         "p99_ms": serve["poisson"]["p99_ms"] if serve else None,
         "p50_ms": serve["poisson"]["p50_ms"] if serve else None,
         "copy_gbs": copy_gbs,
-        "roofline": {"bound": "hbm", "kernel": "batch_phase0_kernel", "achieved": p0_gbs,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": p0_gbs / HBM_PEAK_GBS,
-                     "frac_of_copy": p0_gbs / copy_gbs, "traffic": traffic,
-                     "algorithmic_bytes": phase0_bytes, "avg_launch_ms": p0_ms,
-                     "launches": len(p0)},
-        "sort_rounds": {"kernel": "batch_pass_kernel", "ms": pp_ms, "bytes": pass_bytes,
-                        "gbs": pp_gbs},
+        "roofline": {"bound": "hbm", "kernel": f"batch_phase0_kernel<{dom_idx}> ({dom})",
+                     "achieved": dom_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": dom_gbs / HBM_PEAK_GBS, "frac_of_copy": dom_gbs / copy_gbs,
+                     "traffic": traffic, "algorithmic_bytes": stages[dom]["bytes"],
+                     "avg_launch_ms": stages[dom]["ms"], "launches": reps},
+        "stages": stages,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -93,8 +93,8 @@ class Batch:
                 "bulkx_batch_last_timing")
         return a.value, b.value
 
-    STAGES = ("scan", "reduce_by_key_4b", "reduce_by_key_8b", "merge_by_key", "sort_block",
-              "sort_rounds")
+    STAGES = ("scan_i32", "scan_f32", "reduce_by_key_i32", "reduce_by_key_f32",
+              "reduce_by_key_i64", "merge_by_key", "sort_block", "sort_rounds")
 
     def stage_timing(self) -> dict[str, float]:
         """Per-stage ms of the last eager launch (HIP events on the batch's stream)."""

@@ -1,0 +1,125 @@
+"""Condense rocprofv3 CSV output into the short text summaries kept with the
+sources.
+
+  python -m bulk_amd.rocprof_summary stats <dir> [--command "..."]
+      per-kernel calls / average / total from *_kernel_stats.csv
+
+  python -m bulk_amd.rocprof_summary traffic <fetch_dir> <write_dir> --jobs J --seed S
+      HBM bytes per dispatch from a FETCH_SIZE run and a WRITE_SIZE run
+      (*_counter_collection.csv).  The counters are scaled by what they read
+      for the 1 GiB device copy the benchmark makes (1 GiB read + 1 GiB
+      written): factor = known bytes / counter bytes.
+"""
+from __future__ import annotations
+
+import argparse
+import collections
+import csv
+import glob
+import os
+import re
+import sys
+
+COPY_BYTES = 1 << 30  # the benchmark's calibration copy: 2^28 int32
+
+
+def _short(name: str, width: int = 96) -> str:
+    name = re.sub(r"\(.*", "", name)  # drop the argument list
+    name = name.replace("void ", "").replace("bulkx::", "")
+    return name if len(name) <= width else name[: width - 3] + "..."
+
+
+def _find(d: str, suffix: str) -> str:
+    hits = glob.glob(os.path.join(d, "**", f"*{suffix}"), recursive=True)
+    if not hits:
+        sys.exit(f"no *{suffix} under {d}")
+    return hits[0]
+
+
+def stats(d: str, command: str | None, top: int) -> str:
+    rows = list(csv.DictReader(open(_find(d, "kernel_stats.csv"))))
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    out = []
+    if command:
+        out.append(f"command: {command}\n")
+    out.append("| kernel | calls | avg us | total ms | % |")
+    out.append("|---|---:|---:|---:|---:|")
+    for r in rows[:top]:
+        out.append(f"| {_short(r['Name'])} | {int(r['Calls'])} | {float(r['AverageNs']) / 1e3:.1f} | "
+                   f"{float(r['TotalDurationNs']) / 1e6:.2f} | {float(r['Percentage']):.2f} |")
+    return "\n".join(out) + "\n"
+
+
+def _per_dispatch(d: str, counter: str):
+    """{dispatch id: (kernel name, grid size, counter value)}"""
+    vals = collections.defaultdict(float)
+    meta = {}
+    for r in csv.DictReader(open(_find(d, "counter_collection.csv"))):
+        if r["Counter_Name"] != counter:
+            continue
+        did = int(r["Dispatch_Id"])
+        vals[did] += float(r["Counter_Value"])
+        meta[did] = (r["Kernel_Name"], int(r["Grid_Size"]))
+    return {k: (meta[k][0], meta[k][1], v) for k, v in vals.items()}
+
+
+def traffic(fetch_dir: str, write_dir: str, jobs: int, seed: int, command: str | None) -> str:
+    out = []
+    if command:
+        out.append(f"command: {command}\n")
+    per = {}
+    for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        disp = _per_dispatch(d, counter)
+        # calibration: the largest copy dispatches are the 1 GiB copies
+        copies = [v for (n, g, v) in disp.values() if "copyBuffer" in n or "copy" in n.lower()]
+        big = sorted(copies)[-3:] if copies else []
+        kib = sum(big) / len(big) if big else float("nan")
+        factor = COPY_BYTES / (kib * 1024.0) if big else float("nan")
+        out.append(f"{counter}: copy of {COPY_BYTES} bytes reads {kib:.0f} KiB -> factor {factor:.4f}")
+        groups = collections.defaultdict(list)
+        for n, g, v in disp.values():
+            if "bulkx" in n:
+                groups[_short(n, 200)].append(v * 1024.0 * factor)
+        per[counter] = groups
+    out.append("")
+    out.append(f"jobs: {jobs}")
+    out.append(f"seed: {seed}")
+    out.append("")
+    names = sorted(set(per["FETCH_SIZE"]) | set(per["WRITE_SIZE"]))
+    for n in names:
+        f = per["FETCH_SIZE"].get(n, [])
+        w = per["WRITE_SIZE"].get(n, [])
+        if not f and not w:
+            continue
+        fb = sum(f) / len(f) if f else 0.0
+        wb = sum(w) / len(w) if w else 0.0
+        out.append(f"kernel: {n}")
+        out.append(f"dispatches: {len(f)} / {len(w)}")
+        out.append(f"fetch_bytes_per_launch: {fb:.0f}")
+        out.append(f"write_bytes_per_launch: {wb:.0f}")
+        out.append("")
+    return "\n".join(out)
+
+
+def main(argv=None):
+    p = argparse.ArgumentParser()
+    sub = p.add_subparsers(dest="mode", required=True)
+    s = sub.add_parser("stats")
+    s.add_argument("dir")
+    s.add_argument("--command")
+    s.add_argument("--top", type=int, default=30)
+    t = sub.add_parser("traffic")
+    t.add_argument("fetch_dir")
+    t.add_argument("write_dir")
+    t.add_argument("--jobs", type=int, required=True)
+    t.add_argument("--seed", type=int, required=True)
+    t.add_argument("--command")
+    a = p.parse_args(argv)
+    if a.mode == "stats":
+        print(stats(a.dir, a.command, a.top), end="")
+    else:
+        print(traffic(a.fetch_dir, a.write_dir, a.jobs, a.seed, a.command))
+
+
+if __name__ == "__main__":
+    main()

@@ -266,9 +266,9 @@ int bulkx_batch_launch_per_call(bulkx_batch* batch);
  * there is none. */
 int bulkx_batch_last_timing(bulkx_batch* batch, float* phase0_ms, float* passes_ms);
 /* Per-stage device time in ms of the last eager launch, in launch order:
- * [0] scan/reduce, [1] reduce_by_key with 4-byte values, [2] reduce_by_key
- * with 8-byte values, [3] merge_by_key, [4] sort block phase, [5] sort merge
- * rounds.  Writes min(num_stages, 6) entries (zeros beyond). */
+ * [0] int32 scan/reduce, [1] float32 scan/reduce, [2..4] reduce_by_key with
+ * int32 / float32 / int64 values, [5] merge_by_key, [6] sort block phase,
+ * [7] sort merge rounds.  Writes num_stages entries (zeros beyond 8). */
 int bulkx_batch_stage_timing(bulkx_batch* batch, float* stage_ms, int num_stages);
 
 /* ------------------------------------------------------------------------
