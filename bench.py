@@ -39,6 +39,8 @@ def _parse():
     p.add_argument("--cpu-seconds", type=float, default=10.0,
                    help="wall-clock budget of the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-ops", action="store_true",
+                   help="skip the single-op table (scan/reduce/merge/sort/reduce_by_key at full size)")
     p.add_argument("--no-per-call", action="store_true",
                    help="skip the per-call leg (and the batched == per-call check)")
     p.add_argument("--traffic-file", default=os.path.join(os.path.dirname(os.path.abspath(__file__)),
@@ -111,6 +113,7 @@ def main():
     jobs = W.build_jobs(specs, seed=args.seed + 2)
     torch.cuda.synchronize()
     total = W.total_elements(jobs)
+    njobs = len(jobs)
     _log(f"generated {len(jobs)} jobs, {total} elements in {time.perf_counter() - t0:.1f}s")
 
     # device copy bandwidth (read 1 GiB + write 1 GiB)
@@ -331,6 +334,15 @@ def main():
                          "random order, sequential restatement of the reference algorithm bodies"}
         _log(f"cpu baseline {cpu['value']:.1f} Melem/s over {sample} jobs")
 
+    # ------------------------------------------------- single ops at full size
+    ops = None
+    if not args.no_ops:
+        from bulk_amd import opbench
+        del jobs  # frees the stream's device memory before the 2^28 inputs
+        torch.cuda.empty_cache()
+        ops = opbench.run(iters=10, log=_log)
+        jobs = None
+
     # ------------------------------------------------------------------ report
     bt_mean = float(np.mean(bt_ms))
     value = total / (bt_mean * 1e-3) / 1e6
@@ -338,7 +350,7 @@ def main():
     pc_value = total / (pc_mean * 1e-3) / 1e6 if pc_ms else None
     tr = _read_traffic(args.traffic_file, f"batch_phase0_kernel<{dom_idx}>")
     traffic = None
-    if tr is not None and tr[1] == len(jobs) and tr[2] == args.seed:
+    if tr is not None and tr[1] == njobs and tr[2] == args.seed:
         traffic = tr[0]
     line = {
         "metric": "batched vs per-call Melem/s + p99 latency, 4k mixed jobs; GB/s %HBM peak",
@@ -353,9 +365,9 @@ def main():
         "vs_baseline": None,
         "dtype": "int32+float32",
         "data": "synthetic",
-        "config": {"workload": f"mixed-job stream: {len(jobs)} jobs (scan 40%, reduce_by_key 30%, "
+        "config": {"workload": f"mixed-job stream: {njobs} jobs (scan 40%, reduce_by_key 30%, "
                                "merge 20%, sort_by_key 10%), log-uniform 2^10..2^22 elements, "
-                               "int32/float32", "jobs": len(jobs), "elements": total,
+                               "int32/float32", "jobs": njobs, "elements": total,
                    "seed": args.seed, "hbm_budget_gb": args.budget_gb},
         "per_call_melem_per_s": pc_value,
         "per_call_ms_per_step": pc_mean,
@@ -372,6 +384,7 @@ def main():
                      "traffic": traffic, "algorithmic_bytes": stages[dom]["bytes"],
                      "avg_launch_ms": stages[dom]["ms"], "launches": reps},
         "stages": stages,
+        "ops": ops,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
