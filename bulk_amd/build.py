@@ -65,6 +65,31 @@ def build_library(verbose: bool = True) -> str:
     return LIB
 
 
+CPP_TESTS = os.path.join(ROOT, "tests", "cpp")
+CPP_BIN = os.path.join(CPP_TESTS, "_bin")
+
+
+def build_cpp_tests(verbose: bool = True) -> list[str]:
+    """Compile the C++ programs under tests/cpp (users of the header-only
+    call surface and of the driver facade) against the built library."""
+    os.makedirs(CPP_BIN, exist_ok=True)
+    out = []
+    for name in sorted(os.listdir(CPP_TESTS)):
+        if not name.endswith(".hip"):
+            continue
+        exe = os.path.join(CPP_BIN, name[:-4])
+        cmd = [HIPCC, *[c for c in COMMON if c != "-fPIC"], os.path.join(CPP_TESTS, name),
+               f"-L{LIBDIR}", "-lbulkx", "-Wl,-rpath,$ORIGIN/../../../bulk_amd/lib", "-o", exe]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        out.append(exe)
+    if verbose:
+        print(f"[bulk_amd.build] built {len(out)} C++ test program(s) in {CPP_BIN}", flush=True)
+    return out
+
+
 if __name__ == "__main__":
     build_library()
+    build_cpp_tests()
     sys.exit(0)

@@ -1,0 +1,313 @@
+// algorithm.hpp -- the reference's device-wide drivers as C++ templates over
+// the C-ABI of bulkx.h, with the reference's names and argument order:
+//
+//   inclusive_scan(first, last, result, init, op)          scan.cu:87-148
+//   my_reduce(first, last, init, op)                       reduce.cu:58-97
+//   my_merge(first1, last1, first2, last2, result, comp)   merge.cu:127-162
+//   merge_by_key(k1f, k1l, v1f, k2f, k2l, v2f, kout, vout, comp)
+//                                                           merge.hpp:507-607
+//   stable_merge_sort_by_key(keys_first, keys_last, values_first, comp)
+//                                                           merge_sort_by_key.cu:150-209
+//   my_reduce_by_key(kf, kl, vf, kr, vr, pred, op)         reduce_by_key.cu:176-272
+//   reduce_intervals(first, decomposition, result, op)     reduce_intervals.hpp:31-47
+//
+// Iterators are contiguous device ranges: raw pointers, or (when Thrust is
+// included first) thrust::device_ptr / device_vector iterators.  Operators
+// map onto the library's enums: plus / minimum / maximum for scan and
+// reduce, less / greater for merge and sort, equal_to for reduce_by_key
+// (std:: and thrust:: functors, or the ones below).  Every call is ordered
+// on `bulkx::default_stream()` (the null stream unless set); my_reduce and
+// my_reduce_by_key read their scalar back with a blocking copy, as the
+// reference does (reduce.cu:96, reduce_by_key.cu:269).  Errors are thrown
+// as bulkx::system_error, the counterpart of throw_on_error
+// (bulk/detail/throw_on_error.hpp:33-49).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <functional>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <utility>
+
+#include "../bulkx.h"
+
+namespace bulkx {
+
+class system_error : public std::runtime_error {
+ public:
+  system_error(int status, const std::string& where)
+      : std::runtime_error(where + ": " + bulkx_status_string(status) +
+                           (status == BULKX_ERR_HIP ? std::string(" (") + bulkx_last_error() + ")"
+                                                    : std::string())),
+        status_(status) {}
+  int status() const { return status_; }
+
+ private:
+  int status_;
+};
+
+inline void check(int status, const char* where) {
+  if (status != BULKX_SUCCESS) throw system_error(status, where);
+}
+
+inline hipStream_t& default_stream() {
+  static thread_local hipStream_t s = nullptr;
+  return s;
+}
+
+// ------------------------------------------------------------- functors
+template <class T = void>
+struct plus {
+  __host__ __device__ T operator()(const T& a, const T& b) const { return a + b; }
+};
+template <class T = void>
+struct minimum {
+  __host__ __device__ T operator()(const T& a, const T& b) const { return b < a ? b : a; }
+};
+template <class T = void>
+struct maximum {
+  __host__ __device__ T operator()(const T& a, const T& b) const { return a < b ? b : a; }
+};
+template <class T = void>
+struct less {
+  __host__ __device__ bool operator()(const T& a, const T& b) const { return a < b; }
+};
+template <class T = void>
+struct greater {
+  __host__ __device__ bool operator()(const T& a, const T& b) const { return b < a; }
+};
+template <class T = void>
+struct equal_to {
+  __host__ __device__ bool operator()(const T& a, const T& b) const { return a == b; }
+};
+
+namespace detail {
+
+template <class T>
+struct dtype_of;
+template <> struct dtype_of<int32_t> { static constexpr int value = BULKX_INT32; };
+template <> struct dtype_of<uint32_t> { static constexpr int value = BULKX_UINT32; };
+template <> struct dtype_of<int64_t> { static constexpr int value = BULKX_INT64; };
+template <> struct dtype_of<uint64_t> { static constexpr int value = BULKX_UINT64; };
+template <> struct dtype_of<float> { static constexpr int value = BULKX_FLOAT32; };
+template <> struct dtype_of<double> { static constexpr int value = BULKX_FLOAT64; };
+template <> struct dtype_of<long long> { static constexpr int value = BULKX_INT64; };
+template <> struct dtype_of<unsigned long long> { static constexpr int value = BULKX_UINT64; };
+
+template <template <class> class F, class Op>
+struct is_functor : std::false_type {};
+template <template <class> class F, class T>
+struct is_functor<F, F<T>> : std::true_type {};
+
+template <class Op>
+constexpr int op_code() {
+  if constexpr (is_functor<plus, Op>::value || is_functor<std::plus, Op>::value) return BULKX_PLUS;
+  else if constexpr (is_functor<minimum, Op>::value) return BULKX_MIN;
+  else if constexpr (is_functor<maximum, Op>::value) return BULKX_MAX;
+#ifdef THRUST_VERSION
+  else if constexpr (is_functor<thrust::plus, Op>::value) return BULKX_PLUS;
+  else if constexpr (is_functor<thrust::minimum, Op>::value) return BULKX_MIN;
+  else if constexpr (is_functor<thrust::maximum, Op>::value) return BULKX_MAX;
+#endif
+  else return -1;
+}
+
+template <class Cmp>
+constexpr int cmp_code() {
+  if constexpr (is_functor<less, Cmp>::value || is_functor<std::less, Cmp>::value) return BULKX_LESS;
+  else if constexpr (is_functor<greater, Cmp>::value || is_functor<std::greater, Cmp>::value)
+    return BULKX_GREATER;
+#ifdef THRUST_VERSION
+  else if constexpr (is_functor<thrust::less, Cmp>::value) return BULKX_LESS;
+  else if constexpr (is_functor<thrust::greater, Cmp>::value) return BULKX_GREATER;
+#endif
+  else return -1;
+}
+
+template <class Pred>
+constexpr int pred_code() {
+  if constexpr (is_functor<equal_to, Pred>::value || is_functor<std::equal_to, Pred>::value)
+    return BULKX_EQUAL_TO;
+#ifdef THRUST_VERSION
+  else if constexpr (is_functor<thrust::equal_to, Pred>::value) return BULKX_EQUAL_TO;
+#endif
+  else return -1;
+}
+
+// Raw device pointer of a contiguous iterator.
+template <class T>
+T* raw(T* p) {
+  return p;
+}
+#ifdef THRUST_VERSION
+template <class It>
+auto raw(It it) -> decltype(thrust::raw_pointer_cast(&*it)) {
+  return thrust::raw_pointer_cast(&*it);
+}
+#endif
+
+template <class It>
+using value_t = std::remove_cv_t<std::remove_pointer_t<decltype(raw(std::declval<It>()))>>;
+
+}  // namespace detail
+
+template <class It1, class It2, class T, class Op>
+It2 inclusive_scan(It1 first, It1 last, It2 result, T init, Op op) {
+  using V = detail::value_t<It1>;
+  static_assert(std::is_same<V, detail::value_t<It2>>::value, "input and output types match");
+  static_assert(detail::op_code<Op>() >= 0, "op must be plus, minimum or maximum");
+  const int64_t n = int64_t(last - first);
+  const V iv = V(init);
+  check(bulkx_inclusive_scan(detail::dtype_of<V>::value, detail::op_code<Op>(), detail::raw(first),
+                             detail::raw(result), n, &iv, nullptr, 0, default_stream()),
+        "inclusive_scan");
+  return result + n;
+}
+
+template <class It, class T, class Op>
+T my_reduce(It first, It last, T init, Op op) {
+  using V = detail::value_t<It>;
+  static_assert(detail::op_code<Op>() >= 0, "op must be plus, minimum or maximum");
+  const V iv = V(init);
+  V* d = nullptr;
+  check(hipMallocAsync(reinterpret_cast<void**>(&d), sizeof(V), default_stream()) == hipSuccess
+            ? BULKX_SUCCESS
+            : BULKX_ERR_OUT_OF_MEMORY,
+        "my_reduce");
+  int st = bulkx_reduce(detail::dtype_of<V>::value, detail::op_code<Op>(), detail::raw(first),
+                        int64_t(last - first), &iv, d, nullptr, 0, default_stream());
+  V h = iv;
+  if (st == BULKX_SUCCESS &&
+      hipMemcpyAsync(&h, d, sizeof(V), hipMemcpyDeviceToHost, default_stream()) != hipSuccess)
+    st = BULKX_ERR_HIP;
+  if (hipStreamSynchronize(default_stream()) != hipSuccess && st == BULKX_SUCCESS) st = BULKX_ERR_HIP;
+  (void)hipFreeAsync(d, default_stream());
+  check(st, "my_reduce");
+  return T(h);
+}
+
+template <class It1, class It2, class It3, class Compare>
+It3 my_merge(It1 first1, It1 last1, It2 first2, It2 last2, It3 result, Compare comp) {
+  using K = detail::value_t<It1>;
+  static_assert(detail::cmp_code<Compare>() >= 0, "comp must be less or greater");
+  const int64_t n1 = int64_t(last1 - first1), n2 = int64_t(last2 - first2);
+  check(bulkx_merge(detail::dtype_of<K>::value, detail::cmp_code<Compare>(), detail::raw(first1), n1,
+                    detail::raw(first2), n2, detail::raw(result), nullptr, 0, default_stream()),
+        "my_merge");
+  return result + (n1 + n2);
+}
+
+template <class KIt1, class VIt1, class KIt2, class VIt2, class KIt3, class VIt3, class Compare>
+std::pair<KIt3, VIt3> merge_by_key(KIt1 keys_first1, KIt1 keys_last1, VIt1 values_first1,
+                                   KIt2 keys_first2, KIt2 keys_last2, VIt2 values_first2,
+                                   KIt3 keys_result, VIt3 values_result, Compare comp) {
+  using K = detail::value_t<KIt1>;
+  using V = detail::value_t<VIt1>;
+  static_assert(detail::cmp_code<Compare>() >= 0, "comp must be less or greater");
+  const int64_t n1 = int64_t(keys_last1 - keys_first1), n2 = int64_t(keys_last2 - keys_first2);
+  check(bulkx_merge_by_key(detail::dtype_of<K>::value, detail::dtype_of<V>::value,
+                           detail::cmp_code<Compare>(), detail::raw(keys_first1),
+                           detail::raw(values_first1), n1, detail::raw(keys_first2),
+                           detail::raw(values_first2), n2, detail::raw(keys_result),
+                           detail::raw(values_result), nullptr, 0, default_stream()),
+        "merge_by_key");
+  return {keys_result + (n1 + n2), values_result + (n1 + n2)};
+}
+
+template <class KIt, class VIt, class Compare>
+void stable_merge_sort_by_key(KIt keys_first, KIt keys_last, VIt values_first, Compare comp) {
+  using K = detail::value_t<KIt>;
+  using V = detail::value_t<VIt>;
+  static_assert(detail::cmp_code<Compare>() >= 0, "comp must be less or greater");
+  check(bulkx_stable_sort_by_key(detail::dtype_of<K>::value, detail::dtype_of<V>::value,
+                                 detail::cmp_code<Compare>(), detail::raw(keys_first),
+                                 detail::raw(values_first), int64_t(keys_last - keys_first),
+                                 nullptr, 0, default_stream()),
+        "stable_merge_sort_by_key");
+}
+
+template <class KIt1, class VIt1, class KIt2, class VIt2, class Pred, class Op>
+std::pair<KIt2, VIt2> my_reduce_by_key(KIt1 keys_first, KIt1 keys_last, VIt1 values_first,
+                                       KIt2 keys_result, VIt2 values_result, Pred pred, Op op) {
+  using K = detail::value_t<KIt1>;
+  using V = detail::value_t<VIt1>;
+  static_assert(detail::pred_code<Pred>() >= 0, "pred must be equal_to");
+  static_assert(detail::op_code<Op>() >= 0, "op must be plus, minimum or maximum");
+  int64_t* d = nullptr;
+  check(hipMallocAsync(reinterpret_cast<void**>(&d), sizeof(int64_t), default_stream()) == hipSuccess
+            ? BULKX_SUCCESS
+            : BULKX_ERR_OUT_OF_MEMORY,
+        "my_reduce_by_key");
+  int st = bulkx_reduce_by_key(detail::dtype_of<K>::value, detail::dtype_of<V>::value,
+                               detail::pred_code<Pred>(), detail::op_code<Op>(),
+                               detail::raw(keys_first), detail::raw(values_first),
+                               int64_t(keys_last - keys_first), detail::raw(keys_result),
+                               detail::raw(values_result), d, nullptr, 0, default_stream());
+  int64_t m = 0;
+  if (st == BULKX_SUCCESS &&
+      hipMemcpyAsync(&m, d, sizeof(m), hipMemcpyDeviceToHost, default_stream()) != hipSuccess)
+    st = BULKX_ERR_HIP;
+  if (hipStreamSynchronize(default_stream()) != hipSuccess && st == BULKX_SUCCESS) st = BULKX_ERR_HIP;
+  (void)hipFreeAsync(d, default_stream());
+  check(st, "my_reduce_by_key");
+  return {keys_result + m, values_result + m};
+}
+
+// aligned_decomposition(n, num_partitions, tile): ceil(n / tile) tiles split
+// over the partitions, the first (tiles % P) partitions taking one more
+// (decomposition.hpp:142-218).
+template <class Size>
+class aligned_decomposition {
+ public:
+  typedef Size size_type;
+  typedef std::pair<Size, Size> range;
+  aligned_decomposition(Size n, Size num_partitions, Size aligned_size)
+      : m_n(n), m_p(num_partitions), m_tile(aligned_size) {}
+  range operator[](Size i) const {
+    const Size tiles = (m_n + m_tile - 1) / m_tile;
+    const Size per = tiles / m_p, extra = tiles % m_p;
+    Size b = per * i + (i < extra ? i : extra);
+    Size e = b + per + (i < extra ? 1 : 0);
+    b *= m_tile;
+    e = e * m_tile < m_n ? e * m_tile : m_n;
+    return range(b, e);
+  }
+  Size size() const { return m_p; }
+  Size n() const { return m_n; }
+  Size tile_size() const { return m_tile; }
+
+ private:
+  Size m_n, m_p, m_tile;
+};
+
+template <class Size>
+aligned_decomposition<Size> make_aligned_decomposition(Size n, Size num_partitions, Size aligned_size) {
+  return aligned_decomposition<Size>(n, num_partitions, aligned_size);
+}
+
+// reduce_intervals(first, decomp, result, op): result[g] = the op-reduction
+// of partition g of the decomposition.
+template <class It1, class Size, class It2, class Op>
+It2 reduce_intervals(It1 first, const aligned_decomposition<Size>& decomp, It2 result, Op op) {
+  using V = detail::value_t<It1>;
+  static_assert(detail::op_code<Op>() >= 0, "op must be plus, minimum or maximum");
+  check(bulkx_reduce_intervals(detail::dtype_of<V>::value, detail::op_code<Op>(), detail::raw(first),
+                               int64_t(decomp.n()), int64_t(decomp.size()),
+                               int64_t(decomp.tile_size()), detail::raw(result), default_stream()),
+        "reduce_intervals");
+  return result + decomp.size();
+}
+
+// reduce_intervals(first, last, interval_size, result, op): fixed-size
+// intervals (one tile per partition of an aligned decomposition).
+template <class It1, class Size, class It2, class Op>
+It2 reduce_intervals(It1 first, It1 last, Size interval_size, It2 result, Op op) {
+  const Size n = Size(last - first);
+  const Size p = (n + interval_size - 1) / interval_size;
+  return reduce_intervals(first, aligned_decomposition<Size>(n, p, interval_size), result, op);
+}
+
+}  // namespace bulkx
