@@ -3,6 +3,7 @@
 // each workgroup finds its own tile boundaries with a wavefront search,
 // stages the two input slices in LDS and merges them.
 #include "merge_impl.hpp"
+#include "tuning.hpp"
 
 namespace bulkx {
 
@@ -16,16 +17,24 @@ __global__ __launch_bounds__(BLOCK) void merge_kernel(const K* __restrict__ a,
   merge_two_tile<CMP, K, V, HAS_V, BLOCK, ITEMS>(sm, a, va, n1, b, vb, n2, out, vout, blockIdx.x);
 }
 
-template <class CMP, class K, class V, bool HAS_V>
-static int merge_launch(const void* k1, const void* v1, int64_t n1, const void* k2, const void* v2,
-                        int64_t n2, void* kout, void* vout, hipStream_t stream) {
-  using C = MergeConfig<K>;
+template <class CMP, class K, class V, bool HAS_V, int VAR>
+static int merge_launch_v(const void* k1, const void* v1, int64_t n1, const void* k2,
+                          const void* v2, int64_t n2, void* kout, void* vout, hipStream_t stream) {
+  using C = MergeConfig<K, VAR>;
   const int64_t tiles = (n1 + n2 + C::TILE - 1) / C::TILE;
   merge_kernel<CMP, K, V, HAS_V, C::BLOCK, C::ITEMS><<<dim3(uint32_t(tiles)), C::BLOCK, 0, stream>>>(
       static_cast<const K*>(k1), static_cast<const V*>(v1), n1, static_cast<const K*>(k2),
       static_cast<const V*>(v2), n2, static_cast<K*>(kout), static_cast<V*>(vout));
   BULKX_LAUNCHED();
   return BULKX_SUCCESS;
+}
+
+template <class CMP, class K, class V, bool HAS_V>
+static int merge_launch(const void* k1, const void* v1, int64_t n1, const void* k2, const void* v2,
+                        int64_t n2, void* kout, void* vout, hipStream_t stream, int variant) {
+  if (variant == 1)
+    return merge_launch_v<CMP, K, V, HAS_V, 1>(k1, v1, n1, k2, v2, n2, kout, vout, stream);
+  return merge_launch_v<CMP, K, V, HAS_V, 0>(k1, v1, n1, k2, v2, n2, kout, vout, stream);
 }
 
 size_t merge_workspace_bytes(int, int, int64_t, int64_t) { return 0; }
@@ -43,16 +52,17 @@ static int merge_entry(int kdtype, int vdtype, int comp, const void* k1, const v
   if (has_v && ((n1 > 0 && !v1) || (n2 > 0 && !v2) || !vout)) return BULKX_ERR_INVALID_ARGUMENT;
   if (n1 + n2 > (int64_t(1) << 40)) return BULKX_ERR_INVALID_ARGUMENT;
   hipStream_t stream = static_cast<hipStream_t>(stream_);
+  const int v = tuned_variant(has_v ? BULKX_ALGO_MERGE_BY_KEY : BULKX_ALGO_MERGE, kdtype, n1 + n2);
   int st = BULKX_SUCCESS;
   // The payload is only moved, so it is dispatched by size.
   BULKX_DISPATCH_DTYPE(kdtype, K, BULKX_DISPATCH_CMP(comp, CMP, {
     if (!has_v)
       st = merge_launch<CMP, K, uint32_t, false>(k1, nullptr, n1, k2, nullptr, n2, kout, nullptr,
-                                                 stream);
+                                                 stream, v);
     else if (dtype_size(vdtype) == 4)
-      st = merge_launch<CMP, K, uint32_t, true>(k1, v1, n1, k2, v2, n2, kout, vout, stream);
+      st = merge_launch<CMP, K, uint32_t, true>(k1, v1, n1, k2, v2, n2, kout, vout, stream, v);
     else
-      st = merge_launch<CMP, K, uint64_t, true>(k1, v1, n1, k2, v2, n2, kout, vout, stream);
+      st = merge_launch<CMP, K, uint64_t, true>(k1, v1, n1, k2, v2, n2, kout, vout, stream, v);
   }));
   return st;
 }

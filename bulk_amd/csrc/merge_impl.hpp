@@ -17,12 +17,16 @@
 
 namespace bulkx {
 
-template <class K>
+// Tile shapes (same tile, two thread counts): variant 0 = 256 threads x
+// 16 (4-byte keys) / 8 items, variant 1 = 512 threads x 8 / 4 items (shorter
+// sequential merges per lane, twice the lanes).
+template <class K, int VAR = 0>
 struct MergeConfig {
-  static constexpr int BLOCK = 256;
-  static constexpr int ITEMS = sizeof(K) == 4 ? 16 : 8;
+  static constexpr int BLOCK = VAR == 0 ? 256 : 512;
+  static constexpr int ITEMS = (sizeof(K) == 4 ? 16 : 8) / (VAR == 0 ? 1 : 2);
   static constexpr int TILE = BLOCK * ITEMS;
 };
+constexpr int kMergeVariants = 2;
 
 // Wave-cooperative merge_path over global memory.  Every lane of the
 // calling wavefront must participate; every lane returns the same result.

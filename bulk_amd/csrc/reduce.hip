@@ -15,6 +15,7 @@
 // (decomposition.hpp:142-218).
 #include "device_common.hpp"
 #include "host_common.hpp"
+#include "tuning.hpp"
 
 namespace bulkx {
 
@@ -97,13 +98,19 @@ __global__ __launch_bounds__(BLOCK) void reduce_final_kernel(
   }
 }
 
-static int reduce_grid(int64_t n, int elem_size) {
+// Workgroups per CU of each tuning variant: 0 = 8 (default), 1 = 4, 2 = 16.
+static int reduce_blocks_per_cu(int variant) {
+  return variant == 1 ? 4 : variant == 2 ? 16 : kReduceBlocksPerCU;
+}
+static constexpr int kReduceMaxBlocksPerCU = 16;
+
+static int reduce_grid(int64_t n, int elem_size, int variant) {
   DeviceProps p;
   int cus = current_device_props(&p) == BULKX_SUCCESS ? p.num_cus : 256;
   const int64_t vec = 16 / elem_size;
   const int64_t per_block = int64_t(kReduceBlock) * vec * kReduceUnroll;
   int64_t want = (n + per_block - 1) / per_block;
-  int64_t cap = int64_t(cus) * kReduceBlocksPerCU;
+  int64_t cap = int64_t(cus) * reduce_blocks_per_cu(variant);
   if (want < 1) want = 1;
   return int(want < cap ? want : cap);
 }
@@ -115,15 +122,15 @@ size_t reduce_workspace_bytes(int dtype, int64_t n) {
   DeviceProps p;
   int cus = current_device_props(&p) == BULKX_SUCCESS ? p.num_cus : 256;
   (void)n;
-  return size_t(cus) * kReduceBlocksPerCU * 16;
+  return size_t(cus) * kReduceMaxBlocksPerCU * 16;
 }
 
 template <class T, class Op>
 static int reduce_launch(const T* in, int64_t n, const T* init, T* out, char* ws,
-                         hipStream_t stream) {
+                         hipStream_t stream, int variant) {
   using A = typename Acc<T, Op>::type;
   const T init_v = init ? *init : Op::template identity<T>();
-  const int grid = n > 0 ? reduce_grid(n, sizeof(T)) : 0;
+  const int grid = n > 0 ? reduce_grid(n, sizeof(T), variant) : 0;
   A* partials = reinterpret_cast<A*>(ws);
   if (grid > 0) {
     reduce_partials_kernel<T, Op, kReduceBlock, kReduceUnroll>
@@ -178,9 +185,10 @@ extern "C" int bulkx_reduce(int dtype, int op, const void* in, int64_t n, const 
   Scratch scratch;
   BULKX_TRY(scratch.acquire(workspace, workspace_bytes, reduce_workspace_bytes(dtype, n), stream));
   int st = BULKX_SUCCESS;
+  const int v = tuned_variant(BULKX_ALGO_REDUCE, dtype, n);
   BULKX_DISPATCH_DTYPE(dtype, T, BULKX_DISPATCH_OP(op, OP, {
     st = reduce_launch<T, OP>(static_cast<const T*>(in), n, static_cast<const T*>(init),
-                              static_cast<T*>(out), scratch.ptr(), stream);
+                              static_cast<T*>(out), scratch.ptr(), stream, v);
   }));
   int rs = scratch.release();
   return st != BULKX_SUCCESS ? st : rs;

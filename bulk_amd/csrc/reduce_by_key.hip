@@ -1,6 +1,7 @@
 // reduce_by_key entry point, its preparation kernel, and the materialised
 // head / tail flags (tail_flags.hpp:26-109, bulk/detail/head_flags.hpp:126-232).
 #include "reduce_by_key_impl.hpp"
+#include "tuning.hpp"
 
 namespace bulkx {
 
@@ -20,8 +21,7 @@ __global__ void rbk_prepare_kernel(uint32_t* header, uint64_t* words, int64_t nu
 size_t reduce_by_key_workspace_bytes(int kdtype, int vdtype, int64_t n) {
   size_t ks = dtype_size(kdtype), vs = dtype_size(vdtype);
   if (ks == 0 || vs == 0 || n <= 0) return 0;
-  const int64_t tile = 512 * 16;
-  const int64_t tiles = (n + tile - 1) / tile;
+  const int64_t tiles = (n + kRbkMinTile - 1) / kRbkMinTile;  // fits every tile shape
   if (tiles <= 1) return 0;
   return kRbkHeaderBytes + size_t(tiles) * kRbkMaxStatusWords * sizeof(uint64_t);
 }
@@ -73,13 +73,14 @@ extern "C" int bulkx_reduce_by_key(int key_dtype, int value_dtype, int pred, int
   Scratch scratch;
   BULKX_TRY(scratch.acquire(workspace, workspace_bytes,
                             reduce_by_key_workspace_bytes(key_dtype, value_dtype, n), stream));
+  const int v = tuned_variant(BULKX_ALGO_REDUCE_BY_KEY, value_dtype, n);
   int st;
   if (dtype_size(key_dtype) == 4)
     st = rbk_dispatch_k4(key_dtype, value_dtype, op, keys, values, n, keys_out, values_out,
-                         num_runs, scratch.ptr(), stream);
+                         num_runs, scratch.ptr(), stream, v);
   else
     st = rbk_dispatch_k8(key_dtype, value_dtype, op, keys, values, n, keys_out, values_out,
-                         num_runs, scratch.ptr(), stream);
+                         num_runs, scratch.ptr(), stream, v);
   int rs = scratch.release();
   return st != BULKX_SUCCESS ? st : rs;
 }

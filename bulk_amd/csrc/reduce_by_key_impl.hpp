@@ -72,12 +72,15 @@ struct RowSegOp {
   }
 };
 
-template <class K, class V>
+// Tile shapes: variant 0 = 512 x 16 (default), 1 = 512 x 8, 2 = 256 x 16.
+template <class K, class V, int VAR = 0>
 struct RbkConfig {
-  static constexpr int BLOCK = 512;
-  static constexpr int ITEMS = 16;
+  static constexpr int BLOCK = VAR == 2 ? 256 : 512;
+  static constexpr int ITEMS = VAR == 1 ? 8 : 16;
   static constexpr int TILE = BLOCK * ITEMS;
 };
+constexpr int kRbkVariants = 3;
+constexpr int kRbkMinTile = 4096;  // smallest tile of any variant (workspace sizing)
 
 constexpr size_t kRbkHeaderBytes = 256;
 constexpr int kRbkMaxStatusWords = 6;  // Seg<dd> = 24 bytes
@@ -312,23 +315,23 @@ __global__ __launch_bounds__(BLOCK) void reduce_by_key_kernel(
 __global__ void rbk_empty_kernel(int64_t* num_runs);
 __global__ void rbk_prepare_kernel(uint32_t* header, uint64_t* words, int64_t num_words);
 
-template <class K, class V>
+template <class K, class V, int VAR>
 inline int64_t rbk_num_tiles(int64_t n) {
-  constexpr int TILE = RbkConfig<K, V>::TILE;
+  constexpr int TILE = RbkConfig<K, V, VAR>::TILE;
   return (n + TILE - 1) / TILE;
 }
 
-template <class K, class V, class Op>
+template <class K, class V, class Op, int VAR>
 int rbk_launch(const K* keys, const V* vals, int64_t n, K* kout, V* vout, int64_t* num_runs,
                char* ws, hipStream_t stream) {
-  using C = RbkConfig<K, V>;
+  using C = RbkConfig<K, V, VAR>;
   using S = Seg<typename Acc<V, Op>::type>;
   if (n == 0) {
     rbk_empty_kernel<<<1, 64, 0, stream>>>(num_runs);
     BULKX_LAUNCHED();
     return BULKX_SUCCESS;
   }
-  const int64_t tiles = rbk_num_tiles<K, V>(n);
+  const int64_t tiles = rbk_num_tiles<K, V, VAR>(n);
   if (tiles == 1) {
     reduce_by_key_kernel<K, V, Op, C::BLOCK, C::ITEMS><<<1, C::BLOCK, 0, stream>>>(
         keys, vals, n, kout, vout, num_runs, nullptr, nullptr, true);
@@ -350,18 +353,31 @@ int rbk_launch(const K* keys, const V* vals, int64_t n, K* kout, V* vout, int64_
 // Dispatch over value type and op for one key type K.
 template <class K>
 int rbk_dispatch_values(int vdtype, int op, const void* keys, const void* vals, int64_t n,
-                        void* kout, void* vout, int64_t* num_runs, char* ws, hipStream_t stream) {
+                        void* kout, void* vout, int64_t* num_runs, char* ws, hipStream_t stream,
+                        int variant) {
   int st = BULKX_SUCCESS;
   BULKX_DISPATCH_DTYPE(vdtype, V, BULKX_DISPATCH_OP(op, OP, {
-    st = rbk_launch<K, V, OP>(static_cast<const K*>(keys), static_cast<const V*>(vals), n,
-                              static_cast<K*>(kout), static_cast<V*>(vout), num_runs, ws, stream);
+    const K* k = static_cast<const K*>(keys);
+    const V* v = static_cast<const V*>(vals);
+    K* ko = static_cast<K*>(kout);
+    V* vo = static_cast<V*>(vout);
+    // the other shapes are instantiated for plus only (min / max use the default)
+    if constexpr (std::is_same<OP, OpPlus>::value) {
+      if (variant == 1) st = rbk_launch<K, V, OP, 1>(k, v, n, ko, vo, num_runs, ws, stream);
+      else if (variant == 2) st = rbk_launch<K, V, OP, 2>(k, v, n, ko, vo, num_runs, ws, stream);
+      else st = rbk_launch<K, V, OP, 0>(k, v, n, ko, vo, num_runs, ws, stream);
+    } else {
+      st = rbk_launch<K, V, OP, 0>(k, v, n, ko, vo, num_runs, ws, stream);
+    }
   }));
   return st;
 }
 
 int rbk_dispatch_k4(int kdtype, int vdtype, int op, const void* keys, const void* vals, int64_t n,
-                    void* kout, void* vout, int64_t* num_runs, char* ws, hipStream_t stream);
+                    void* kout, void* vout, int64_t* num_runs, char* ws, hipStream_t stream,
+                    int variant);
 int rbk_dispatch_k8(int kdtype, int vdtype, int op, const void* keys, const void* vals, int64_t n,
-                    void* kout, void* vout, int64_t* num_runs, char* ws, hipStream_t stream);
+                    void* kout, void* vout, int64_t* num_runs, char* ws, hipStream_t stream,
+                    int variant);
 
 }  // namespace bulkx

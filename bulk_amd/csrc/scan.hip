@@ -16,6 +16,7 @@
 #include "host_common.hpp"
 #include "scan_config.hpp"
 #include "scan_impl.hpp"
+#include "tuning.hpp"
 
 namespace bulkx {
 
@@ -45,9 +46,9 @@ __global__ __launch_bounds__(BLOCK) void scan_kernel(const T* in, T* out, int64_
                                             header + 1, single_tile, nullptr);
 }
 
-template <class T>
+template <class T, int V>
 static int64_t scan_num_tiles(int64_t n) {
-  constexpr int TILE = ScanConfig<T>::BLOCK * ScanConfig<T>::ITEMS;
+  constexpr int TILE = ScanConfig<T, V>::TILE;
   return (n + TILE - 1) / TILE;
 }
 
@@ -68,16 +69,17 @@ static int status_words_per_tile(int dtype) {
 size_t scan_workspace_bytes(int dtype, int64_t n) {
   size_t es = dtype_size(dtype);
   if (es == 0 || n <= 0) return 0;
-  int64_t tiles = es == 4 ? scan_num_tiles<float>(n) : scan_num_tiles<double>(n);
+  // sized for the variant with the smallest tile, so any variant fits
+  int64_t tiles = es == 4 ? scan_num_tiles<float, 1>(n) : scan_num_tiles<double, 1>(n);
   if (tiles <= 1) return 0;
   return kHeaderBytes + size_t(tiles) * status_words_per_tile(dtype) * sizeof(uint64_t);
 }
 
-template <class T, class Op, bool EXCLUSIVE>
+template <class T, class Op, bool EXCLUSIVE, int V>
 static int scan_launch(const T* in, T* out, int64_t n, const T* init, char* ws,
                        hipStream_t stream) {
-  using C = ScanConfig<T>;
-  const int64_t tiles = scan_num_tiles<T>(n);
+  using C = ScanConfig<T, V>;
+  const int64_t tiles = scan_num_tiles<T, V>(n);
   using A = typename Acc<T, Op>::type;
   const T init_v = init ? *init : Op::template identity<T>();
   if (tiles <= 1) {
@@ -112,9 +114,21 @@ static int scan_entry(int dtype, int op, const void* in, void* out, int64_t n, c
   Scratch scratch;
   BULKX_TRY(scratch.acquire(workspace, workspace_bytes, scan_workspace_bytes(dtype, n), stream));
   int st = BULKX_SUCCESS;
+  const int v = tuned_variant(EXCLUSIVE ? BULKX_ALGO_EXCLUSIVE_SCAN : BULKX_ALGO_INCLUSIVE_SCAN,
+                              dtype, n);
   BULKX_DISPATCH_DTYPE(dtype, T, BULKX_DISPATCH_OP(op, OP, {
-    st = scan_launch<T, OP, EXCLUSIVE>(static_cast<const T*>(in), static_cast<T*>(out), n,
-                                       static_cast<const T*>(init), scratch.ptr(), stream);
+    // the second shape is instantiated for plus only (min / max use the default)
+    bool done = false;
+    if constexpr (std::is_same<OP, OpPlus>::value) {
+      if (v == 1) {
+        st = scan_launch<T, OP, EXCLUSIVE, 1>(static_cast<const T*>(in), static_cast<T*>(out), n,
+                                              static_cast<const T*>(init), scratch.ptr(), stream);
+        done = true;
+      }
+    }
+    if (!done)
+      st = scan_launch<T, OP, EXCLUSIVE, 0>(static_cast<const T*>(in), static_cast<T*>(out), n,
+                                            static_cast<const T*>(init), scratch.ptr(), stream);
   }));
   int rs = scratch.release();
   return st != BULKX_SUCCESS ? st : rs;

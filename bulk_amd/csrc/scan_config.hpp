@@ -4,12 +4,15 @@
 
 namespace bulkx {
 
-// 1024-thread workgroups amortise one look-back over a 128 KB tile; the
-// shapes come from a block x items sweep on MI355X at 2^24 and 2^28 elements.
-template <class T>
+// Variant 0 (the default): 1024-thread workgroups amortise one look-back
+// over a 128 KB tile.  Variant 1: a quarter tile (more tiles in flight for
+// mid-size inputs).  The tuning table picks one per dtype and size class.
+template <class T, int V = 0>
 struct ScanConfig {
-  static constexpr int BLOCK = 1024;
-  static constexpr int ITEMS = sizeof(T) == 4 ? 32 : 16;
+  static constexpr int BLOCK = V == 0 ? 1024 : 512;
+  static constexpr int ITEMS = (sizeof(T) == 4 ? 32 : 16) / (V == 0 ? 1 : 2);
+  static constexpr int TILE = BLOCK * ITEMS;
 };
+constexpr int kScanVariants = 2;
 
 }  // namespace bulkx

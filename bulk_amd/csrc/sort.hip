@@ -1,5 +1,6 @@
 // stable_sort_by_key entry point.
 #include "sort_impl.hpp"
+#include "tuning.hpp"
 
 namespace bulkx {
 
@@ -30,11 +31,12 @@ extern "C" int bulkx_stable_sort_by_key(int key_dtype, int value_dtype, int comp
   Scratch scratch;
   BULKX_TRY(scratch.acquire(workspace, workspace_bytes,
                             sort_workspace_bytes(key_dtype, value_dtype, n), stream));
+  const int v = tuned_variant(BULKX_ALGO_SORT_BY_KEY, key_dtype, n);
   int st;
   if (dtype_size(key_dtype) == 4)
-    st = sort_dispatch_k4(key_dtype, value_dtype, comp, keys, values, n, scratch.ptr(), stream);
+    st = sort_dispatch_k4(key_dtype, value_dtype, comp, keys, values, n, scratch.ptr(), stream, v);
   else
-    st = sort_dispatch_k8(key_dtype, value_dtype, comp, keys, values, n, scratch.ptr(), stream);
+    st = sort_dispatch_k8(key_dtype, value_dtype, comp, keys, values, n, scratch.ptr(), stream, v);
   int rs = scratch.release();
   return st != BULKX_SUCCESS ? st : rs;
 }

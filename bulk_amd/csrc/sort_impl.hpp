@@ -23,8 +23,8 @@
 
 namespace bulkx {
 
-template <class K>
-using SortConfig = MergeConfig<K>;
+template <class K, int VAR = 0>
+using SortConfig = MergeConfig<K, VAR>;
 
 template <class K, class V, bool HAS_V, int BLOCK, int ITEMS>
 struct SortSmem {
@@ -237,9 +237,9 @@ struct SortWorkspace {
   }
 };
 
-template <class CMP, class K, class V, bool HAS_V>
+template <class CMP, class K, class V, bool HAS_V, int VAR>
 int sort_launch(K* keys, V* vals, int64_t n, char* ws, hipStream_t stream) {
-  using C = SortConfig<K>;
+  using C = SortConfig<K, VAR>;
   const int64_t tiles = (n + C::TILE - 1) / C::TILE;
   const int passes = sort_num_passes(tiles);
   const SortWorkspace w(sizeof(K), HAS_V ? sizeof(V) : 0, n, tiles);
@@ -273,26 +273,33 @@ int sort_launch(K* keys, V* vals, int64_t n, char* ws, hipStream_t stream) {
   return BULKX_SUCCESS;
 }
 
-template <class K>
-int sort_dispatch_key(int vdtype, int comp, void* keys, void* vals, int64_t n, char* ws,
-                      hipStream_t stream) {
+template <class K, int VAR>
+int sort_dispatch_variant(int vdtype, int comp, void* keys, void* vals, int64_t n, char* ws,
+                          hipStream_t stream) {
   int st = BULKX_SUCCESS;
   BULKX_DISPATCH_CMP(comp, CMP, {
     if (vdtype < 0)
-      st = sort_launch<CMP, K, uint32_t, false>(static_cast<K*>(keys), nullptr, n, ws, stream);
+      st = sort_launch<CMP, K, uint32_t, false, VAR>(static_cast<K*>(keys), nullptr, n, ws, stream);
     else if (dtype_size(vdtype) == 4)
-      st = sort_launch<CMP, K, uint32_t, true>(static_cast<K*>(keys), static_cast<uint32_t*>(vals),
-                                               n, ws, stream);
+      st = sort_launch<CMP, K, uint32_t, true, VAR>(static_cast<K*>(keys),
+                                                    static_cast<uint32_t*>(vals), n, ws, stream);
     else
-      st = sort_launch<CMP, K, uint64_t, true>(static_cast<K*>(keys), static_cast<uint64_t*>(vals),
-                                               n, ws, stream);
+      st = sort_launch<CMP, K, uint64_t, true, VAR>(static_cast<K*>(keys),
+                                                    static_cast<uint64_t*>(vals), n, ws, stream);
   });
   return st;
 }
 
+template <class K>
+int sort_dispatch_key(int vdtype, int comp, void* keys, void* vals, int64_t n, char* ws,
+                      hipStream_t stream, int variant) {
+  if (variant == 1) return sort_dispatch_variant<K, 1>(vdtype, comp, keys, vals, n, ws, stream);
+  return sort_dispatch_variant<K, 0>(vdtype, comp, keys, vals, n, ws, stream);
+}
+
 int sort_dispatch_k4(int kdtype, int vdtype, int comp, void* keys, void* vals, int64_t n,
-                     char* ws, hipStream_t stream);
+                     char* ws, hipStream_t stream, int variant);
 int sort_dispatch_k8(int kdtype, int vdtype, int comp, void* keys, void* vals, int64_t n,
-                     char* ws, hipStream_t stream);
+                     char* ws, hipStream_t stream, int variant);
 
 }  // namespace bulkx

@@ -109,6 +109,17 @@ int bulkx_choose_sizes(int algo, int dtype, int64_t n,
  */
 int bulkx_load_tuning(const char* path, int* entries);
 
+/*
+ * Autotuner interface (no counterpart in the reference, whose group and grain
+ * sizes are constants in each driver): every algorithm is compiled with
+ * bulkx_tuning_variants() tile shapes; the table maps (algo, dtype, size
+ * class) to one of them.  Size classes: 0: n < 2^16, 1: < 2^20, 2: < 2^24,
+ * 3: >= 2^24.  The dtype key is the key/element dtype, and the value dtype
+ * for reduce_by_key.  variant = -1 removes the entry (built-in default).
+ */
+int bulkx_tuning_variants(int algo, int* count);
+int bulkx_set_tuning(int algo, int dtype, int size_class, int variant);
+
 /* Bytes of scratch `algo` needs for n1 (+ n2) elements; *bytes may be 0. */
 int bulkx_workspace_size(int algo, int dtype, int value_dtype, int64_t n1, int64_t n2,
                          size_t* bytes);

@@ -62,13 +62,23 @@ def test_workspace_size_query_is_host_only():
     # one look-back record per tile plus a 256-byte header; records are one
     # 64-bit word per 32 bits of the accumulator (int32: 1, float64 as
     # double-double: 4)
+    # double-double: 4).  The workspace is sized for the smallest tile of any
+    # compiled shape, so it serves whichever shape the tuning table picks.
+    nvar = ctypes.c_int()
+    assert lib.bulkx_tuning_variants(C.ALGO_INCLUSIVE_SCAN, ctypes.byref(nvar)) == 0
     for dt, n, words in [(C.INT32, 1 << 24, 1), (C.FLOAT64, 1 << 20, 4), (C.FLOAT32, 12345679, 2)]:
         assert lib.bulkx_workspace_size(C.ALGO_INCLUSIVE_SCAN, dt, -1, n, 0, ctypes.byref(nb)) == 0
-        assert lib.bulkx_choose_sizes(C.ALGO_INCLUSIVE_SCAN, dt, n, ctypes.byref(ng), ctypes.byref(gs),
-                                      ctypes.byref(gr)) == 0
-        tile = gs.value * gr.value
-        assert ng.value == -(-n // tile)
-        assert nb.value == 256 + 8 * words * ng.value
+        tiles = []
+        for v in range(nvar.value):
+            for sc in range(4):
+                assert lib.bulkx_set_tuning(C.ALGO_INCLUSIVE_SCAN, dt, sc, v) == 0
+            assert lib.bulkx_choose_sizes(C.ALGO_INCLUSIVE_SCAN, dt, n, ctypes.byref(ng),
+                                          ctypes.byref(gs), ctypes.byref(gr)) == 0
+            tile = gs.value * gr.value
+            assert ng.value == -(-n // tile)
+            tiles.append(ng.value)
+        assert nb.value == 256 + 8 * words * max(tiles)
+    assert lib.bulkx_load_tuning(None, None) == 0
     # a single tile needs no scratch
     assert lib.bulkx_workspace_size(C.ALGO_INCLUSIVE_SCAN, C.INT32, -1, 1000, 0, ctypes.byref(nb)) == 0
     assert nb.value == 0
@@ -83,3 +93,32 @@ def test_device_info_without_gpu_reports_no_device():
     cus = ctypes.c_int()
     st = lib.bulkx_device_info(ctypes.byref(cus), None, None, None, None)
     assert st in (5, 6)
+
+
+def test_tuning_table_round_trip(tmp_path):
+    """A table written in the autotuner's format changes choose_sizes; bad
+    entries are ignored; bulkx_set_tuning validates its arguments."""
+    lib = C.lib()
+    ng, gs, gr = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()
+    shapes = {}
+    nvar = ctypes.c_int()
+    assert lib.bulkx_tuning_variants(C.ALGO_REDUCE_BY_KEY, ctypes.byref(nvar)) == 0
+    assert nvar.value == 3
+    for v in range(nvar.value):
+        path = tmp_path / f"t{v}.json"
+        path.write_text('{"device": "x", "entries": ['
+                        f'{{"algo": {C.ALGO_REDUCE_BY_KEY}, "dtype": {C.FLOAT32}, "size_class": 3, "variant": {v}}},'
+                        f'{{"algo": {C.ALGO_REDUCE_BY_KEY}, "dtype": {C.FLOAT32}, "size_class": 3, "variant": 99}},'
+                        '{"algo": 42, "dtype": 0, "size_class": 0, "variant": 0}]}')
+        entries = ctypes.c_int()
+        assert lib.bulkx_load_tuning(str(path).encode(), ctypes.byref(entries)) == 0
+        assert entries.value == 1
+        assert lib.bulkx_choose_sizes(C.ALGO_REDUCE_BY_KEY, C.FLOAT32, 1 << 26, ctypes.byref(ng),
+                                      ctypes.byref(gs), ctypes.byref(gr)) == 0
+        shapes[v] = (gs.value, gr.value)
+        assert ng.value == (1 << 26) // (gs.value * gr.value)
+    assert shapes == {0: (512, 16), 1: (512, 8), 2: (256, 16)}
+    assert lib.bulkx_set_tuning(C.ALGO_SCAN if hasattr(C, "ALGO_SCAN") else 0, 0, 4, 0) == 1
+    assert lib.bulkx_set_tuning(C.ALGO_REDUCE_BY_KEY, C.FLOAT32, 0, 3) == 1
+    assert lib.bulkx_load_tuning(str(tmp_path / "missing.json").encode(), None) == 1
+    assert lib.bulkx_load_tuning(None, None) == 0

@@ -1,0 +1,116 @@
+"""Every compiled tile shape (tuning variant) of every algorithm gives the
+oracle's result: integer work, merge, sort and reduce_by_key bit-exact,
+float sums within 4*ceil(log2 n)*eps*sum|x|.  Each test forces one variant
+for every size class with bulkx_set_tuning and restores the packaged table
+afterwards."""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from bulk_amd import _capi as C
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _variants(algo):
+    c = ctypes.c_int()
+    C.check(C.lib().bulkx_tuning_variants(algo, ctypes.byref(c)), "variants")
+    return c.value
+
+
+@pytest.fixture
+def force(bx):
+    def f(algo, dtype, v):
+        for sc in range(4):
+            C.check(C.lib().bulkx_set_tuning(algo, dtype, sc, v), "bulkx_set_tuning")
+    yield f
+    C.check(C.lib().bulkx_load_tuning(None, None), "bulkx_load_tuning")
+
+
+SIZES = [1, 4095, 4096, 4097, 8193, 16385, 100003, 1 << 20]
+
+
+@pytest.mark.parametrize("v", [0, 1])
+def test_scan_variants(force, orc, v):
+    from bulk_amd import api
+    assert _variants(C.ALGO_INCLUSIVE_SCAN) == 2
+    force(C.ALGO_INCLUSIVE_SCAN, C.INT32, v)
+    force(C.ALGO_EXCLUSIVE_SCAN, C.INT32, v)
+    force(C.ALGO_INCLUSIVE_SCAN, C.FLOAT64, v)
+    for n in SIZES + [(1 << 20) + 12345]:
+        x = orc.gen_uniform_i32(n, -1000, 1000, n)
+        d = _dev(x)
+        assert np.array_equal(api.inclusive_scan(d, init=3).cpu().numpy(), orc.inclusive_scan(x, init=3)), n
+        assert np.array_equal(api.exclusive_scan(d, init=3).cpu().numpy(), orc.exclusive_scan(x, init=3)), n
+        f = orc.gen_uniform_f64(n, n)
+        out = api.inclusive_scan(_dev(f)).cpu().numpy()
+        hi, lo, ab = orc.prefix_sum_exact(f)
+        assert np.all(np.abs(out - hi - lo) <= orc.float_tolerance(n, 2.0 ** -53, ab)), n
+
+
+@pytest.mark.parametrize("v", [0, 1, 2])
+def test_reduce_variants(force, orc, v):
+    from bulk_amd import api
+    force(C.ALGO_REDUCE, C.INT32, v)
+    force(C.ALGO_REDUCE, C.FLOAT32, v)
+    for n in SIZES + [(1 << 22) + 3]:
+        x = orc.gen_uniform_i32(n, -1000, 1000, n)
+        assert api.my_reduce(_dev(x), 5) == int(orc.reduce(x, 5)), n
+        f = orc.gen_uniform_f32(n, n)
+        hi, lo, ab = orc.sum_exact(f)
+        assert abs(api.my_reduce(_dev(f), 0.0) - (hi + lo)) <= orc.float_tolerance(n, 2.0 ** -24, ab), n
+
+
+@pytest.mark.parametrize("v", [0, 1])
+def test_merge_and_sort_variants(force, orc, v):
+    from bulk_amd import api
+    for a in (C.ALGO_MERGE, C.ALGO_MERGE_BY_KEY):
+        force(a, C.FLOAT32, v)
+        force(a, C.INT32, v)
+    force(C.ALGO_SORT_BY_KEY, C.INT32, v)
+    force(C.ALGO_SORT_BY_KEY, C.INT64, v)
+    pool = orc.gen_uniform_f32(7, 64)
+    for n1, n2 in [(1, 0), (4095, 1), (4096, 4097), (70000, 30001), (1 << 19, (1 << 19) + 7)]:
+        a = orc.gen_merge_keys(n1, 0, n1, pool, 0.1)
+        b = orc.gen_merge_keys(n1, 1, n2, pool, 0.1)
+        va = np.arange(n1, dtype=np.int32)
+        vb = (1 << 28) + np.arange(n2, dtype=np.int32)
+        ko, vo = api.merge_by_key(_dev(a), _dev(va), _dev(b), _dev(vb))
+        rk, rv = orc.merge_by_key(a, va, b, vb)
+        assert np.array_equal(ko.cpu().numpy(), rk) and np.array_equal(vo.cpu().numpy(), rv), (n1, n2)
+        assert np.array_equal(api.my_merge(_dev(a), _dev(b)).cpu().numpy(), orc.merge(a, b)), (n1, n2)
+    for n in SIZES:
+        k = orc.gen_uniform_i32(n, 0, max(2, n // 8), n)
+        vals = np.arange(n, dtype=np.int32)
+        dk, dv = _dev(k), _dev(vals)
+        api.stable_merge_sort_by_key(dk, dv)
+        order = np.argsort(k, kind="stable")
+        assert np.array_equal(dk.cpu().numpy(), k[order]) and np.array_equal(dv.cpu().numpy(), vals[order]), n
+        k8 = orc.gen_uniform_i64(n, -50, 50, n)
+        d8 = _dev(k8)
+        api.stable_merge_sort_by_key(d8)
+        assert np.array_equal(d8.cpu().numpy(), np.sort(k8, kind="stable")), n
+
+
+@pytest.mark.parametrize("v", [0, 1, 2])
+def test_reduce_by_key_variants(force, orc, v):
+    from bulk_amd import api
+    for code in (C.INT32, C.INT64, C.FLOAT32):
+        force(C.ALGO_REDUCE_BY_KEY, code, v)
+    for n in SIZES + [(1 << 21) + 5]:
+        keys, _ = orc.gen_runs_geometric(n, n, 8.0)
+        for vals in (orc.gen_uniform_i32(n + 1, -50, 50, n), orc.gen_uniform_i64(n + 2, -(1 << 20), 1 << 20, n)):
+            ko, vo, m = api.my_reduce_by_key(_dev(keys), _dev(vals))
+            rk, rv = orc.reduce_by_key(keys, vals, textbook=True)
+            assert m == rk.size and np.array_equal(ko[:m].cpu().numpy(), rk) and \
+                np.array_equal(vo[:m].cpu().numpy(), rv), (n, vals.dtype)
+        f = orc.gen_uniform_f32(n + 3, n, -1.0, 1.0)
+        ko, vo, m = api.my_reduce_by_key(_dev(keys), _dev(f))
+        rk, _ = orc.reduce_by_key(keys, f.astype(np.float64), textbook=True)
+        assert m == rk.size and np.array_equal(ko[:m].cpu().numpy(), rk), n
