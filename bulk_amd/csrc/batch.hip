@@ -366,6 +366,7 @@ static size_t per_call_workspace(const bulkx_job& j) {
     case BULKX_JOB_REDUCE: return reduce_workspace_bytes(j.dtype, j.n0);
     case BULKX_JOB_REDUCE_BY_KEY: return reduce_by_key_workspace_bytes(j.dtype, j.value_dtype, j.n0);
     case BULKX_JOB_SORT_BY_KEY: return sort_workspace_bytes(j.dtype, j.value_dtype, j.n0);
+    case BULKX_JOB_MERGE_BY_KEY: return merge_workspace_bytes(j.dtype, j.value_dtype, j.n0, j.n1);
     default: return 0;
   }
 }
@@ -803,7 +804,7 @@ extern "C" int bulkx_batch_launch_per_call(bulkx_batch* b) {
         break;
       case BULKX_JOB_MERGE_BY_KEY:
         st = bulkx_merge_by_key(j.dtype, j.value_dtype, BULKX_LESS, j.in0, j.in1, j.n0, j.in2,
-                                j.in3, j.n1, j.out0, j.out1, nullptr, 0, s);
+                                j.in3, j.n1, j.out0, j.out1, b->d_ws, b->ws_bytes, s);
         break;
       case BULKX_JOB_SORT_BY_KEY:
         st = bulkx_stable_sort_by_key(j.dtype, j.value_dtype, BULKX_LESS, j.out0, j.out1, j.n0,

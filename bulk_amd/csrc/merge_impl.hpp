@@ -66,22 +66,38 @@ __device__ int64_t wave_merge_path(const K* a, int64_t n1, const K* b, int64_t n
   return lo;
 }
 
-// Sequential merge_path over LDS: A = s[0, n1), B = s[b_off, b_off + n2).
+// Padded LDS index: one pad element per 128 bytes, so that lanes reading
+// elements a multiple of 32 words apart (blocked per-thread layouts, merge
+// paths of neighbouring lanes) fall into different banks.
+template <class T>
+__device__ __forceinline__ int lpad(int e) {
+  return e + e / (128 / int(sizeof(T)));
+}
+// LDS elements a padded array of n logical elements needs.
+template <class T>
+constexpr int lpad_elems(int n) {
+  return n + n / (128 / int(sizeof(T)));
+}
+
+// Sequential merge_path over padded LDS: A = s[base, base + n1),
+// B = s[base + b_off, base + b_off + n2) (logical indices).
 template <class CMP, class K>
-__device__ __forceinline__ int lds_merge_path(const K* s, int n1, int b_off, int n2, int diag) {
+__device__ __forceinline__ int lds_merge_path(const K* s, int base, int n1, int b_off, int n2,
+                                              int diag) {
   int lo = diag > n2 ? diag - n2 : 0;
   int hi = diag < n1 ? diag : n1;
   while (lo < hi) {
     int m = (lo + hi) >> 1;
-    if (CMP::apply(s[b_off + diag - 1 - m], s[m])) hi = m;
+    if (CMP::apply(s[lpad<K>(base + b_off + diag - 1 - m)], s[lpad<K>(base + m)])) hi = m;
     else lo = m + 1;
   }
   return lo;
 }
 
-// Sequential agent merge of up to ITEMS outputs from LDS runs
-// A = [i1, e1), B = [i2, e2) (the merge.hpp:109-129 body: take A unless B's
-// head compares strictly less).  Produces keys and the LDS index they came from.
+// Sequential agent merge of up to ITEMS outputs from padded LDS runs
+// A = [i1, e1), B = [i2, e2) (logical indices; the merge.hpp:109-129 body:
+// take A unless B's head compares strictly less).  Produces keys and the
+// logical LDS index they came from.
 template <class CMP, class K, int ITEMS>
 __device__ __forceinline__ void lds_serial_merge(const K* s, int i1, int e1, int i2, int e2,
                                                  int count, K (&rk)[ITEMS], int (&ri)[ITEMS]) {
@@ -89,8 +105,8 @@ __device__ __forceinline__ void lds_serial_merge(const K* s, int i1, int e1, int
   for (int i = 0; i < ITEMS; ++i) {
     if (i < count) {
       const bool has1 = i1 < e1, has2 = i2 < e2;
-      const K ka = has1 ? s[i1] : K();
-      const K kb = has2 ? s[i2] : K();
+      const K ka = has1 ? s[lpad<K>(i1)] : K();
+      const K kb = has2 ? s[lpad<K>(i2)] : K();
       const bool p = !has2 || (has1 && !CMP::apply(kb, ka));
       rk[i] = p ? ka : kb;
       ri[i] = p ? i1 : i2;
@@ -105,6 +121,7 @@ __device__ __forceinline__ void lds_serial_merge(const K* s, int i1, int e1, int
 template <class T, int BLOCK, int ITEMS>
 struct MergeLds {
   static constexpr int ELEMS = (BLOCK / kWave) * WaveTile<T, ITEMS>::LDS_ELEMS;
+  static_assert(ELEMS >= lpad_elems<T>(BLOCK * ITEMS), "room for the padded stage");
 };
 
 // Merge of one tile: A[0, na) and B[0, nb) (na + nb <= TILE) into
@@ -138,14 +155,14 @@ __device__ __forceinline__ void merge_tile(const K* a, const V* va, int na, cons
     for (int i = 0; i < ITEMS; ++i) {
       const int j = threadIdx.x + i * BLOCK;
       if (j < cnt) {
-        s_k[j] = lk[i];
-        if constexpr (HAS_V) s_v[j] = lv[i];
+        s_k[lpad<K>(j)] = lk[i];
+        if constexpr (HAS_V) s_v[lpad<V>(j)] = lv[i];
       }
     }
   }
   __syncthreads();
   const int d = threadIdx.x * ITEMS < cnt ? threadIdx.x * ITEMS : cnt;
-  const int ai = lds_merge_path<CMP>(s_k, na, na, nb, d);
+  const int ai = lds_merge_path<CMP>(s_k, 0, na, na, nb, d);
   const int bi = na + (d - ai);
   K rk[ITEMS];
   V rv[ITEMS];
@@ -154,7 +171,7 @@ __device__ __forceinline__ void merge_tile(const K* a, const V* va, int na, cons
   lds_serial_merge<CMP, K, ITEMS>(s_k, ai, na, bi, cnt, count, rk, ri);
   if constexpr (HAS_V) {
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) rv[i] = i < count ? s_v[ri[i]] : V();
+    for (int i = 0; i < ITEMS; ++i) rv[i] = i < count ? s_v[lpad<V>(ri[i])] : V();
   }
   __syncthreads();  // every read of the stage is done; reuse it for the transposes
   const int wid = threadIdx.x / kWave;

@@ -29,7 +29,7 @@ using SortConfig = MergeConfig<K, VAR>;
 template <class K, class V, bool HAS_V, int BLOCK, int ITEMS>
 struct SortSmem {
   K k[MergeLds<K, BLOCK, ITEMS>::ELEMS];
-  uint16_t i[BLOCK * ITEMS];
+  uint32_t i[lpad_elems<uint32_t>(BLOCK * ITEMS)];
   V v[HAS_V ? MergeLds<V, BLOCK, ITEMS>::ELEMS : 1];
 };
 
@@ -42,7 +42,7 @@ __device__ __forceinline__ void block_sort_tile(SortSmem<K, V, HAS_V, BLOCK, ITE
   using WK = WaveTile<K, ITEMS>;
   using WV = WaveTile<V, ITEMS>;
   K* s_k = sm.k;
-  uint16_t* s_i = sm.i;
+  uint32_t* s_i = sm.i;
   V* s_v = sm.v;
 
   const int64_t base = tile * TILE;
@@ -70,7 +70,7 @@ __device__ __forceinline__ void block_sort_tile(SortSmem<K, V, HAS_V, BLOCK, ITE
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
       const int j = threadIdx.x + i * BLOCK;
-      if (j < cnt) s_v[j] = lv[i];
+      if (j < cnt) s_v[lpad<V>(j)] = lv[i];
     }
   }
 #pragma unroll
@@ -98,8 +98,8 @@ __device__ __forceinline__ void block_sort_tile(SortSmem<K, V, HAS_V, BLOCK, ITE
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
       if (i < vt) {
-        s_k[mine + i] = k[i];
-        s_i[mine + i] = uint16_t(idx[i]);
+        s_k[lpad<K>(mine + i)] = k[i];
+        s_i[lpad<uint32_t>(mine + i)] = uint32_t(idx[i]);
       }
     }
     __syncthreads();
@@ -110,7 +110,7 @@ __device__ __forceinline__ void block_sort_tile(SortSmem<K, V, HAS_V, BLOCK, ITE
       int nb = cnt - ps - w;
       nb = nb < 0 ? 0 : (nb > w ? w : nb);
       const int diag = mine - ps;
-      const int ai = lds_merge_path<CMP>(s_k + ps, na, w, nb, diag);
+      const int ai = lds_merge_path<CMP>(s_k, ps, na, w, nb, diag);
       const int bi = diag - ai;
       K rk[ITEMS];
       int ri[ITEMS];
@@ -119,7 +119,7 @@ __device__ __forceinline__ void block_sort_tile(SortSmem<K, V, HAS_V, BLOCK, ITE
       for (int i = 0; i < ITEMS; ++i) {
         if (i < vt) {
           k[i] = rk[i];
-          idx[i] = s_i[ri[i]];
+          idx[i] = int(s_i[lpad<uint32_t>(ri[i])]);
         }
       }
     }
@@ -128,7 +128,7 @@ __device__ __forceinline__ void block_sort_tile(SortSmem<K, V, HAS_V, BLOCK, ITE
   V rv[ITEMS];
   if constexpr (HAS_V) {
 #pragma unroll
-    for (int i = 0; i < ITEMS; ++i) rv[i] = i < vt ? s_v[idx[i]] : V();
+    for (int i = 0; i < ITEMS; ++i) rv[i] = i < vt ? s_v[lpad<V>(idx[i])] : V();
   }
   __syncthreads();  // all LDS reads done; reuse s_k / s_v as transpose regions
   if (wcount > 0) {

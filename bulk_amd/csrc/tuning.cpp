@@ -162,9 +162,9 @@ int variant_shape(int algo, int dtype, int64_t n, int v, LaunchShape* out) {
       break;
     }
     case BULKX_ALGO_REDUCE: {
-      const int per_cu = v == 1 ? 4 : v == 2 ? 16 : 8;
+      const int per_cu = v == 1 ? 2 : v == 2 ? 6 : 3;
       s.group_size = 256;
-      s.grain_size = int(16 / es) * 4;
+      s.grain_size = int(16 / es) * 2;
       const int64_t tile = int64_t(s.group_size) * s.grain_size;
       const int64_t want = (n + tile - 1) / tile;
       s.num_groups = want < int64_t(cus) * per_cu ? want : int64_t(cus) * per_cu;

@@ -158,7 +158,8 @@ def run(iters: int = 10, log=print, large: int = 28) -> list[dict]:
     vb = va + h
     ko = torch.empty(2 * h, device=dev)
     vo = torch.empty(2 * h, dtype=torch.int32, device=dev)
-    ms = _timeit(lambda: api.merge_by_key(a, va, b, vb, ko, vo), iters)
+    wsm = api.Workspace.for_algo(C.ALGO_MERGE_BY_KEY, C.FLOAT32, C.INT32, h, h)
+    ms = _timeit(lambda: api.merge_by_key(a, va, b, vb, ko, vo, workspace=wsm), iters)
     ok = bool(torch.equal(ko, torch.sort(torch.cat([a, b]), stable=True).values))
     rows.append(_row(f"merge_by_key f32+i32 2 x 2^{large - 1}", ms, 2 * (2 * h) * 8, 2 * h,
                      keys_sorted=ok))
