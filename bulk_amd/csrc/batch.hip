@@ -79,7 +79,7 @@ struct BatchDesc {
   int64_t fam_begin[FAM_COUNT + 1];  // family f owns tiles [fam_begin[f], fam_begin[f+1])
   int64_t total_sort_tiles;
   int64_t num_status_words;
-  uint32_t counter[FAM_COUNT];
+  int64_t num_jobs;
   uint32_t err;
 };
 
@@ -122,12 +122,12 @@ struct FamilySmem<FAM_SORT> {
   };
 };
 
-__global__ void batch_clear_kernel(BatchDesc* d, uint64_t* status) {
+__global__ void batch_clear_kernel(BatchDesc* d, uint64_t* status, uint32_t* job_ctr) {
   const int64_t nw = d->num_status_words;
-  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nw;
-       i += int64_t(gridDim.x) * blockDim.x)
-    status[i] = 0;
-  if (blockIdx.x == 0 && threadIdx.x < FAM_COUNT) d->counter[threadIdx.x] = 0;
+  const int64_t gs = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nw; i += gs) status[i] = 0;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < d->num_jobs; i += gs)
+    job_ctr[i] = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) d->err = 0;
 }
 
@@ -202,24 +202,37 @@ __device__ __forceinline__ void run_tile(FamilySmem<FAM>& sm, BatchDesc* d, cons
 
 // Phase-0 kernel of one family: one workgroup per tile, launched with one
 // workgroup per tile of the family (the host resizes the grid, also inside
-// the captured graph, whenever the job list changes).  Tiles are claimed
-// from an atomic counter in queue order, so a tile's look-back predecessors
-// always belong to workgroups that are already running or done.  (A
+// the captured graph, whenever the job list changes).  Workgroup b serves a
+// tile of job tile_job[begin + b].  Families with a look-back chain (scan,
+// reduce, reduce_by_key) take the tile's index within the job from the
+// job's own atomic counter, so a tile's predecessors always belong to
+// workgroups that already run or are done; the counters of different jobs
+// are different addresses, so claims do not serialise across jobs (one
+// device-wide counter would cost ~11 ns per tile on MI355X).  Merge and sort
+// tiles are independent and use the workgroup index directly.  (A
 // persistent loop here would keep the tile body's per-lane address math live
 // across iterations and cost ~40 registers per lane.)
 template <int FAM>
 __global__ __launch_bounds__(kBatchBlock) void batch_phase0_kernel(
-    BatchDesc* d, const BatchJobDev* __restrict__ jobs, const uint32_t* __restrict__ tile_job) {
+    BatchDesc* d, const BatchJobDev* __restrict__ jobs, const uint32_t* __restrict__ tile_job,
+    uint32_t* __restrict__ job_ctr) {
   __shared__ FamilySmem<FAM> sm;
-  __shared__ uint32_t s_t;
   const int64_t begin = d->fam_begin[FAM];
   const int64_t count = d->fam_begin[FAM + 1] - begin;
-  if (threadIdx.x == 0) s_t = atomicAdd(&d->counter[FAM], 1u);
-  __syncthreads();
-  const int64_t t = wave_uniform(s_t);
-  if (t >= count) return;
-  const BatchJobDev& job = jobs[tile_job[begin + t]];
-  run_tile<FAM>(sm, d, job, begin + t - job.tile_begin0);
+  const int64_t g = int64_t(blockIdx.x);
+  if (g >= count) return;
+  const uint32_t j = tile_job[begin + g];
+  const BatchJobDev& job = jobs[j];
+  int64_t lt;
+  if constexpr (FAM == FAM_MERGE || FAM == FAM_SORT) {
+    lt = begin + g - job.tile_begin0;
+  } else {
+    __shared__ uint32_t s_t;
+    if (threadIdx.x == 0) s_t = atomicAdd(&job_ctr[j], 1u);
+    __syncthreads();
+    lt = wave_uniform(s_t);
+  }
+  run_tile<FAM>(sm, d, job, lt);
 }
 
 template <class K>
@@ -390,6 +403,7 @@ struct bulkx_batch {
   uint32_t* d_tile_job = nullptr;
   uint32_t* d_sort_tile_job = nullptr;
   int64_t* d_splits = nullptr;  // merge-path splits of the current merge round
+  uint32_t* d_job_ctr = nullptr;  // per-job tile counters of the look-back families
   uint64_t* d_status = nullptr;
   char* d_tmp = nullptr;
   size_t status_words_cap = 0;
@@ -428,7 +442,7 @@ static unsigned batch_grid(const bulkx_batch* b, int kind) {
   return unsigned(g > 0 ? g : 1);
 }
 
-using Phase0Kernel = void (*)(BatchDesc*, const BatchJobDev*, const uint32_t*);
+using Phase0Kernel = void (*)(BatchDesc*, const BatchJobDev*, const uint32_t*, uint32_t*);
 static const Phase0Kernel kPhase0[FAM_COUNT] = {
     batch_phase0_kernel<FAM_SCAN_I32>, batch_phase0_kernel<FAM_SCAN_F32>,
     batch_phase0_kernel<FAM_RBK_I32>,  batch_phase0_kernel<FAM_RBK_F32>,
@@ -436,11 +450,12 @@ static const Phase0Kernel kPhase0[FAM_COUNT] = {
     batch_phase0_kernel<FAM_SORT>};
 
 static int batch_enqueue_kernels(bulkx_batch* b, hipStream_t s, bool timed) {
-  batch_clear_kernel<<<256, 256, 0, s>>>(b->d_desc, b->d_status);
+  batch_clear_kernel<<<256, 256, 0, s>>>(b->d_desc, b->d_status, b->d_job_ctr);
   BULKX_LAUNCHED();
   for (int f = 0; f < FAM_COUNT; ++f) {
     if (timed) BULKX_HIP(hipEventRecord(b->ev[f], s));
-    kPhase0[f]<<<batch_grid(b, f), kBatchBlock, 0, s>>>(b->d_desc, b->d_jobs, b->d_tile_job);
+    kPhase0[f]<<<batch_grid(b, f), kBatchBlock, 0, s>>>(b->d_desc, b->d_jobs, b->d_tile_job,
+                                                        b->d_job_ctr);
     BULKX_LAUNCHED();
   }
   if (timed) BULKX_HIP(hipEventRecord(b->ev[FAM_COUNT], s));
@@ -499,6 +514,7 @@ extern "C" int bulkx_batch_create(int max_jobs, int64_t max_elements, void* stre
   dmalloc(reinterpret_cast<void**>(&b->d_tile_job), sizeof(uint32_t) * b->max_tiles);
   dmalloc(reinterpret_cast<void**>(&b->d_sort_tile_job), sizeof(uint32_t) * b->max_tiles);
   dmalloc(reinterpret_cast<void**>(&b->d_splits), sizeof(int64_t) * b->max_tiles);
+  dmalloc(reinterpret_cast<void**>(&b->d_job_ctr), sizeof(uint32_t) * max_jobs);
   dmalloc(reinterpret_cast<void**>(&b->d_status), sizeof(uint64_t) * b->status_words_cap);
   dmalloc(reinterpret_cast<void**>(&b->d_ws), b->ws_bytes);
   hmalloc(reinterpret_cast<void**>(&b->h_desc), sizeof(BatchDesc));
@@ -536,7 +552,7 @@ extern "C" int bulkx_batch_destroy(bulkx_batch* b) {
     if (e) (void)hipEventDestroy(e);
   for (void* p : {static_cast<void*>(b->d_desc), static_cast<void*>(b->d_jobs),
                   static_cast<void*>(b->d_tile_job), static_cast<void*>(b->d_sort_tile_job),
-                  static_cast<void*>(b->d_splits),
+                  static_cast<void*>(b->d_splits), static_cast<void*>(b->d_job_ctr),
                   static_cast<void*>(b->d_status), static_cast<void*>(b->d_tmp),
                   static_cast<void*>(b->d_ws)})
     if (p) (void)hipFree(p);
@@ -686,6 +702,7 @@ extern "C" int bulkx_batch_set_jobs(bulkx_batch* b, const bulkx_job* jobs, int n
   const int64_t t0 = b->h_desc->fam_begin[FAM_COUNT];
   for (int f = 0; f < FAM_COUNT; ++f) b->fam_tiles[f] = fam_tiles[f];
   b->h_desc->total_sort_tiles = ts;
+  b->h_desc->num_jobs = num_jobs;
   b->h_desc->num_status_words = int64_t(sw);
   b->total_tiles0 = t0;
   b->total_sort_tiles = ts;

@@ -72,15 +72,24 @@ struct RowSegOp {
   }
 };
 
-// Tile shapes: variant 0 = 512 x 16 (default), 1 = 512 x 8, 2 = 256 x 16.
+// Tile shapes: variant 0 = 512 x 16 (default), 1 = 512 x 8, 2 = 256 x 16,
+// 3 = 1024 x 16, 4 = 1024 x 8.
 template <class K, class V, int VAR = 0>
 struct RbkConfig {
-  static constexpr int BLOCK = VAR == 2 ? 256 : 512;
-  static constexpr int ITEMS = VAR == 1 ? 8 : 16;
+  static constexpr int BLOCK = VAR == 2 ? 256 : VAR >= 3 ? 1024 : 512;
+  static constexpr int ITEMS = (VAR == 1 || VAR == 4) ? 8 : 16;
   static constexpr int TILE = BLOCK * ITEMS;
 };
-constexpr int kRbkVariants = 3;
+constexpr int kRbkVariants = 5;
 constexpr int kRbkMinTile = 4096;  // smallest tile of any variant (workspace sizing)
+
+// Look-back window of 64*Q tiles (Q tiles per lane).  Q = 1 measured
+// fastest on MI355X at 2^28 elements (Q = 2, 4, 8: 8 %, 90 %, 130 % slower).
+#ifndef BULKX_RBK_LOOKBACK_Q
+#define BULKX_RBK_LOOKBACK_Q 1
+#endif
+constexpr int kRbkLookbackQ = BULKX_RBK_LOOKBACK_Q;
+
 
 constexpr size_t kRbkHeaderBytes = 256;
 constexpr int kRbkMaxStatusWords = 6;  // Seg<dd> = 24 bytes
@@ -237,38 +246,37 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
     tile_agg = SOp::apply(tile_agg, x);
   }
 
-  S tile_prefix = SOp::template identity<S>();
-  if (!single_tile) {
-    if (tile == 0) {
-      if (threadIdx.x == 0) st.publish(0, kStatusPrefix, tile_agg);
-    } else {
-      if (threadIdx.x == 0) st.publish(tile, kStatusAggregate, tile_agg);
-      if (threadIdx.x < kWave) {
-        S excl = lookback_exclusive_prefix<SOp>(st, tile, err);
-        if (threadIdx.x == 0) {
-          st.publish(tile, kStatusPrefix, SOp::apply(excl, tile_agg));
-          sm.prefix = excl;
-        }
-      }
-      __syncthreads();
-      tile_prefix = sm.prefix;
+  // Publish this tile's aggregate (or, for tile 0, its inclusive prefix),
+  // then let wavefront 0 look back while the other wavefronts already walk
+  // their chunks: the walk only needs the in-tile prefix, except for the
+  // value of the tile's first run when that run began in an earlier tile,
+  // which is patched once the tile's prefix is known.
+  const bool chained = !single_tile && tile > 0;
+  if (!single_tile && threadIdx.x == 0)
+    st.publish(tile, tile == 0 ? kStatusPrefix : kStatusAggregate, tile_agg);
+  if (chained && threadIdx.x < kWave) {
+    S excl = lookback_exclusive_prefix<SOp, kRbkLookbackQ>(st, tile, err);
+    if (threadIdx.x == 0) {
+      st.publish(tile, kStatusPrefix, SOp::apply(excl, tile_agg));
+      sm.prefix = excl;
     }
   }
-  // The tile holding element n-1 reports the run count.
-  if (threadIdx.x == 0 && tile * TILE + TILE >= n)
-    *num_runs = int64_t(tile_prefix.c + tile_agg.c);
 
   // Walk: a tail's run has slot (heads so far) - 1.  The chunk's tails hold
   // consecutive slots from base (the run open at the chunk's first element),
-  // so slot - base < 64*ITEMS is a 32-bit stage index.
-  const S p = SOp::apply(SOp::apply(tile_prefix, wave_prefix), lift(lane_prefix));
-  const uint64_t base = readlane(p.c + (h & 1u) - 1, 0);
+  // so slot - base < 64*ITEMS is a 32-bit stage index.  Slots are kept
+  // relative to the tile (base_rel) until the tile's prefix arrives.
+  const S q = SOp::apply(wave_prefix, lift(lane_prefix));  // in-tile prefix of the lane
+  const int64_t base_rel = readlane(int64_t(q.c) + int64_t(h & 1u) - 1, 0);
   const uint32_t ntails = wave_reduce<OpPlus>(uint32_t(__builtin_popcount(t)));
   K* sk = lk;
   V* sv = lv;
-  uint32_t e = uint32_t(p.c - base);
+  uint32_t e = uint32_t(int64_t(q.c) - base_rel);
   V local = Op::template identity<V>();
   bool started = false;  // a run began inside this lane's slice
+  bool fix = false;      // this lane holds the tail of the run carried into the tile
+  uint32_t fix_slot = 0;
+  A fix_local = AOp::template identity<A>();
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     const bool hi = (h >> i) & 1u;
@@ -279,15 +287,31 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
       const uint32_t slot = e - 1;
       if (slot < uint32_t(N)) {
         sk[slot] = k[i];
-        sv[slot] = started ? local : AccP::to(AOp::apply(p.v, AccP::from(local)));
+        if (started) {
+          sv[slot] = local;
+        } else if (q.c > 0) {
+          sv[slot] = AccP::to(AOp::apply(q.v, AccP::from(local)));
+        } else {
+          fix = true;
+          fix_slot = slot;
+          fix_local = AccP::from(local);
+        }
       }
     }
   }
+  __syncthreads();  // the tile's prefix is in sm.prefix
+  const S tile_prefix = chained ? sm.prefix : SOp::template identity<S>();
+  // The tile holding element n-1 reports the run count.
+  if (threadIdx.x == 0 && tile * TILE + TILE >= n)
+    *num_runs = int64_t(tile_prefix.c + tile_agg.c);
+  // same association as folding the full prefix first: (tile_prefix . q) . local
+  if (fix) sv[fix_slot] = AccP::to(AOp::apply(AOp::apply(tile_prefix.v, q.v), fix_local));
   wave_lds_sync();
+  const int64_t base = int64_t(tile_prefix.c) + base_rel;
   // bounds guard: slots stay below n when the scan is consistent
   uint32_t cnt = ntails < uint32_t(N) ? ntails : uint32_t(N);
-  if (base >= uint64_t(n)) cnt = 0;
-  else if (base + cnt > uint64_t(n)) cnt = uint32_t(uint64_t(n) - base);
+  if (base < 0 || base >= n) cnt = 0;
+  else if (base + int64_t(cnt) > n) cnt = uint32_t(n - base);
   for (uint32_t x = lane; x < cnt; x += kWave) {
     kout[base + x] = sk[x];
     vout[base + x] = sv[x];
@@ -302,14 +326,15 @@ __global__ __launch_bounds__(BLOCK) void reduce_by_key_kernel(
     bool single_tile) {
   __shared__ RbkSmem<K, V, Op, BLOCK, ITEMS> sm;
   __shared__ uint32_t s_tile;
-  int64_t tile = 0;
-  if (!single_tile) {
-    if (threadIdx.x == 0) s_tile = atomicAdd(header, 1u);
-    __syncthreads();
-    tile = wave_uniform(s_tile);
+  if (single_tile) {
+    rbk_tile<K, V, Op, BLOCK, ITEMS>(sm, keys, vals, n, kout, vout, num_runs, 0, words,
+                                     header + 1, true);
+    return;
   }
-  rbk_tile<K, V, Op, BLOCK, ITEMS>(sm, keys, vals, n, kout, vout, num_runs, tile, words,
-                                   header + 1, single_tile);
+  if (threadIdx.x == 0) s_tile = atomicAdd(header, 1u);
+  __syncthreads();
+  rbk_tile<K, V, Op, BLOCK, ITEMS>(sm, keys, vals, n, kout, vout, num_runs,
+                                   int64_t(wave_uniform(s_tile)), words, header + 1, false);
 }
 
 __global__ void rbk_empty_kernel(int64_t* num_runs);
@@ -363,9 +388,24 @@ int rbk_dispatch_values(int vdtype, int op, const void* keys, const void* vals, 
     V* vo = static_cast<V*>(vout);
     // the other shapes are instantiated for plus only (min / max use the default)
     if constexpr (std::is_same<OP, OpPlus>::value) {
-      if (variant == 1) st = rbk_launch<K, V, OP, 1>(k, v, n, ko, vo, num_runs, ws, stream);
-      else if (variant == 2) st = rbk_launch<K, V, OP, 2>(k, v, n, ko, vo, num_runs, ws, stream);
-      else st = rbk_launch<K, V, OP, 0>(k, v, n, ko, vo, num_runs, ws, stream);
+      switch (variant) {
+        case 1: st = rbk_launch<K, V, OP, 1>(k, v, n, ko, vo, num_runs, ws, stream); break;
+        case 2: st = rbk_launch<K, V, OP, 2>(k, v, n, ko, vo, num_runs, ws, stream); break;
+        // 1024-lane tiles fit the LDS only with 4-byte keys and values
+        case 3:
+          if constexpr (sizeof(K) == 4 && sizeof(V) == 4)
+            st = rbk_launch<K, V, OP, 3>(k, v, n, ko, vo, num_runs, ws, stream);
+          else
+            st = rbk_launch<K, V, OP, 0>(k, v, n, ko, vo, num_runs, ws, stream);
+          break;
+        case 4:
+          if constexpr (sizeof(K) == 4 && sizeof(V) == 4)
+            st = rbk_launch<K, V, OP, 4>(k, v, n, ko, vo, num_runs, ws, stream);
+          else
+            st = rbk_launch<K, V, OP, 0>(k, v, n, ko, vo, num_runs, ws, stream);
+          break;
+        default: st = rbk_launch<K, V, OP, 0>(k, v, n, ko, vo, num_runs, ws, stream); break;
+      }
     } else {
       st = rbk_launch<K, V, OP, 0>(k, v, n, ko, vo, num_runs, ws, stream);
     }

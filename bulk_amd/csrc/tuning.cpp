@@ -177,9 +177,15 @@ int variant_shape(int algo, int dtype, int64_t n, int v, LaunchShape* out) {
       } else if (v == 1) {
         s.group_size = RbkConfig<int, int, 1>::BLOCK;
         s.grain_size = RbkConfig<int, int, 1>::ITEMS;
-      } else {
+      } else if (v == 2) {
         s.group_size = RbkConfig<int, int, 2>::BLOCK;
         s.grain_size = RbkConfig<int, int, 2>::ITEMS;
+      } else if (v == 3) {
+        s.group_size = RbkConfig<int, int, 3>::BLOCK;
+        s.grain_size = RbkConfig<int, int, 3>::ITEMS;
+      } else {
+        s.group_size = RbkConfig<int, int, 4>::BLOCK;
+        s.grain_size = RbkConfig<int, int, 4>::ITEMS;
       }
       const int64_t tile = int64_t(s.group_size) * s.grain_size;
       s.num_groups = (n + tile - 1) / tile;
