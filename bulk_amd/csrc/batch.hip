@@ -73,7 +73,11 @@ enum BatchFamily : int32_t {
   FAM_COUNT
 };
 constexpr int kBatchStages = FAM_COUNT + 1;  // the family kernels, then the sort merge rounds
-constexpr int kGridPartition = FAM_COUNT, kGridPass = FAM_COUNT + 1, kGridKinds = FAM_COUNT + 2;
+// Launch-grid kinds: one per phase-0 family, then (partition, pass) for
+// every merge round.
+constexpr int kGridKinds = FAM_COUNT + 2 * kBatchMaxPasses;
+constexpr int grid_kind_partition(int round) { return FAM_COUNT + 2 * round; }
+constexpr int grid_kind_pass(int round) { return FAM_COUNT + 2 * round + 1; }
 
 struct BatchDesc {
   int64_t fam_begin[FAM_COUNT + 1];  // family f owns tiles [fam_begin[f], fam_begin[f+1])
@@ -419,6 +423,9 @@ struct bulkx_batch {
   std::vector<bulkx_job> jobs;
   int num_jobs = 0;
   int64_t total_tiles0 = 0, total_sort_tiles = 0;
+  // sort tiles still merging in round r (sort tiles are ordered by rounds,
+  // most first, so they are a prefix of the sort tile table)
+  int64_t active_sort_tiles[kBatchMaxPasses] = {};
   // stage timing of eager launches: ev[f]..ev[f+1] brackets family f,
   // ev[FAM_COUNT]..ev[FAM_COUNT+1] the sort merge rounds
   hipEvent_t ev[kBatchStages + 1] = {};
@@ -431,14 +438,18 @@ struct bulkx_batch {
 };
 
 // Grid of each job-list-sized launch: the phase-0 families (one workgroup per
-// tile), the merge-round partition (one wavefront per sort tile) and the
-// merge rounds (one workgroup per sort tile).  Never 0: an idle launch runs
-// one workgroup that finds no work.
+// tile), and per merge round the partition (one wavefront per sort tile still
+// merging) and the round itself (one workgroup per such tile).  Never 0: an
+// idle launch runs one workgroup that finds no work.
 static unsigned batch_grid(const bulkx_batch* b, int kind) {
   int64_t g;
-  if (kind < FAM_COUNT) g = b->fam_tiles[kind];
-  else if (kind == kGridPartition) g = (b->total_sort_tiles + 3) / 4;
-  else g = b->total_sort_tiles;
+  if (kind < FAM_COUNT) {
+    g = b->fam_tiles[kind];
+  } else {
+    const int r = (kind - FAM_COUNT) / 2;
+    const int64_t active = b->active_sort_tiles[r];
+    g = ((kind - FAM_COUNT) & 1) ? active : (active + 3) / 4;
+  }
   return unsigned(g > 0 ? g : 1);
 }
 
@@ -460,10 +471,10 @@ static int batch_enqueue_kernels(bulkx_batch* b, hipStream_t s, bool timed) {
   }
   if (timed) BULKX_HIP(hipEventRecord(b->ev[FAM_COUNT], s));
   for (int p = 0; p < b->max_passes; ++p) {
-    batch_partition_kernel<<<batch_grid(b, kGridPartition), 256, 0, s>>>(
+    batch_partition_kernel<<<batch_grid(b, grid_kind_partition(p)), 256, 0, s>>>(
         b->d_desc, b->d_jobs, b->d_sort_tile_job, b->d_splits, p);
     BULKX_LAUNCHED();
-    batch_pass_kernel<<<batch_grid(b, kGridPass), kBatchBlock, 0, s>>>(
+    batch_pass_kernel<<<batch_grid(b, grid_kind_pass(p)), kBatchBlock, 0, s>>>(
         b->d_desc, b->d_jobs, b->d_sort_tile_job, b->d_splits, p);
     BULKX_LAUNCHED();
   }
@@ -655,6 +666,7 @@ extern "C" int bulkx_batch_set_jobs(bulkx_batch* b, const bulkx_job* jobs, int n
   }
   int64_t ts = 0;
   size_t sw = 0, tb = 0;
+  std::vector<int> sort_jobs;
   for (int i = 0; i < num_jobs; ++i) {
     const bulkx_job& j = jobs[i];
     const int op = batch_op(j);
@@ -686,18 +698,29 @@ extern "C" int bulkx_batch_set_jobs(bulkx_batch* b, const bulkx_job* jobs, int n
     }
     if (op >= OP_SORT_I32) {
       dj.passes = passes_for(t);
-      dj.sort_tile_begin = ts;
       if (t > 1) {
         dj.tmp_k = b->d_tmp + tb;
         tb += align_up(size_t(n) * 4, 256);
         dj.tmp_v = b->d_tmp + tb;
         tb += align_up(size_t(n) * 4, 256);
       }
-      for (int64_t k = 0; k < t; ++k) b->h_sort_tile_job[ts + k] = uint32_t(i);
-      ts += t;
+      sort_jobs.push_back(i);
     }
     for (int64_t k = 0; k < t; ++k) b->h_tile_job[cursor[f] + k] = uint32_t(i);
     cursor[f] += t;
+  }
+  // Sort tiles ordered by merge rounds, most first: the tiles still merging
+  // in round r are then a prefix of the table, and each round's launch
+  // covers exactly them.
+  std::stable_sort(sort_jobs.begin(), sort_jobs.end(),
+                   [&](int x, int y) { return b->h_jobs[x].passes > b->h_jobs[y].passes; });
+  for (int r = 0; r < kBatchMaxPasses; ++r) b->active_sort_tiles[r] = 0;
+  for (int i : sort_jobs) {
+    BatchJobDev& dj = b->h_jobs[i];
+    dj.sort_tile_begin = ts;
+    for (int64_t k = 0; k < dj.tiles0; ++k) b->h_sort_tile_job[ts + k] = uint32_t(i);
+    ts += dj.tiles0;
+    for (int r = 0; r < dj.passes && r < kBatchMaxPasses; ++r) b->active_sort_tiles[r] = ts;
   }
   const int64_t t0 = b->h_desc->fam_begin[FAM_COUNT];
   for (int f = 0; f < FAM_COUNT; ++f) b->fam_tiles[f] = fam_tiles[f];
@@ -756,8 +779,14 @@ static int batch_capture(bulkx_batch* b) {
     int kind = -1;
     for (int f = 0; f < FAM_COUNT; ++f)
       if (kp.func == reinterpret_cast<void*>(kPhase0[f])) kind = f;
-    if (kp.func == reinterpret_cast<void*>(batch_partition_kernel)) kind = kGridPartition;
-    if (kp.func == reinterpret_cast<void*>(batch_pass_kernel)) kind = kGridPass;
+    // the merge-round kernels' last argument is the round
+    if (kp.func == reinterpret_cast<void*>(batch_partition_kernel) ||
+        kp.func == reinterpret_cast<void*>(batch_pass_kernel)) {
+      const int round = *static_cast<const int*>(kp.kernelParams[4]);
+      if (round < 0 || round >= kBatchMaxPasses) return BULKX_ERR_INTERNAL;
+      kind = kp.func == reinterpret_cast<void*>(batch_pass_kernel) ? grid_kind_pass(round)
+                                                                   : grid_kind_partition(round);
+    }
     if (kind >= 0) b->sized_nodes.emplace_back(node, kind);
   }
   BULKX_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
