@@ -76,6 +76,7 @@ constexpr int kBatchStages = FAM_COUNT + 1;  // the family kernels, then the sor
 // Launch-grid kinds: one per phase-0 family, then (partition, pass) for
 // every merge round.
 constexpr int kGridKinds = FAM_COUNT + 2 * kBatchMaxPasses;
+constexpr int kBatchBranches = 4;
 constexpr int grid_kind_partition(int round) { return FAM_COUNT + 2 * round; }
 constexpr int grid_kind_pass(int round) { return FAM_COUNT + 2 * round + 1; }
 
@@ -396,6 +397,10 @@ struct bulkx_batch {
   hipStream_t stream = nullptr;
   hipStream_t capture_stream = nullptr;  // graphs are recorded here (capture is not
                                          // permitted on the null stream)
+  // the graph's concurrent branches (sort chain, merge, reduce_by_key, scan)
+  hipStream_t side[kBatchBranches] = {};
+  hipEvent_t fork_ev = nullptr;
+  hipEvent_t join_ev[kBatchBranches] = {};
   int max_jobs = 0;
   int64_t max_elements = 0;
   int64_t max_tiles = 0;
@@ -482,6 +487,48 @@ static int batch_enqueue_kernels(bulkx_batch* b, hipStream_t s, bool timed) {
   return BULKX_SUCCESS;
 }
 
+// The graph form: after the clear kernel the work forks into independent
+// branches -- the sort chain (block sort, then the merge rounds), merge,
+// reduce_by_key and scan -- which the device runs concurrently, so the
+// compute-bound sort block and the look-back stalls of reduce_by_key overlap
+// the memory-bound families.  Branches share only read-only tables and the
+// error flag.
+static int batch_enqueue_graph(bulkx_batch* b, hipStream_t s) {
+  batch_clear_kernel<<<256, 256, 0, s>>>(b->d_desc, b->d_status, b->d_job_ctr);
+  BULKX_LAUNCHED();
+  BULKX_HIP(hipEventRecord(b->fork_ev, s));
+  for (int br = 0; br < kBatchBranches; ++br) BULKX_HIP(hipStreamWaitEvent(b->side[br], b->fork_ev, 0));
+  auto family = [&](int f, hipStream_t q) {
+    kPhase0[f]<<<batch_grid(b, f), kBatchBlock, 0, q>>>(b->d_desc, b->d_jobs, b->d_tile_job,
+                                                        b->d_job_ctr);
+  };
+  family(FAM_SORT, b->side[0]);
+  BULKX_LAUNCHED();
+  for (int p = 0; p < b->max_passes; ++p) {
+    batch_partition_kernel<<<batch_grid(b, grid_kind_partition(p)), 256, 0, b->side[0]>>>(
+        b->d_desc, b->d_jobs, b->d_sort_tile_job, b->d_splits, p);
+    BULKX_LAUNCHED();
+    batch_pass_kernel<<<batch_grid(b, grid_kind_pass(p)), kBatchBlock, 0, b->side[0]>>>(
+        b->d_desc, b->d_jobs, b->d_sort_tile_job, b->d_splits, p);
+    BULKX_LAUNCHED();
+  }
+  family(FAM_MERGE, b->side[1]);
+  BULKX_LAUNCHED();
+  for (int f : {int(FAM_RBK_I32), int(FAM_RBK_F32), int(FAM_RBK_I64)}) {
+    family(f, b->side[2]);
+    BULKX_LAUNCHED();
+  }
+  for (int f : {int(FAM_SCAN_I32), int(FAM_SCAN_F32)}) {
+    family(f, b->side[3]);
+    BULKX_LAUNCHED();
+  }
+  for (int br = 0; br < kBatchBranches; ++br) {
+    BULKX_HIP(hipEventRecord(b->join_ev[br], b->side[br]));
+    BULKX_HIP(hipStreamWaitEvent(s, b->join_ev[br], 0));
+  }
+  return BULKX_SUCCESS;
+}
+
 static void batch_drop_graph(bulkx_batch* b);
 
 extern "C" int bulkx_batch_create(int max_jobs, int64_t max_elements, void* stream,
@@ -539,6 +586,11 @@ extern "C" int bulkx_batch_create(int max_jobs, int64_t max_elements, void* stre
   memset(b->h_desc, 0, sizeof(BatchDesc));
   hipError_t r = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking);
   for (int i = 0; i <= kBatchStages && r == hipSuccess; ++i) r = hipEventCreate(&b->ev[i]);
+  for (int i = 0; i < kBatchBranches && r == hipSuccess; ++i) {
+    r = hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking);
+    if (r == hipSuccess) r = hipEventCreateWithFlags(&b->join_ev[i], hipEventDisableTiming);
+  }
+  if (r == hipSuccess) r = hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming);
   if (r != hipSuccess) {
     set_last_error("hipStreamCreateWithFlags", r);
     bulkx_batch_destroy(b);
@@ -559,6 +611,11 @@ extern "C" int bulkx_batch_destroy(bulkx_batch* b) {
   (void)hipStreamSynchronize(b->stream);
   batch_drop_graph(b);
   if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
+  for (int i = 0; i < kBatchBranches; ++i) {
+    if (b->side[i]) (void)hipStreamDestroy(b->side[i]);
+    if (b->join_ev[i]) (void)hipEventDestroy(b->join_ev[i]);
+  }
+  if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
   for (hipEvent_t e : b->ev)
     if (e) (void)hipEventDestroy(e);
   for (void* p : {static_cast<void*>(b->d_desc), static_cast<void*>(b->d_jobs),
@@ -754,7 +811,7 @@ static void batch_drop_graph(bulkx_batch* b) {
 // and finds the phase-0 kernel nodes, whose grids follow the job list.
 static int batch_capture(bulkx_batch* b) {
   BULKX_HIP(hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal));
-  int st = batch_enqueue_kernels(b, b->capture_stream, false);
+  int st = batch_enqueue_graph(b, b->capture_stream);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(b->capture_stream, &g);
   if (st != BULKX_SUCCESS) {
