@@ -276,12 +276,12 @@ struct PassBuffers {
 };
 
 // Merge paths of the first diagonal of every sort tile of round `pass`,
-// one wavefront per tile (tiles of jobs that need fewer rounds are skipped).
+// one lane per tile (tiles of jobs that need fewer rounds are skipped).
 __global__ __launch_bounds__(256) void batch_partition_kernel(
     BatchDesc* d, const BatchJobDev* __restrict__ jobs, const uint32_t* __restrict__ sort_tile_job,
     int64_t* __restrict__ splits, int pass) {
-  const int64_t t = int64_t(blockIdx.x) * (256 / kWave) + threadIdx.x / kWave;
-  if (t >= d->total_sort_tiles) return;  // uniform per wavefront
+  const int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (t >= d->total_sort_tiles) return;
   const BatchJobDev& job = jobs[sort_tile_job[t]];
   if (pass >= job.passes) return;
   const int64_t lt = t - job.sort_tile_begin;
@@ -301,7 +301,7 @@ __global__ __launch_bounds__(256) void batch_partition_kernel(
                                                       width, lt);
       break;
   }
-  if (lane_id() == 0) splits[t] = r;
+  splits[t] = r;
 }
 
 template <class K>
@@ -453,7 +453,7 @@ static unsigned batch_grid(const bulkx_batch* b, int kind) {
   } else {
     const int r = (kind - FAM_COUNT) / 2;
     const int64_t active = b->active_sort_tiles[r];
-    g = ((kind - FAM_COUNT) & 1) ? active : (active + 3) / 4;
+    g = ((kind - FAM_COUNT) & 1) ? active : (active + 255) / 256;
   }
   return unsigned(g > 0 ? g : 1);
 }

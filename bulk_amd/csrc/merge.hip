@@ -1,8 +1,8 @@
 // merge / merge_by_key: replaces my_merge (merge.cu:127-162) and the
 // device-wide use of bulk::merge_by_key (merge.hpp:507-607).  Two launches,
 // as the reference's locate_merge_path + merge_kernel (merge.cu:100-124,
-// 39-97): a partition kernel finds every tile boundary (one wavefront per
-// boundary, 64-ary search), then each workgroup stages its two input slices
+// 39-97): a partition kernel finds every tile boundary (one lane per
+// boundary, binary search), then each workgroup stages its two input slices
 // in LDS and merges them.  Splitting the search out keeps its chain of
 // dependent loads off the merge's critical path.
 #include "merge_impl.hpp"
@@ -16,12 +16,11 @@ __global__ __launch_bounds__(256) void merge_split_kernel(const K* __restrict__ 
                                                           const K* __restrict__ b, int64_t n2,
                                                           int64_t tiles,
                                                           int64_t* __restrict__ splits) {
-  const int64_t t = int64_t(blockIdx.x) * (256 / kWave) + threadIdx.x / kWave;
-  if (t > tiles) return;  // uniform per wavefront
+  const int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (t > tiles) return;
   const int64_t n = n1 + n2;
   const int64_t d = t * TILE < n ? t * TILE : n;
-  const int64_t r = wave_merge_path<CMP>(a, n1, b, n2, d);
-  if (lane_id() == 0) splits[t] = r;
+  splits[t] = lane_merge_path<CMP>(a, n1, b, n2, d);
 }
 
 template <class CMP, class K, class V, bool HAS_V, int BLOCK, int ITEMS>
@@ -55,7 +54,7 @@ static int merge_launch_v(const void* k1, const void* v1, int64_t n1, const void
   int64_t* splits = nullptr;
   if (tiles > 1) {
     splits = reinterpret_cast<int64_t*>(ws);
-    const int64_t groups = (tiles + 1 + 3) / 4;
+    const int64_t groups = (tiles + 1 + 255) / 256;
     merge_split_kernel<CMP, K, C::TILE><<<dim3(uint32_t(groups)), 256, 0, stream>>>(
         static_cast<const K*>(k1), n1, static_cast<const K*>(k2), n2, tiles, splits);
     BULKX_LAUNCHED();

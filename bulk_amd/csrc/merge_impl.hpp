@@ -66,12 +66,31 @@ __device__ int64_t wave_merge_path(const K* a, int64_t n1, const K* b, int64_t n
   return lo;
 }
 
+// merge_path searched by one lane: a plain binary search.  Used where many
+// boundaries are searched at once (the partition kernels): each search
+// touches about 2*log2(n) cache lines, against ~128 per step of the
+// wave-cooperative search, and the dependent loads of different lanes
+// overlap in flight.
+template <class CMP, class K>
+__device__ __forceinline__ int64_t lane_merge_path(const K* a, int64_t n1, const K* b, int64_t n2,
+                                                   int64_t diag) {
+  int64_t lo = diag > n2 ? diag - n2 : 0;
+  int64_t hi = diag < n1 ? diag : n1;
+  while (lo < hi) {
+    const int64_t m = (lo + hi) >> 1;
+    if (CMP::apply(b[diag - 1 - m], a[m])) hi = m;
+    else lo = m + 1;
+  }
+  return lo;
+}
+
 // Padded LDS index: one pad element per 128 bytes, so that lanes reading
 // elements a multiple of 32 words apart (blocked per-thread layouts, merge
 // paths of neighbouring lanes) fall into different banks.
+// Indices are non-negative, so the division is a shift.
 template <class T>
 __device__ __forceinline__ int lpad(int e) {
-  return e + e / (128 / int(sizeof(T)));
+  return e + int(unsigned(e) / unsigned(128 / sizeof(T)));
 }
 // LDS elements a padded array of n logical elements needs.
 template <class T>
@@ -97,21 +116,33 @@ __device__ __forceinline__ int lds_merge_path(const K* s, int base, int n1, int 
 // Sequential agent merge of up to ITEMS outputs from padded LDS runs
 // A = [i1, e1), B = [i2, e2) (logical indices; the merge.hpp:109-129 body:
 // take A unless B's head compares strictly less).  Produces keys and the
-// logical LDS index they came from.
-template <class CMP, class K, int ITEMS>
+// logical LDS index they came from.  Both heads stay in registers, so each
+// step reads one element (the successor of the head it consumed): LDS reads
+// at data-dependent addresses are the bank-conflict-bound part of a merge.
+// A read past the end of its run is clamped to LAST (the stage's last
+// logical index) and never used, so it needs no branch; outputs past the
+// caller's count are computed the same way and ignored by the caller.
+template <class CMP, class K, int ITEMS, int LAST>
 __device__ __forceinline__ void lds_serial_merge(const K* s, int i1, int e1, int i2, int e2,
-                                                 int count, K (&rk)[ITEMS], int (&ri)[ITEMS]) {
+                                                 K (&rk)[ITEMS], int (&ri)[ITEMS]) {
+  K ka = s[lpad<K>(i1 < LAST ? i1 : LAST)];
+  K kb = s[lpad<K>(i2 < LAST ? i2 : LAST)];
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
-    if (i < count) {
-      const bool has1 = i1 < e1, has2 = i2 < e2;
-      const K ka = has1 ? s[lpad<K>(i1)] : K();
-      const K kb = has2 ? s[lpad<K>(i2)] : K();
-      const bool p = !has2 || (has1 && !CMP::apply(kb, ka));
-      rk[i] = p ? ka : kb;
-      ri[i] = p ? i1 : i2;
-      if (p) ++i1;
-      else ++i2;
+    const bool has1 = i1 < e1, has2 = i2 < e2;
+    const bool p = !has2 || (has1 && !CMP::apply(kb, ka));
+    rk[i] = p ? ka : kb;
+    ri[i] = p ? i1 : i2;
+    if (i + 1 < ITEMS) {
+      const int nx = (p ? i1 : i2) + 1;
+      const K kn = s[lpad<K>(nx < LAST ? nx : LAST)];
+      if (p) {
+        i1 = nx;
+        ka = kn;
+      } else {
+        i2 = nx;
+        kb = kn;
+      }
     }
   }
 }
@@ -168,7 +199,7 @@ __device__ __forceinline__ void merge_tile(const K* a, const V* va, int na, cons
   V rv[ITEMS];
   int ri[ITEMS];
   const int count = cnt - d < ITEMS ? cnt - d : ITEMS;
-  lds_serial_merge<CMP, K, ITEMS>(s_k, ai, na, bi, cnt, count, rk, ri);
+  lds_serial_merge<CMP, K, ITEMS, TILE - 1>(s_k, ai, na, bi, cnt, rk, ri);
   if constexpr (HAS_V) {
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) rv[i] = i < count ? s_v[lpad<V>(ri[i])] : V();

@@ -73,14 +73,14 @@ struct RowSegOp {
 };
 
 // Tile shapes: variant 0 = 512 x 16 (default), 1 = 512 x 8, 2 = 256 x 16,
-// 3 = 1024 x 16, 4 = 1024 x 8.
+// 3 = 1024 x 16, 4 = 1024 x 8, 5 = 256 x 32.
 template <class K, class V, int VAR = 0>
 struct RbkConfig {
-  static constexpr int BLOCK = VAR == 2 ? 256 : VAR >= 3 ? 1024 : 512;
-  static constexpr int ITEMS = (VAR == 1 || VAR == 4) ? 8 : 16;
+  static constexpr int BLOCK = (VAR == 2 || VAR == 5) ? 256 : (VAR == 3 || VAR == 4) ? 1024 : 512;
+  static constexpr int ITEMS = (VAR == 1 || VAR == 4) ? 8 : VAR == 5 ? 32 : 16;
   static constexpr int TILE = BLOCK * ITEMS;
 };
-constexpr int kRbkVariants = 5;
+constexpr int kRbkVariants = 6;
 constexpr int kRbkMinTile = 4096;  // smallest tile of any variant (workspace sizing)
 
 // Look-back window of 64*Q tiles (Q tiles per lane).  Q = 1 measured
@@ -94,14 +94,17 @@ constexpr int kRbkLookbackQ = BULKX_RBK_LOOKBACK_Q;
 constexpr size_t kRbkHeaderBytes = 256;
 constexpr int kRbkMaxStatusWords = 6;  // Seg<dd> = 24 bytes
 
-// Per-wave LDS region: the padded transposes of keys and values, reused as
-// the stage of the wave's outputs.
+// Per-wave LDS region, used in turn for the padded transpose of the keys,
+// that of the values, the stage of the wave's output keys and that of its
+// output values: one region instead of two halves the LDS of a workgroup,
+// so twice as many workgroups (and their loads) fit on a CU.
 template <class K, class V, int ITEMS>
 struct RbkLds {
-  static constexpr int KBYTES = (WaveTile<K, ITEMS>::LDS_ELEMS * int(sizeof(K)) + 15) / 16 * 16;
+  static constexpr int KBYTES = WaveTile<K, ITEMS>::LDS_ELEMS * int(sizeof(K));
   static constexpr int VBYTES = WaveTile<V, ITEMS>::LDS_ELEMS * int(sizeof(V));
-  static constexpr int BYTES = KBYTES + VBYTES;
-  static_assert(kWave * ITEMS * int(sizeof(K)) <= KBYTES, "key stage fits");
+  static constexpr int BYTES = ((KBYTES > VBYTES ? KBYTES : VBYTES) + 15) / 16 * 16;
+  static_assert(kWave * ITEMS * int(sizeof(K)) <= BYTES, "key stage fits");
+  static_assert(kWave * ITEMS * int(sizeof(V)) <= BYTES, "value stage fits");
 };
 
 // Layout: a wavefront owns a contiguous chunk of 64*ITEMS elements, loaded
@@ -136,7 +139,6 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
   using RSOp = RowSegOp<Op, V>;
   using WK = WaveTile<K, ITEMS>;
   using WV = WaveTile<V, ITEMS>;
-  using L = RbkLds<K, V, ITEMS>;
   constexpr int TILE = BLOCK * ITEMS;
   constexpr int NW = BLOCK / kWave;
   constexpr int N = WK::N;
@@ -148,60 +150,69 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
   const int count = rem <= 0 ? 0 : (rem >= N ? N : int(rem));
   const int64_t cb = count > 0 ? chunk_base : 0;
   K* lk = reinterpret_cast<K*>(sm.region[wid]);
-  V* lv = reinterpret_cast<V*>(sm.region[wid] + L::KBYTES);
+  V* lv = reinterpret_cast<V*>(sm.region[wid]);
 
-  // Coalesced loads, transposed to ITEMS consecutive elements per lane.
-  K k[ITEMS];
-  V v[ITEMS];
-  {
+  // Coalesced 16-byte loads of keys and values, all in flight before the
+  // first LDS write, then transposed in turn through the region to ITEMS
+  // consecutive elements per lane.
+  const bool fullk = count >= N && aligned16(keys + cb);
+  const bool fullv = count >= N && aligned16(vals + cb);
+  u32x4 tk[WK::NV], tv[WV::NV];
+  if (fullk) {
     const u32x4* q = reinterpret_cast<const u32x4*>(keys + cb);
-    if (count >= N && aligned16(keys + cb)) {
 #pragma unroll
-      for (int j = 0; j < WK::NV; ++j) {
-        const u32x4 t = __builtin_nontemporal_load(q + j * kWave + lane);
-        const K* e = reinterpret_cast<const K*>(&t);
-        const int base = WK::pos((j * kWave + lane) * WK::VE);
-#pragma unroll
-        for (int c = 0; c < WK::VE; ++c) lk[base + c] = e[c];
-      }
-    } else {
-#pragma unroll
-      for (int j = 0; j < ITEMS; ++j) {
-        const int e = j * kWave + lane;
-        lk[WK::pos(e)] = e < count ? keys[cb + e] : K(0);
-      }
-    }
+    for (int j = 0; j < WK::NV; ++j) tk[j] = __builtin_nontemporal_load(q + j * kWave + lane);
   }
-  {
+  if (fullv) {
     const u32x4* q = reinterpret_cast<const u32x4*>(vals + cb);
-    if (count >= N && aligned16(vals + cb)) {
 #pragma unroll
-      for (int j = 0; j < WV::NV; ++j) {
-        const u32x4 t = __builtin_nontemporal_load(q + j * kWave + lane);
-        const V* e = reinterpret_cast<const V*>(&t);
-        const int base = WV::pos((j * kWave + lane) * WV::VE);
-#pragma unroll
-        for (int c = 0; c < WV::VE; ++c) lv[base + c] = e[c];
-      }
-    } else {
-#pragma unroll
-      for (int j = 0; j < ITEMS; ++j) {
-        const int e = j * kWave + lane;
-        lv[WV::pos(e)] = e < count ? vals[cb + e] : V(0);
-      }
-    }
+    for (int j = 0; j < WV::NV; ++j) tv[j] = __builtin_nontemporal_load(q + j * kWave + lane);
   }
   K gprev = K(0), gnext = K(0);
   if (lane == 0 && count > 0 && chunk_base > 0) gprev = keys[chunk_base - 1];
   if (lane == kWave - 1 && count >= N && chunk_base + N < n) gnext = keys[chunk_base + N];
+  K k[ITEMS];
+  V v[ITEMS];
+  if (fullk) {
+#pragma unroll
+    for (int j = 0; j < WK::NV; ++j) {
+      const K* e = reinterpret_cast<const K*>(&tk[j]);
+      const int base = WK::pos((j * kWave + lane) * WK::VE);
+#pragma unroll
+      for (int c = 0; c < WK::VE; ++c) lk[base + c] = e[c];
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const int e = j * kWave + lane;
+      lk[WK::pos(e)] = e < count ? keys[cb + e] : K(0);
+    }
+  }
   wave_lds_sync();
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) k[i] = lk[WK::pos(lane * ITEMS + i)];
-#pragma unroll
-  for (int i = 0; i < ITEMS; ++i) v[i] = lv[WV::pos(lane * ITEMS + i)];
   const K kprev = lane > 0 ? lk[WK::pos(lane * ITEMS - 1)] : gprev;
   const K knext = lane < kWave - 1 ? lk[WK::pos(lane * ITEMS + ITEMS)] : gnext;
-  wave_lds_sync();  // the region is free again (used as the output stage below)
+  wave_lds_sync();  // key reads done before the values reuse the region
+  if (fullv) {
+#pragma unroll
+    for (int j = 0; j < WV::NV; ++j) {
+      const V* e = reinterpret_cast<const V*>(&tv[j]);
+      const int base = WV::pos((j * kWave + lane) * WV::VE);
+#pragma unroll
+      for (int c = 0; c < WV::VE; ++c) lv[base + c] = e[c];
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      const int e = j * kWave + lane;
+      lv[WV::pos(e)] = e < count ? vals[cb + e] : V(0);
+    }
+  }
+  wave_lds_sync();
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) v[i] = lv[WV::pos(lane * ITEMS + i)];
+  wave_lds_sync();  // the region is free again (used as the output stages below)
 
   // Head / tail flags (head_flags.hpp, tail_flags.hpp): one key comparison
   // per element; a tail is the element before a head.
@@ -265,17 +276,19 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
   // Walk: a tail's run has slot (heads so far) - 1.  The chunk's tails hold
   // consecutive slots from base (the run open at the chunk's first element),
   // so slot - base < 64*ITEMS is a 32-bit stage index.  Slots are kept
-  // relative to the tile (base_rel) until the tile's prefix arrives.
+  // relative to the tile (base_rel) until the tile's prefix arrives.  The
+  // walk stages the output keys and leaves each tail's output value in v[i].
   const S q = SOp::apply(wave_prefix, lift(lane_prefix));  // in-tile prefix of the lane
   const int64_t base_rel = readlane(int64_t(q.c) + int64_t(h & 1u) - 1, 0);
   const uint32_t ntails = wave_reduce<OpPlus>(uint32_t(__builtin_popcount(t)));
   K* sk = lk;
   V* sv = lv;
-  uint32_t e = uint32_t(int64_t(q.c) - base_rel);
+  const uint32_t e0 = uint32_t(int64_t(q.c) - base_rel);
+  uint32_t e = e0;
   V local = Op::template identity<V>();
   bool started = false;  // a run began inside this lane's slice
   bool fix = false;      // this lane holds the tail of the run carried into the tile
-  uint32_t fix_slot = 0;
+  int fix_i = 0;
   A fix_local = AOp::template identity<A>();
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
@@ -285,17 +298,15 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
     started |= hi;
     if ((t >> i) & 1u) {
       const uint32_t slot = e - 1;
-      if (slot < uint32_t(N)) {
-        sk[slot] = k[i];
-        if (started) {
-          sv[slot] = local;
-        } else if (q.c > 0) {
-          sv[slot] = AccP::to(AOp::apply(q.v, AccP::from(local)));
-        } else {
-          fix = true;
-          fix_slot = slot;
-          fix_local = AccP::from(local);
-        }
+      if (slot < uint32_t(N)) sk[slot] = k[i];
+      if (started) {
+        v[i] = local;
+      } else if (q.c > 0) {
+        v[i] = AccP::to(AOp::apply(q.v, AccP::from(local)));
+      } else {
+        fix = true;
+        fix_i = i;
+        fix_local = AccP::from(local);
       }
     }
   }
@@ -305,17 +316,30 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
   if (threadIdx.x == 0 && tile * TILE + TILE >= n)
     *num_runs = int64_t(tile_prefix.c + tile_agg.c);
   // same association as folding the full prefix first: (tile_prefix . q) . local
-  if (fix) sv[fix_slot] = AccP::to(AOp::apply(AOp::apply(tile_prefix.v, q.v), fix_local));
-  wave_lds_sync();
+  if (fix) {
+    const V fixed = AccP::to(AOp::apply(AOp::apply(tile_prefix.v, q.v), fix_local));
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i)
+      if (i == fix_i) v[i] = fixed;
+  }
   const int64_t base = int64_t(tile_prefix.c) + base_rel;
   // bounds guard: slots stay below n when the scan is consistent
   uint32_t cnt = ntails < uint32_t(N) ? ntails : uint32_t(N);
   if (base < 0 || base >= n) cnt = 0;
   else if (base + int64_t(cnt) > n) cnt = uint32_t(n - base);
-  for (uint32_t x = lane; x < cnt; x += kWave) {
-    kout[base + x] = sk[x];
-    vout[base + x] = sv[x];
+  for (uint32_t x = lane; x < cnt; x += kWave) kout[base + x] = sk[x];
+  wave_lds_sync();  // key stage read before the value stage overwrites it
+  e = e0;
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    e += (h >> i) & 1u;
+    if ((t >> i) & 1u) {
+      const uint32_t slot = e - 1;
+      if (slot < uint32_t(N)) sv[slot] = v[i];
+    }
   }
+  wave_lds_sync();
+  for (uint32_t x = lane; x < cnt; x += kWave) vout[base + x] = sv[x];
 }
 
 
@@ -404,6 +428,7 @@ int rbk_dispatch_values(int vdtype, int op, const void* keys, const void* vals, 
           else
             st = rbk_launch<K, V, OP, 0>(k, v, n, ko, vo, num_runs, ws, stream);
           break;
+        case 5: st = rbk_launch<K, V, OP, 5>(k, v, n, ko, vo, num_runs, ws, stream); break;
         default: st = rbk_launch<K, V, OP, 0>(k, v, n, ko, vo, num_runs, ws, stream); break;
       }
     } else {

@@ -71,6 +71,9 @@ size_t scan_workspace_bytes(int dtype, int64_t n) {
   if (es == 0 || n <= 0) return 0;
   // sized for the variant with the smallest tile, so any variant fits
   int64_t tiles = es == 4 ? scan_num_tiles<float, 1>(n) : scan_num_tiles<double, 1>(n);
+  static_assert(ScanConfig<float, 1>::TILE <= ScanConfig<float, 3>::TILE &&
+                    ScanConfig<double, 1>::TILE <= ScanConfig<double, 3>::TILE,
+                "variant 1 has the smallest tile");
   if (tiles <= 1) return 0;
   return kHeaderBytes + size_t(tiles) * status_words_per_tile(dtype) * sizeof(uint64_t);
 }
@@ -117,13 +120,18 @@ static int scan_entry(int dtype, int op, const void* in, void* out, int64_t n, c
   const int v = tuned_variant(EXCLUSIVE ? BULKX_ALGO_EXCLUSIVE_SCAN : BULKX_ALGO_INCLUSIVE_SCAN,
                               dtype, n);
   BULKX_DISPATCH_DTYPE(dtype, T, BULKX_DISPATCH_OP(op, OP, {
-    // the second shape is instantiated for plus only (min / max use the default)
+    // the other shapes are instantiated for plus only (min / max use the default)
     bool done = false;
     if constexpr (std::is_same<OP, OpPlus>::value) {
-      if (v == 1) {
-        st = scan_launch<T, OP, EXCLUSIVE, 1>(static_cast<const T*>(in), static_cast<T*>(out), n,
-                                              static_cast<const T*>(init), scratch.ptr(), stream);
-        done = true;
+      const T* i = static_cast<const T*>(in);
+      T* o = static_cast<T*>(out);
+      const T* ini = static_cast<const T*>(init);
+      done = true;
+      switch (v) {
+        case 1: st = scan_launch<T, OP, EXCLUSIVE, 1>(i, o, n, ini, scratch.ptr(), stream); break;
+        case 2: st = scan_launch<T, OP, EXCLUSIVE, 2>(i, o, n, ini, scratch.ptr(), stream); break;
+        case 3: st = scan_launch<T, OP, EXCLUSIVE, 3>(i, o, n, ini, scratch.ptr(), stream); break;
+        default: done = false; break;
       }
     }
     if (!done)

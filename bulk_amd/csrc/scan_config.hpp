@@ -6,13 +6,16 @@ namespace bulkx {
 
 // Variant 0 (the default): 1024-thread workgroups amortise one look-back
 // over a 128 KB tile.  Variant 1: a quarter tile (more tiles in flight for
-// mid-size inputs).  The tuning table picks one per dtype and size class.
+// mid-size inputs).  Variants 2 and 3: the same 128 bytes per thread in 512-
+// and 256-thread workgroups, so two or four workgroups share a CU and one
+// streams its tile while another scans or looks back.  The tuning table
+// picks one per dtype and size class.
 template <class T, int V = 0>
 struct ScanConfig {
-  static constexpr int BLOCK = V == 0 ? 1024 : 512;
-  static constexpr int ITEMS = (sizeof(T) == 4 ? 32 : 16) / (V == 0 ? 1 : 2);
+  static constexpr int BLOCK = V == 0 ? 1024 : V == 3 ? 256 : 512;
+  static constexpr int ITEMS = (sizeof(T) == 4 ? 32 : 16) / (V == 1 ? 2 : 1);
   static constexpr int TILE = BLOCK * ITEMS;
 };
-constexpr int kScanVariants = 2;
+constexpr int kScanVariants = 4;
 
 }  // namespace bulkx

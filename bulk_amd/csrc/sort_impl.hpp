@@ -6,10 +6,11 @@
 // BLOCK*ITEMS keys, every thread sorts its grain by odd-even transposition
 // (sort.hpp:33-75, stable with a strict comparator), then log2(BLOCK)
 // rounds of LDS merge-path merges double the sorted runs.  Only (key, tile
-// index) pairs move between rounds; values are gathered once at the end.
-// Keys and indices live in separate LDS arrays with a barrier between the
-// reads of one round and the writes of the next, so the reference's
-// shared key/value stage hazard (stable_merge_sort.hpp:97-115) cannot occur.
+// index) pairs move between rounds, packed into one LDS element so that the
+// merge reads the index with the key; values are gathered once at the end.
+// Values have their own LDS array and a barrier separates the reads of one
+// round from the writes of the next, so the reference's shared key/value
+// stage hazard (stable_merge_sort.hpp:97-115) cannot occur.
 //
 // Phase 2 (merge passes, as merge_by_key_kernel merge_sort_by_key.cu:61-87
 // and locate_merge_path :90-131): runs of width W = TILE, 2*TILE, ... are
@@ -26,12 +27,67 @@ namespace bulkx {
 template <class K, int VAR = 0>
 using SortConfig = MergeConfig<K, VAR>;
 
+// A key and its index in the tile: 8 bytes for 4-byte keys (one ds_read_b64
+// per merge step), 16 for 8-byte keys.
+template <class K>
+struct alignas(2 * sizeof(K)) KeyIdx {
+  K k;
+  uint32_t i;
+};
+
 template <class K, class V, bool HAS_V, int BLOCK, int ITEMS>
 struct SortSmem {
-  K k[MergeLds<K, BLOCK, ITEMS>::ELEMS];
-  uint32_t i[lpad_elems<uint32_t>(BLOCK * ITEMS)];
+  using P = KeyIdx<K>;
+  static constexpr int PBYTES = lpad_elems<P>(BLOCK * ITEMS) * int(sizeof(P));
+  static constexpr int KBYTES = MergeLds<K, BLOCK, ITEMS>::ELEMS * int(sizeof(K));
+  // the (key, index) rounds, or the per-wave key transposes of the load and store
+  __align__(16) char kp[PBYTES > KBYTES ? PBYTES : KBYTES];
   V v[HAS_V ? MergeLds<V, BLOCK, ITEMS>::ELEMS : 1];
 };
+
+// merge_path over padded (key, index) runs in LDS: A = s[base, base + n1),
+// B = s[base + b_off, base + b_off + n2) (logical indices); reads keys only.
+template <class CMP, class K>
+__device__ __forceinline__ int pair_merge_path(const KeyIdx<K>* s, int base, int n1, int b_off,
+                                               int n2, int diag) {
+  using P = KeyIdx<K>;
+  int lo = diag > n2 ? diag - n2 : 0;
+  int hi = diag < n1 ? diag : n1;
+  while (lo < hi) {
+    const int m = (lo + hi) >> 1;
+    if (CMP::apply(s[lpad<P>(base + b_off + diag - 1 - m)].k, s[lpad<P>(base + m)].k)) hi = m;
+    else lo = m + 1;
+  }
+  return lo;
+}
+
+// lds_serial_merge over (key, index) runs: the index travels with its key,
+// so no gather by LDS position follows the merge.
+template <class CMP, class K, int ITEMS, int LAST>
+__device__ __forceinline__ void pair_serial_merge(const KeyIdx<K>* s, int i1, int e1, int i2,
+                                                  int e2, K (&rk)[ITEMS], int (&ridx)[ITEMS]) {
+  using P = KeyIdx<K>;
+  P pa = s[lpad<P>(i1 < LAST ? i1 : LAST)];
+  P pb = s[lpad<P>(i2 < LAST ? i2 : LAST)];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const bool has1 = i1 < e1, has2 = i2 < e2;
+    const bool p = !has2 || (has1 && !CMP::apply(pb.k, pa.k));
+    rk[i] = p ? pa.k : pb.k;
+    ridx[i] = int(p ? pa.i : pb.i);
+    if (i + 1 < ITEMS) {
+      const int nx = (p ? i1 : i2) + 1;
+      const P pn = s[lpad<P>(nx < LAST ? nx : LAST)];
+      if (p) {
+        i1 = nx;
+        pa = pn;
+      } else {
+        i2 = nx;
+        pb = pn;
+      }
+    }
+  }
+}
 
 // Block sort of tile `tile` of keys/vals[0, n) into kout/vout (may alias).
 template <class CMP, class K, class V, bool HAS_V, int BLOCK, int ITEMS>
@@ -41,8 +97,9 @@ __device__ __forceinline__ void block_sort_tile(SortSmem<K, V, HAS_V, BLOCK, ITE
   constexpr int TILE = BLOCK * ITEMS;
   using WK = WaveTile<K, ITEMS>;
   using WV = WaveTile<V, ITEMS>;
-  K* s_k = sm.k;
-  uint32_t* s_i = sm.i;
+  using P = KeyIdx<K>;
+  K* s_k = reinterpret_cast<K*>(sm.kp);  // transpose regions of the load and the store
+  P* s_p = reinterpret_cast<P*>(sm.kp);  // the merge rounds
   V* s_v = sm.v;
 
   const int64_t base = tile * TILE;
@@ -56,7 +113,7 @@ __device__ __forceinline__ void block_sort_tile(SortSmem<K, V, HAS_V, BLOCK, ITE
 
   K k[ITEMS];
   int idx[ITEMS];
-  // Coalesced load through the wave's transpose region (inside s_k, which
+  // Coalesced load through the wave's transpose region (inside kp, which
   // the merge rounds only reuse after their first barrier).
   wave_load_blocked<K, ITEMS>(keys + base + (wcount > 0 ? wb : 0), wcount, k, K(),
                               s_k + wid * WK::LDS_ELEMS);
@@ -97,10 +154,7 @@ __device__ __forceinline__ void block_sort_tile(SortSmem<K, V, HAS_V, BLOCK, ITE
     __syncthreads();  // previous round's LDS reads are done
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
-      if (i < vt) {
-        s_k[lpad<K>(mine + i)] = k[i];
-        s_i[lpad<uint32_t>(mine + i)] = uint32_t(idx[i]);
-      }
+      if (i < vt) s_p[lpad<P>(mine + i)] = P{k[i], uint32_t(idx[i])};
     }
     __syncthreads();
     if (vt > 0) {
@@ -110,16 +164,17 @@ __device__ __forceinline__ void block_sort_tile(SortSmem<K, V, HAS_V, BLOCK, ITE
       int nb = cnt - ps - w;
       nb = nb < 0 ? 0 : (nb > w ? w : nb);
       const int diag = mine - ps;
-      const int ai = lds_merge_path<CMP>(s_k, ps, na, w, nb, diag);
+      const int ai = pair_merge_path<CMP>(s_p, ps, na, w, nb, diag);
       const int bi = diag - ai;
       K rk[ITEMS];
       int ri[ITEMS];
-      lds_serial_merge<CMP, K, ITEMS>(s_k, ps + ai, ps + na, ps + w + bi, ps + w + nb, vt, rk, ri);
+      pair_serial_merge<CMP, K, ITEMS, TILE - 1>(s_p, ps + ai, ps + na, ps + w + bi, ps + w + nb,
+                                                 rk, ri);
 #pragma unroll
       for (int i = 0; i < ITEMS; ++i) {
         if (i < vt) {
           k[i] = rk[i];
-          idx[i] = int(s_i[lpad<uint32_t>(ri[i])]);
+          idx[i] = ri[i];
         }
       }
     }
@@ -167,13 +222,12 @@ __device__ __forceinline__ PassTile pass_tile(int64_t n, int64_t width, int64_t 
 }
 
 // Merge path of a tile's first diagonal in a merge pass (the reference's
-// locate_merge_path, merge_sort_by_key.cu:90-131), searched by the calling
-// wavefront; every lane returns the result.
+// locate_merge_path, merge_sort_by_key.cu:90-131), searched by one lane.
 template <class CMP, class K, int TILE>
 __device__ __forceinline__ int64_t pass_split(const K* src, int64_t n, int64_t width,
                                               int64_t tile) {
   const PassTile g = pass_tile<TILE>(n, width, tile);
-  return wave_merge_path<CMP>(src + g.ps, g.na, src + g.ps + width, g.nb, g.d0);
+  return lane_merge_path<CMP>(src + g.ps, g.na, src + g.ps + width, g.nb, g.d0);
 }
 
 // One tile of a merge pass: pairs of sorted runs of width `width` (a
@@ -197,16 +251,15 @@ __device__ __forceinline__ void merge_pass_tile(MergeSmem<K, V, HAS_V, BLOCK, IT
       HAS_V ? dst_v + g.ps + g.d0 : nullptr, sm.k, sm.v);
 }
 
-// One wavefront per tile: splits[t] for every tile of a merge pass.
+// One lane per tile: splits[t] for every tile of a merge pass.
 template <class CMP, class K, int TILE>
 __global__ __launch_bounds__(256) void merge_partition_kernel(const K* __restrict__ src,
                                                               int64_t n, int64_t width,
                                                               int64_t tiles,
                                                               int64_t* __restrict__ splits) {
-  const int64_t t = int64_t(blockIdx.x) * (256 / kWave) + threadIdx.x / kWave;
-  if (t >= tiles) return;  // uniform per wavefront
-  const int64_t r = pass_split<CMP, K, TILE>(src, n, width, t);
-  if (lane_id() == 0) splits[t] = r;
+  const int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (t >= tiles) return;
+  splits[t] = pass_split<CMP, K, TILE>(src, n, width, t);
 }
 
 template <class CMP, class K, class V, bool HAS_V, int BLOCK, int ITEMS>
@@ -255,7 +308,7 @@ int sort_launch(K* keys, V* vals, int64_t n, char* ws, hipStream_t stream) {
       keys, vals, n, ck, cv);
   BULKX_LAUNCHED();
   int64_t width = C::TILE;
-  const uint32_t pgrid = uint32_t((tiles + 3) / 4);
+  const uint32_t pgrid = uint32_t((tiles + 255) / 256);
   for (int p = 0; p < passes; ++p, width *= 2) {
     merge_partition_kernel<CMP, K, C::TILE><<<dim3(pgrid), 256, 0, stream>>>(ck, n, width, tiles,
                                                                              splits);

@@ -150,12 +150,17 @@ int variant_shape(int algo, int dtype, int64_t n, int v, LaunchShape* out) {
   switch (algo) {
     case BULKX_ALGO_INCLUSIVE_SCAN:
     case BULKX_ALGO_EXCLUSIVE_SCAN: {
-      if (v == 0) {
-        s.group_size = es == 4 ? ScanConfig<float, 0>::BLOCK : ScanConfig<double, 0>::BLOCK;
-        s.grain_size = es == 4 ? ScanConfig<float, 0>::ITEMS : ScanConfig<double, 0>::ITEMS;
-      } else {
-        s.group_size = es == 4 ? ScanConfig<float, 1>::BLOCK : ScanConfig<double, 1>::BLOCK;
-        s.grain_size = es == 4 ? ScanConfig<float, 1>::ITEMS : ScanConfig<double, 1>::ITEMS;
+      switch (v) {
+#define BULKX_SCAN_SHAPE(V)                                                                 \
+  case V:                                                                                   \
+    s.group_size = es == 4 ? ScanConfig<float, V>::BLOCK : ScanConfig<double, V>::BLOCK;    \
+    s.grain_size = es == 4 ? ScanConfig<float, V>::ITEMS : ScanConfig<double, V>::ITEMS;    \
+    break;
+        BULKX_SCAN_SHAPE(0)
+        BULKX_SCAN_SHAPE(1)
+        BULKX_SCAN_SHAPE(2)
+        BULKX_SCAN_SHAPE(3)
+#undef BULKX_SCAN_SHAPE
       }
       const int64_t tile = int64_t(s.group_size) * s.grain_size;
       s.num_groups = (n + tile - 1) / tile;
@@ -183,9 +188,12 @@ int variant_shape(int algo, int dtype, int64_t n, int v, LaunchShape* out) {
       } else if (v == 3) {
         s.group_size = RbkConfig<int, int, 3>::BLOCK;
         s.grain_size = RbkConfig<int, int, 3>::ITEMS;
-      } else {
+      } else if (v == 4) {
         s.group_size = RbkConfig<int, int, 4>::BLOCK;
         s.grain_size = RbkConfig<int, int, 4>::ITEMS;
+      } else {
+        s.group_size = RbkConfig<int, int, 5>::BLOCK;
+        s.grain_size = RbkConfig<int, int, 5>::ITEMS;
       }
       const int64_t tile = int64_t(s.group_size) * s.grain_size;
       s.num_groups = (n + tile - 1) / tile;

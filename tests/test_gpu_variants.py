@@ -36,10 +36,10 @@ def force(bx):
 SIZES = [1, 4095, 4096, 4097, 8193, 16385, 100003, 1 << 20]
 
 
-@pytest.mark.parametrize("v", [0, 1])
+@pytest.mark.parametrize("v", [0, 1, 2, 3])
 def test_scan_variants(force, orc, v):
     from bulk_amd import api
-    assert _variants(C.ALGO_INCLUSIVE_SCAN) == 2
+    assert _variants(C.ALGO_INCLUSIVE_SCAN) == 4
     force(C.ALGO_INCLUSIVE_SCAN, C.INT32, v)
     force(C.ALGO_EXCLUSIVE_SCAN, C.INT32, v)
     force(C.ALGO_INCLUSIVE_SCAN, C.FLOAT64, v)
@@ -98,7 +98,7 @@ def test_merge_and_sort_variants(force, orc, v):
         assert np.array_equal(d8.cpu().numpy(), np.sort(k8, kind="stable")), n
 
 
-@pytest.mark.parametrize("v", [0, 1, 2])
+@pytest.mark.parametrize("v", [0, 1, 2, 3, 4, 5])
 def test_reduce_by_key_variants(force, orc, v):
     from bulk_amd import api
     for code in (C.INT32, C.INT64, C.FLOAT32):
