@@ -44,7 +44,7 @@ def _parse():
     p.add_argument("--no-per-call", action="store_true",
                    help="skip the per-call leg (and the batched == per-call check)")
     p.add_argument("--traffic-file", default=os.path.join(os.path.dirname(os.path.abspath(__file__)),
-                                                          "profiles", "batch_phase0_traffic.md"))
+                                                          "profiles", "mixed_stream_4096_batch_traffic.md"))
     return p.parse_args()
 
 
@@ -246,15 +246,21 @@ def main():
         else:
             stage_bytes["sort_block"] += j.algorithmic_bytes()
             stage_bytes["sort_rounds"] += _sort_passes(n) * 16 * n
+    stage_bytes["sort_pass_kernels"] = stage_bytes["sort_rounds"]
+    max_rounds = max([_sort_passes(j.spec.n) for j in jobs if j.spec.kind == C.JOB_SORT_BY_KEY] + [0])
     stages = {k: {"ms": acc[k], "bytes": stage_bytes[k],
                   "gbs": stage_bytes[k] / (acc[k] * 1e-3) / 1e9 if acc[k] > 0 else None}
               for k in Batch.STAGES}
     for k, v in stages.items():
         if v["bytes"]:
             _log(f"stage {k:18s} {v['ms']:7.3f} ms {v['gbs'] or 0:7.0f} GB/s")
-    # roofline: the phase-0 family kernel with the largest device time
-    dom = max(Batch.STAGES[:-1], key=lambda k: acc[k])
-    dom_idx = Batch.STAGES.index(dom)
+    # roofline: the kernel with the largest device time per step -- a
+    # phase-0 family kernel (one launch) or the merge-pass kernel (one launch
+    # per sort round); per-launch bytes and duration are the step's averages
+    kernel_of = {k: f"batch_phase0_kernel<{i}>" for i, k in enumerate(Batch.STAGES[:7])}
+    kernel_of["sort_pass_kernels"] = "batch_pass_kernel"
+    dom = max(kernel_of, key=lambda k: acc[k])
+    dom_launches = max(1, max_rounds) if dom == "sort_pass_kernels" else 1
     dom_gbs = stages[dom]["gbs"]
 
     # ---------------------------------------------------------------- serving
@@ -348,7 +354,7 @@ def main():
     value = total / (bt_mean * 1e-3) / 1e6
     pc_mean = float(np.mean(pc_ms)) if pc_ms else None
     pc_value = total / (pc_mean * 1e-3) / 1e6 if pc_ms else None
-    tr = _read_traffic(args.traffic_file, f"batch_phase0_kernel<{dom_idx}>")
+    tr = _read_traffic(args.traffic_file, kernel_of[dom])
     traffic = None
     if tr is not None and tr[1] == njobs and tr[2] == args.seed:
         traffic = tr[0]
@@ -378,11 +384,12 @@ def main():
         "p99_ms": serve["poisson"]["p99_ms"] if serve else None,
         "p50_ms": serve["poisson"]["p50_ms"] if serve else None,
         "copy_gbs": copy_gbs,
-        "roofline": {"bound": "hbm", "kernel": f"batch_phase0_kernel<{dom_idx}> ({dom})",
+        "roofline": {"bound": "hbm", "kernel": f"{kernel_of[dom]} ({dom})",
                      "achieved": dom_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": dom_gbs / HBM_PEAK_GBS, "frac_of_copy": dom_gbs / copy_gbs,
-                     "traffic": traffic, "algorithmic_bytes": stages[dom]["bytes"],
-                     "avg_launch_ms": stages[dom]["ms"], "launches": reps},
+                     "traffic": traffic, "algorithmic_bytes": stages[dom]["bytes"] / dom_launches,
+                     "avg_launch_ms": stages[dom]["ms"] / dom_launches,
+                     "launches_per_step": dom_launches},
         "stages": stages,
         "ops": ops,
         "cpu_baseline": cpu,

@@ -94,7 +94,8 @@ class Batch:
         return a.value, b.value
 
     STAGES = ("scan_i32", "scan_f32", "reduce_by_key_i32", "reduce_by_key_f32",
-              "reduce_by_key_i64", "merge_by_key", "sort_block", "sort_rounds")
+              "reduce_by_key_i64", "merge_by_key", "sort_block", "sort_rounds",
+              "sort_pass_kernels", "sort_partition_kernels")
 
     def stage_timing(self) -> dict[str, float]:
         """Per-stage ms of the last eager launch (HIP events on the batch's stream)."""

@@ -432,8 +432,12 @@ struct bulkx_batch {
   // most first, so they are a prefix of the sort tile table)
   int64_t active_sort_tiles[kBatchMaxPasses] = {};
   // stage timing of eager launches: ev[f]..ev[f+1] brackets family f,
-  // ev[FAM_COUNT]..ev[FAM_COUNT+1] the sort merge rounds
+  // ev[FAM_COUNT]..ev[FAM_COUNT+1] the sort merge rounds; in round r,
+  // rev[2r]..rev[2r+1] brackets the partition kernel, rev[2r+1]..rev[2r+2]
+  // the pass kernel
   hipEvent_t ev[kBatchStages + 1] = {};
+  hipEvent_t rev[2 * kBatchMaxPasses + 1] = {};
+  int timed_passes = 0;
   bool timed = false;
   // graph, and its kernel nodes whose grid follows the job list
   hipGraph_t graph = nullptr;
@@ -476,14 +480,18 @@ static int batch_enqueue_kernels(bulkx_batch* b, hipStream_t s, bool timed) {
   }
   if (timed) BULKX_HIP(hipEventRecord(b->ev[FAM_COUNT], s));
   for (int p = 0; p < b->max_passes; ++p) {
+    if (timed) BULKX_HIP(hipEventRecord(b->rev[2 * p], s));
     batch_partition_kernel<<<batch_grid(b, grid_kind_partition(p)), 256, 0, s>>>(
         b->d_desc, b->d_jobs, b->d_sort_tile_job, b->d_splits, p);
     BULKX_LAUNCHED();
+    if (timed) BULKX_HIP(hipEventRecord(b->rev[2 * p + 1], s));
     batch_pass_kernel<<<batch_grid(b, grid_kind_pass(p)), kBatchBlock, 0, s>>>(
         b->d_desc, b->d_jobs, b->d_sort_tile_job, b->d_splits, p);
     BULKX_LAUNCHED();
+    if (timed) BULKX_HIP(hipEventRecord(b->rev[2 * p + 2], s));
   }
   if (timed) BULKX_HIP(hipEventRecord(b->ev[FAM_COUNT + 1], s));
+  b->timed_passes = b->max_passes;
   return BULKX_SUCCESS;
 }
 
@@ -586,6 +594,7 @@ extern "C" int bulkx_batch_create(int max_jobs, int64_t max_elements, void* stre
   memset(b->h_desc, 0, sizeof(BatchDesc));
   hipError_t r = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking);
   for (int i = 0; i <= kBatchStages && r == hipSuccess; ++i) r = hipEventCreate(&b->ev[i]);
+  for (int i = 0; i <= 2 * kBatchMaxPasses && r == hipSuccess; ++i) r = hipEventCreate(&b->rev[i]);
   for (int i = 0; i < kBatchBranches && r == hipSuccess; ++i) {
     r = hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking);
     if (r == hipSuccess) r = hipEventCreateWithFlags(&b->join_ev[i], hipEventDisableTiming);
@@ -617,6 +626,8 @@ extern "C" int bulkx_batch_destroy(bulkx_batch* b) {
   }
   if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
   for (hipEvent_t e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : b->rev)
     if (e) (void)hipEventDestroy(e);
   for (void* p : {static_cast<void*>(b->d_desc), static_cast<void*>(b->d_jobs),
                   static_cast<void*>(b->d_tile_job), static_cast<void*>(b->d_sort_tile_job),
@@ -936,9 +947,21 @@ extern "C" int bulkx_batch_stage_timing(bulkx_batch* b, float* stage_ms, int num
   if (!b || !stage_ms || num_stages < 0) return BULKX_ERR_INVALID_ARGUMENT;
   if (!b->timed) return BULKX_ERR_NOT_READY;
   BULKX_HIP(hipEventSynchronize(b->ev[kBatchStages]));
+  // stages [0, kBatchStages): families and the merge rounds; then the
+  // pass kernels and the partition kernels of the rounds on their own
+  float pass_ms = 0.f, part_ms = 0.f;
+  for (int p = 0; p < b->timed_passes && num_stages > kBatchStages; ++p) {
+    float a = 0.f, c = 0.f;
+    BULKX_HIP(hipEventElapsedTime(&a, b->rev[2 * p], b->rev[2 * p + 1]));
+    BULKX_HIP(hipEventElapsedTime(&c, b->rev[2 * p + 1], b->rev[2 * p + 2]));
+    part_ms += a;
+    pass_ms += c;
+  }
   for (int k = 0; k < num_stages; ++k) {
     float ms = 0.f;
     if (k < kBatchStages) BULKX_HIP(hipEventElapsedTime(&ms, b->ev[k], b->ev[k + 1]));
+    else if (k == kBatchStages) ms = pass_ms;
+    else if (k == kBatchStages + 1) ms = part_ms;
     stage_ms[k] = ms;
   }
   return BULKX_SUCCESS;

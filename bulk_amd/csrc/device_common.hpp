@@ -299,6 +299,11 @@ constexpr uint32_t kStatusInvalid = 0, kStatusAggregate = 1, kStatusPrefix = 2;
 // flags the error word and treats the predecessor as an identity prefix, so
 // no kernel can hang forever on a protocol bug.
 constexpr uint32_t kSpinLimit = 1u << 22;
+// Sleep between polls of a window whose statuses are not all published
+// (units of 64 clocks).
+#ifndef BULKX_LOOKBACK_SLEEP
+#define BULKX_LOOKBACK_SLEEP 1
+#endif
 
 template <class T>
 struct TileStatus {
@@ -359,7 +364,7 @@ __device__ T lookback_exclusive_prefix(const TileStatus<T>& st, int64_t tile, ui
 #pragma unroll
       for (int j = 0; j < Q; ++j) missing |= kind[j] == kStatusInvalid;
       if (!__any(missing)) break;
-      __builtin_amdgcn_s_sleep(1);
+      __builtin_amdgcn_s_sleep(BULKX_LOOKBACK_SLEEP);
 #pragma unroll
       for (int j = 0; j < Q; ++j)
         if (kind[j] == kStatusInvalid) kind[j] = st.read(first + j, &val[j]);

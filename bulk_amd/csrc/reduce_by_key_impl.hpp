@@ -73,14 +73,19 @@ struct RowSegOp {
 };
 
 // Tile shapes: variant 0 = 512 x 16 (default), 1 = 512 x 8, 2 = 256 x 16,
-// 3 = 1024 x 16, 4 = 1024 x 8, 5 = 256 x 32.
+// 3 = 1024 x 16, 4 = 1024 x 8, 5 = 256 x 32, all single-pass with
+// decoupled look-back; variants 6, 7 and 8 are the three-pass form
+// (rbk_upsweep_kernel) of shapes 0, 2 and 5.
 template <class K, class V, int VAR = 0>
 struct RbkConfig {
   static constexpr int BLOCK = (VAR == 2 || VAR == 5) ? 256 : (VAR == 3 || VAR == 4) ? 1024 : 512;
   static constexpr int ITEMS = (VAR == 1 || VAR == 4) ? 8 : VAR == 5 ? 32 : 16;
   static constexpr int TILE = BLOCK * ITEMS;
 };
-constexpr int kRbkVariants = 6;
+constexpr int kRbkShapes = 6;
+constexpr int kRbkVariants = 9;
+// tile shape of a variant
+constexpr int rbk_shape(int variant) { return variant == 6 ? 0 : variant == 7 ? 2 : variant == 8 ? 5 : variant; }
 constexpr int kRbkMinTile = 4096;  // smallest tile of any variant (workspace sizing)
 
 // Look-back window of 64*Q tiles (Q tiles per lane).  Q = 1 measured
@@ -129,7 +134,8 @@ template <class K, class V, class Op, int BLOCK, int ITEMS>
 __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, const K* keys,
                                          const V* vals, int64_t n, K* kout, V* vout,
                                          int64_t* num_runs, int64_t tile, uint64_t* words,
-                                         uint32_t* err, bool single_tile) {
+                                         uint32_t* err, bool single_tile,
+                                         const Seg<typename Acc<V, Op>::type>* prefixes = nullptr) {
   using AccP = Acc<V, Op>;
   using A = typename AccP::type;
   using AOp = typename AccP::op;
@@ -262,8 +268,10 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
   // their chunks: the walk only needs the in-tile prefix, except for the
   // value of the tile's first run when that run began in an earlier tile,
   // which is patched once the tile's prefix is known.
-  const bool chained = !single_tile && tile > 0;
-  if (!single_tile && threadIdx.x == 0)
+  // With `prefixes` (the three-pass form) the tile's exclusive prefix is
+  // already known: nothing is published and nothing looked back.
+  const bool chained = !single_tile && tile > 0 && prefixes == nullptr;
+  if (!single_tile && prefixes == nullptr && threadIdx.x == 0)
     st.publish(tile, tile == 0 ? kStatusPrefix : kStatusAggregate, tile_agg);
   if (chained && threadIdx.x < kWave) {
     S excl = lookback_exclusive_prefix<SOp, kRbkLookbackQ>(st, tile, err);
@@ -311,7 +319,8 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
     }
   }
   __syncthreads();  // the tile's prefix is in sm.prefix
-  const S tile_prefix = chained ? sm.prefix : SOp::template identity<S>();
+  const S tile_prefix =
+      prefixes ? prefixes[tile] : chained ? sm.prefix : SOp::template identity<S>();
   // The tile holding element n-1 reports the run count.
   if (threadIdx.x == 0 && tile * TILE + TILE >= n)
     *num_runs = int64_t(tile_prefix.c + tile_agg.c);
@@ -361,6 +370,146 @@ __global__ __launch_bounds__(BLOCK) void reduce_by_key_kernel(
                                    int64_t(wave_uniform(s_tile)), words, header + 1, false);
 }
 
+// Three-pass form (variants kRbkShapes and up): the tiles never wait on
+// one another.  Pass 1 writes every tile's aggregate (its head count and
+// the fold of the values from its last head, or from its start when it has
+// none, to its end), reading the keys and only those values; pass 2 scans
+// the aggregates in one workgroup; pass 3 is rbk_tile with the tile's
+// exclusive prefix read from pass 2.  The aggregates are those the
+// scan_head_flags operator (reduce_by_key.hpp:41-62) folds, so pass 3
+// produces what the single-pass kernel does.  It reads the keys twice
+// (12 bytes per element for 4-byte keys and values instead of 8) but has
+// no look-back, whose waits on the slowest predecessor of a window bound
+// the single-pass kernel on large inputs.
+template <class K, class V, class Op, int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void rbk_upsweep_kernel(
+    const K* __restrict__ keys, const V* __restrict__ vals, int64_t n,
+    Seg<typename Acc<V, Op>::type>* __restrict__ aggs) {
+  using AccP = Acc<V, Op>;
+  using A = typename AccP::type;
+  using AOp = typename AccP::op;
+  constexpr int TILE = BLOCK * ITEMS;
+  constexpr int NW = BLOCK / kWave;
+  static_assert(ITEMS <= 32, "head flags are one 32-bit mask per thread");
+  __shared__ uint32_t s_cnt[NW];
+  __shared__ int s_last[NW];
+  __shared__ A s_val[NW];
+  const int wid = threadIdx.x / kWave;
+  const int64_t base = int64_t(blockIdx.x) * TILE;
+  const int cnt = int(n - base < TILE ? n - base : TILE);
+  const int mine = threadIdx.x * ITEMS;
+  int vt = cnt - mine;
+  vt = vt < 0 ? 0 : (vt > ITEMS ? ITEMS : vt);
+  const bool full = vt == ITEMS && aligned16(keys + base + mine);
+  K k[ITEMS];
+  load_blocked<K, ITEMS>(keys + base + mine, vt, k, K(), full);
+  const K kp = (vt > 0 && base + mine > 0) ? keys[base + mine - 1] : K();
+  uint32_t h = 0;
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) h |= uint32_t(!(k[i] == (i ? k[i - 1] : kp))) << i;
+  if (base + mine == 0) h |= 1u;  // the array's first element
+  h &= vt >= 32 ? ~0u : ((1u << vt) - 1);
+  const int last = h ? mine + 31 - __builtin_clz(h) : -1;
+  const uint32_t wc = wave_reduce<OpPlus>(uint32_t(__builtin_popcount(h)));
+  const int wl = wave_reduce<OpMax>(last);
+  if (lane_id() == 0) {
+    s_cnt[wid] = wc;
+    s_last[wid] = wl;
+  }
+  __syncthreads();
+  uint64_t tc = 0;
+  int from = -1;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    tc += s_cnt[w];
+    from = s_last[w] > from ? s_last[w] : from;
+  }
+  from = from < 0 ? 0 : from;  // no head: the whole tile continues the open run
+  // fold of values [from, cnt) in element order: per thread, then across lanes
+  // and waves in order
+  A part = AOp::template identity<A>();
+  if (mine + vt > from) {
+    if (mine >= from && vt == ITEMS && aligned16(vals + base + mine)) {
+      V v[ITEMS];
+      load_blocked<V, ITEMS>(vals + base + mine, ITEMS, v, V(), true);
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i) part = AOp::apply(part, AccP::from(v[i]));
+    } else {
+#pragma unroll
+      for (int i = 0; i < ITEMS; ++i)
+        if (i < vt && mine + i >= from) part = AOp::apply(part, AccP::from(vals[base + mine + i]));
+    }
+  }
+  const A wv = wave_reduce<AOp>(part);
+  if (lane_id() == 0) s_val[wid] = wv;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    A v = AOp::template identity<A>();
+#pragma unroll
+    for (int w = 0; w < NW; ++w) v = AOp::apply(v, s_val[w]);
+    Seg<A> g;
+    g.c = tc;
+    g.v = v;
+    aggs[blockIdx.x] = g;
+  }
+}
+
+// Pass 2: exclusive scan of the tile aggregates in place, by one
+// workgroup, ITEMS aggregates per thread per step.
+template <class V, class Op>
+__global__ __launch_bounds__(1024) void rbk_scan_aggs_kernel(
+    Seg<typename Acc<V, Op>::type>* __restrict__ aggs, int64_t tiles) {
+  using AccP = Acc<V, Op>;
+  using A = typename AccP::type;
+  using S = Seg<A>;
+  using SOp = SegOp<typename AccP::op, A>;
+  constexpr int ITEMS = 4;
+  constexpr int NW = 1024 / kWave;
+  __shared__ S s_wave[NW];
+  const int wid = threadIdx.x / kWave;
+  const S id = SOp::template identity<S>();
+  S carry = id;
+  for (int64_t c0 = 0; c0 < tiles; c0 += 1024 * ITEMS) {
+    const int64_t i0 = c0 + int64_t(threadIdx.x) * ITEMS;
+    S x[ITEMS];
+    S t = id;
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      x[j] = i0 + j < tiles ? aggs[i0 + j] : id;
+      t = SOp::apply(t, x[j]);
+    }
+    const S inc = wave_inclusive_scan<SOp>(t);
+    const S lane_excl = wave_shift_right<SOp>(inc);
+    if (lane_id() == kWave - 1) s_wave[wid] = inc;
+    __syncthreads();
+    S wave_excl = id, total = id;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      if (w < wid) wave_excl = SOp::apply(wave_excl, s_wave[w]);
+      total = SOp::apply(total, s_wave[w]);
+    }
+    S pre = SOp::apply(carry, SOp::apply(wave_excl, lane_excl));
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) {
+      if (i0 + j < tiles) aggs[i0 + j] = pre;
+      pre = SOp::apply(pre, x[j]);
+    }
+    carry = SOp::apply(carry, total);
+    __syncthreads();  // s_wave is rewritten by the next step
+  }
+}
+
+// Pass 3: one tile per workgroup, in any order.
+template <class K, class V, class Op, int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void rbk_downsweep_kernel(
+    const K* __restrict__ keys, const V* __restrict__ vals, int64_t n, K* __restrict__ kout,
+    V* __restrict__ vout, int64_t* __restrict__ num_runs,
+    const Seg<typename Acc<V, Op>::type>* __restrict__ prefixes) {
+  __shared__ RbkSmem<K, V, Op, BLOCK, ITEMS> sm;
+  rbk_tile<K, V, Op, BLOCK, ITEMS>(sm, keys, vals, n, kout, vout, num_runs, blockIdx.x, nullptr,
+                                   nullptr, false, prefixes);
+}
+
 __global__ void rbk_empty_kernel(int64_t* num_runs);
 __global__ void rbk_prepare_kernel(uint32_t* header, uint64_t* words, int64_t num_words);
 
@@ -370,7 +519,7 @@ inline int64_t rbk_num_tiles(int64_t n) {
   return (n + TILE - 1) / TILE;
 }
 
-template <class K, class V, class Op, int VAR>
+template <class K, class V, class Op, int VAR, bool THREE_PASS = false>
 int rbk_launch(const K* keys, const V* vals, int64_t n, K* kout, V* vout, int64_t* num_runs,
                char* ws, hipStream_t stream) {
   using C = RbkConfig<K, V, VAR>;
@@ -384,6 +533,20 @@ int rbk_launch(const K* keys, const V* vals, int64_t n, K* kout, V* vout, int64_
   if (tiles == 1) {
     reduce_by_key_kernel<K, V, Op, C::BLOCK, C::ITEMS><<<1, C::BLOCK, 0, stream>>>(
         keys, vals, n, kout, vout, num_runs, nullptr, nullptr, true);
+    BULKX_LAUNCHED();
+    return BULKX_SUCCESS;
+  }
+  if constexpr (THREE_PASS) {
+    // the aggregates take half the room of the status words
+    static_assert(sizeof(S) <= TileStatus<S>::W * sizeof(uint64_t), "aggregates fit");
+    S* aggs = reinterpret_cast<S*>(ws + kRbkHeaderBytes);
+    rbk_upsweep_kernel<K, V, Op, C::BLOCK, C::ITEMS><<<dim3(uint32_t(tiles)), C::BLOCK, 0, stream>>>(
+        keys, vals, n, aggs);
+    BULKX_LAUNCHED();
+    rbk_scan_aggs_kernel<V, Op><<<1, 1024, 0, stream>>>(aggs, tiles);
+    BULKX_LAUNCHED();
+    rbk_downsweep_kernel<K, V, Op, C::BLOCK, C::ITEMS>
+        <<<dim3(uint32_t(tiles)), C::BLOCK, 0, stream>>>(keys, vals, n, kout, vout, num_runs, aggs);
     BULKX_LAUNCHED();
     return BULKX_SUCCESS;
   }
@@ -429,6 +592,10 @@ int rbk_dispatch_values(int vdtype, int op, const void* keys, const void* vals, 
             st = rbk_launch<K, V, OP, 0>(k, v, n, ko, vo, num_runs, ws, stream);
           break;
         case 5: st = rbk_launch<K, V, OP, 5>(k, v, n, ko, vo, num_runs, ws, stream); break;
+        // three-pass forms of shapes 0, 2 and 5
+        case 6: st = rbk_launch<K, V, OP, 0, true>(k, v, n, ko, vo, num_runs, ws, stream); break;
+        case 7: st = rbk_launch<K, V, OP, 2, true>(k, v, n, ko, vo, num_runs, ws, stream); break;
+        case 8: st = rbk_launch<K, V, OP, 5, true>(k, v, n, ko, vo, num_runs, ws, stream); break;
         default: st = rbk_launch<K, V, OP, 0>(k, v, n, ko, vo, num_runs, ws, stream); break;
       }
     } else {

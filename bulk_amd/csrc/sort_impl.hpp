@@ -7,9 +7,9 @@
 // (sort.hpp:33-75, stable with a strict comparator), then log2(BLOCK)
 // rounds of LDS merge-path merges double the sorted runs.  Only (key, tile
 // index) pairs move between rounds, packed into one LDS element so that the
-// merge reads the index with the key; values are gathered once at the end.
-// Values have their own LDS array and a barrier separates the reads of one
-// round from the writes of the next, so the reference's shared key/value
+// merge reads the index with the key; values wait in registers and are
+// staged and gathered once at the end.  A barrier separates every read of
+// the stage from the next write to it, so the reference's shared key/value
 // stage hazard (stable_merge_sort.hpp:97-115) cannot occur.
 //
 // Phase 2 (merge passes, as merge_by_key_kernel merge_sort_by_key.cu:61-87
@@ -27,6 +27,23 @@ namespace bulkx {
 template <class K, int VAR = 0>
 using SortConfig = MergeConfig<K, VAR>;
 
+// Sort shapes: the merge passes use the merge tile of MergeConfig (variant
+// 0 or 1); the block sort writes runs of RUN0 elements.  Variants 2 and 3
+// sort 8192- and 16384-element runs (512 and 1024 threads x 16) in LDS, so
+// one or two fewer passes stream the whole array through HBM; they are
+// compiled for 4-byte keys and values (LDS and registers), other types use
+// variant 0.
+template <class K, class V, int VAR>
+struct SortShape {
+  static constexpr bool kBig = VAR >= 2 && sizeof(K) == 4 && sizeof(V) == 4;
+  using Pass = MergeConfig<K, VAR == 1 ? 1 : 0>;
+  static constexpr int BBLOCK = kBig ? (VAR == 2 ? 512 : 1024) : Pass::BLOCK;
+  static constexpr int BITEMS = kBig ? 16 : Pass::ITEMS;
+  static constexpr int RUN0 = BBLOCK * BITEMS;
+  static_assert(RUN0 % Pass::TILE == 0, "runs are whole pass tiles");
+};
+constexpr int kSortVariants = 4;
+
 // A key and its index in the tile: 8 bytes for 4-byte keys (one ds_read_b64
 // per merge step), 16 for 8-byte keys.
 template <class K>
@@ -35,14 +52,17 @@ struct alignas(2 * sizeof(K)) KeyIdx {
   uint32_t i;
 };
 
+// One LDS stage, used in turn for the key transposes of the load, the
+// (key, index) rounds, the value gather and the transposes of the store:
+// 35 KB for 4096 4-byte keys and values, so four workgroups share a CU.
 template <class K, class V, bool HAS_V, int BLOCK, int ITEMS>
 struct SortSmem {
   using P = KeyIdx<K>;
+  static constexpr int cmax(int a, int b) { return a > b ? a : b; }
   static constexpr int PBYTES = lpad_elems<P>(BLOCK * ITEMS) * int(sizeof(P));
   static constexpr int KBYTES = MergeLds<K, BLOCK, ITEMS>::ELEMS * int(sizeof(K));
-  // the (key, index) rounds, or the per-wave key transposes of the load and store
-  __align__(16) char kp[PBYTES > KBYTES ? PBYTES : KBYTES];
-  V v[HAS_V ? MergeLds<V, BLOCK, ITEMS>::ELEMS : 1];
+  static constexpr int VBYTES = HAS_V ? MergeLds<V, BLOCK, ITEMS>::ELEMS * int(sizeof(V)) : 0;
+  __align__(16) char kp[cmax(cmax(PBYTES, KBYTES), VBYTES)];
 };
 
 // merge_path over padded (key, index) runs in LDS: A = s[base, base + n1),
@@ -98,9 +118,9 @@ __device__ __forceinline__ void block_sort_tile(SortSmem<K, V, HAS_V, BLOCK, ITE
   using WK = WaveTile<K, ITEMS>;
   using WV = WaveTile<V, ITEMS>;
   using P = KeyIdx<K>;
-  K* s_k = reinterpret_cast<K*>(sm.kp);  // transpose regions of the load and the store
+  K* s_k = reinterpret_cast<K*>(sm.kp);  // key transposes of the load and the store
   P* s_p = reinterpret_cast<P*>(sm.kp);  // the merge rounds
-  V* s_v = sm.v;
+  V* s_v = reinterpret_cast<V*>(sm.kp);  // value gather and transposes
 
   const int64_t base = tile * TILE;
   const int cnt = int(n - base < TILE ? n - base : TILE);
@@ -113,23 +133,20 @@ __device__ __forceinline__ void block_sort_tile(SortSmem<K, V, HAS_V, BLOCK, ITE
 
   K k[ITEMS];
   int idx[ITEMS];
-  // Coalesced load through the wave's transpose region (inside kp, which
-  // the merge rounds only reuse after their first barrier).
-  wave_load_blocked<K, ITEMS>(keys + base + (wcount > 0 ? wb : 0), wcount, k, K(),
-                              s_k + wid * WK::LDS_ELEMS);
-  if constexpr (HAS_V) {  // striped staging: every load in flight before the LDS writes
-    V lv[ITEMS];
+  // Striped value loads first, so they are in flight with the keys; the
+  // values stay in registers until the gather.
+  V lv[ITEMS];
+  if constexpr (HAS_V) {
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) {
       const int j = threadIdx.x + i * BLOCK;
       if (j < cnt) lv[i] = vals[base + j];
     }
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i) {
-      const int j = threadIdx.x + i * BLOCK;
-      if (j < cnt) s_v[lpad<V>(j)] = lv[i];
-    }
   }
+  // Coalesced key load through the wave's transpose region (inside kp, which
+  // the merge rounds only reuse after their first barrier).
+  wave_load_blocked<K, ITEMS>(keys + base + (wcount > 0 ? wb : 0), wcount, k, K(),
+                              s_k + wid * WK::LDS_ELEMS);
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) idx[i] = mine + i;
 
@@ -182,13 +199,25 @@ __device__ __forceinline__ void block_sort_tile(SortSmem<K, V, HAS_V, BLOCK, ITE
 
   V rv[ITEMS];
   if constexpr (HAS_V) {
+    __syncthreads();  // the last round's reads of the stage are done
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const int j = threadIdx.x + i * BLOCK;
+      if (j < cnt) s_v[lpad<V>(j)] = lv[i];
+    }
+    __syncthreads();
 #pragma unroll
     for (int i = 0; i < ITEMS; ++i) rv[i] = i < vt ? s_v[lpad<V>(idx[i])] : V();
   }
-  __syncthreads();  // all LDS reads done; reuse s_k / s_v as transpose regions
-  if (wcount > 0) {
+  __syncthreads();  // all LDS reads done; reuse the stage as transpose regions
+  if (wcount > 0)
     wave_store_blocked<K, ITEMS>(kout + base + wb, wcount, k, s_k + wid * WK::LDS_ELEMS);
-    if constexpr (HAS_V)
+  if constexpr (HAS_V) {
+    // a wave's value region is its key region when K and V have one size;
+    // otherwise it overlaps other waves' key regions
+    if constexpr (sizeof(K) == sizeof(V)) wave_lds_sync();
+    else __syncthreads();
+    if (wcount > 0)
       wave_store_blocked<V, ITEMS>(vout + base + wb, wcount, rv, s_v + wid * WV::LDS_ELEMS);
   }
 }
@@ -292,9 +321,11 @@ struct SortWorkspace {
 
 template <class CMP, class K, class V, bool HAS_V, int VAR>
 int sort_launch(K* keys, V* vals, int64_t n, char* ws, hipStream_t stream) {
-  using C = SortConfig<K, VAR>;
-  const int64_t tiles = (n + C::TILE - 1) / C::TILE;
-  const int passes = sort_num_passes(tiles);
+  using S = SortShape<K, V, VAR>;
+  using C = typename S::Pass;
+  const int64_t tiles = (n + C::TILE - 1) / C::TILE;   // merge-pass tiles
+  const int64_t runs = (n + S::RUN0 - 1) / S::RUN0;    // block-sorted runs
+  const int passes = sort_num_passes(runs);
   const SortWorkspace w(sizeof(K), HAS_V ? sizeof(V) : 0, n, tiles);
   K* tk = reinterpret_cast<K*>(ws + w.keys);
   V* tv = HAS_V ? reinterpret_cast<V*>(ws + w.vals) : nullptr;
@@ -304,10 +335,10 @@ int sort_launch(K* keys, V* vals, int64_t n, char* ws, hipStream_t stream) {
   V* cv = (passes & 1) ? tv : vals;
   K* ok = (passes & 1) ? keys : tk;
   V* ov = (passes & 1) ? vals : tv;
-  block_sort_kernel<CMP, K, V, HAS_V, C::BLOCK, C::ITEMS><<<dim3(uint32_t(tiles)), C::BLOCK, 0, stream>>>(
-      keys, vals, n, ck, cv);
+  block_sort_kernel<CMP, K, V, HAS_V, S::BBLOCK, S::BITEMS>
+      <<<dim3(uint32_t(runs)), S::BBLOCK, 0, stream>>>(keys, vals, n, ck, cv);
   BULKX_LAUNCHED();
-  int64_t width = C::TILE;
+  int64_t width = S::RUN0;
   const uint32_t pgrid = uint32_t((tiles + 255) / 256);
   for (int p = 0; p < passes; ++p, width *= 2) {
     merge_partition_kernel<CMP, K, C::TILE><<<dim3(pgrid), 256, 0, stream>>>(ck, n, width, tiles,
@@ -346,8 +377,12 @@ int sort_dispatch_variant(int vdtype, int comp, void* keys, void* vals, int64_t 
 template <class K>
 int sort_dispatch_key(int vdtype, int comp, void* keys, void* vals, int64_t n, char* ws,
                       hipStream_t stream, int variant) {
-  if (variant == 1) return sort_dispatch_variant<K, 1>(vdtype, comp, keys, vals, n, ws, stream);
-  return sort_dispatch_variant<K, 0>(vdtype, comp, keys, vals, n, ws, stream);
+  switch (variant) {
+    case 1: return sort_dispatch_variant<K, 1>(vdtype, comp, keys, vals, n, ws, stream);
+    case 2: return sort_dispatch_variant<K, 2>(vdtype, comp, keys, vals, n, ws, stream);
+    case 3: return sort_dispatch_variant<K, 3>(vdtype, comp, keys, vals, n, ws, stream);
+    default: return sort_dispatch_variant<K, 0>(vdtype, comp, keys, vals, n, ws, stream);
+  }
 }
 
 int sort_dispatch_k4(int kdtype, int vdtype, int comp, void* keys, void* vals, int64_t n,

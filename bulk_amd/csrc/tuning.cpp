@@ -18,6 +18,7 @@
 #include "merge_impl.hpp"
 #include "reduce_by_key_impl.hpp"
 #include "scan_config.hpp"
+#include "sort_impl.hpp"
 #include "tuning.hpp"
 
 namespace bulkx {
@@ -124,8 +125,8 @@ int num_variants(int algo) {
     case BULKX_ALGO_EXCLUSIVE_SCAN: return kScanVariants;
     case BULKX_ALGO_REDUCE: return 3;
     case BULKX_ALGO_MERGE:
-    case BULKX_ALGO_MERGE_BY_KEY:
-    case BULKX_ALGO_SORT_BY_KEY: return kMergeVariants;
+    case BULKX_ALGO_MERGE_BY_KEY: return kMergeVariants;
+    case BULKX_ALGO_SORT_BY_KEY: return kSortVariants;
     case BULKX_ALGO_REDUCE_BY_KEY: return kRbkVariants;
     default: return 1;
   }
@@ -176,6 +177,7 @@ int variant_shape(int algo, int dtype, int64_t n, int v, LaunchShape* out) {
       break;
     }
     case BULKX_ALGO_REDUCE_BY_KEY: {
+      v = rbk_shape(v);
       if (v == 0) {
         s.group_size = RbkConfig<int, int, 0>::BLOCK;
         s.grain_size = RbkConfig<int, int, 0>::ITEMS;
@@ -205,9 +207,15 @@ int variant_shape(int algo, int dtype, int64_t n, int v, LaunchShape* out) {
       if (v == 0) {
         s.group_size = es == 4 ? MergeConfig<float, 0>::BLOCK : MergeConfig<double, 0>::BLOCK;
         s.grain_size = es == 4 ? MergeConfig<float, 0>::ITEMS : MergeConfig<double, 0>::ITEMS;
-      } else {
+      } else if (v == 1) {
         s.group_size = es == 4 ? MergeConfig<float, 1>::BLOCK : MergeConfig<double, 1>::BLOCK;
         s.grain_size = es == 4 ? MergeConfig<float, 1>::ITEMS : MergeConfig<double, 1>::ITEMS;
+      } else {  // sort variants 2 and 3: the block sort's shape (4-byte keys)
+        const bool k4 = es == 4;
+        s.group_size = k4 ? (v == 2 ? SortShape<float, float, 2>::BBLOCK
+                                    : SortShape<float, float, 3>::BBLOCK)
+                          : SortShape<double, double, 0>::BBLOCK;
+        s.grain_size = k4 ? SortShape<float, float, 2>::BITEMS : SortShape<double, double, 0>::BITEMS;
       }
       const int64_t tile = int64_t(s.group_size) * s.grain_size;
       s.num_groups = (n + tile - 1) / tile;

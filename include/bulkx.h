@@ -279,7 +279,9 @@ int bulkx_batch_last_timing(bulkx_batch* batch, float* phase0_ms, float* passes_
 /* Per-stage device time in ms of the last eager launch, in launch order:
  * [0] int32 scan/reduce, [1] float32 scan/reduce, [2..4] reduce_by_key with
  * int32 / float32 / int64 values, [5] merge_by_key, [6] sort block phase,
- * [7] sort merge rounds.  Writes num_stages entries (zeros beyond 8). */
+ * [7] sort merge rounds; then the rounds split by kernel: [8] the merge
+ * pass kernels, [9] the partition kernels.  Writes num_stages entries
+ * (zeros beyond 10). */
 int bulkx_batch_stage_timing(bulkx_batch* batch, float* stage_ms, int num_stages);
 
 /* ------------------------------------------------------------------------

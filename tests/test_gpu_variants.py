@@ -67,12 +67,13 @@ def test_reduce_variants(force, orc, v):
         assert abs(api.my_reduce(_dev(f), 0.0) - (hi + lo)) <= orc.float_tolerance(n, 2.0 ** -24, ab), n
 
 
-@pytest.mark.parametrize("v", [0, 1])
+@pytest.mark.parametrize("v", [0, 1, 2, 3])
 def test_merge_and_sort_variants(force, orc, v):
     from bulk_amd import api
+    assert _variants(C.ALGO_SORT_BY_KEY) == 4 and _variants(C.ALGO_MERGE) == 2
     for a in (C.ALGO_MERGE, C.ALGO_MERGE_BY_KEY):
-        force(a, C.FLOAT32, v)
-        force(a, C.INT32, v)
+        force(a, C.FLOAT32, min(v, 1))
+        force(a, C.INT32, min(v, 1))
     force(C.ALGO_SORT_BY_KEY, C.INT32, v)
     force(C.ALGO_SORT_BY_KEY, C.INT64, v)
     pool = orc.gen_uniform_f32(7, 64)
@@ -85,7 +86,7 @@ def test_merge_and_sort_variants(force, orc, v):
         rk, rv = orc.merge_by_key(a, va, b, vb)
         assert np.array_equal(ko.cpu().numpy(), rk) and np.array_equal(vo.cpu().numpy(), rv), (n1, n2)
         assert np.array_equal(api.my_merge(_dev(a), _dev(b)).cpu().numpy(), orc.merge(a, b)), (n1, n2)
-    for n in SIZES:
+    for n in SIZES + [16384 * 5 + 3, 3 * (1 << 18) + 1]:
         k = orc.gen_uniform_i32(n, 0, max(2, n // 8), n)
         vals = np.arange(n, dtype=np.int32)
         dk, dv = _dev(k), _dev(vals)
@@ -98,7 +99,7 @@ def test_merge_and_sort_variants(force, orc, v):
         assert np.array_equal(d8.cpu().numpy(), np.sort(k8, kind="stable")), n
 
 
-@pytest.mark.parametrize("v", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("v", list(range(9)))
 def test_reduce_by_key_variants(force, orc, v):
     from bulk_amd import api
     for code in (C.INT32, C.INT64, C.FLOAT32):
@@ -114,3 +115,21 @@ def test_reduce_by_key_variants(force, orc, v):
         ko, vo, m = api.my_reduce_by_key(_dev(keys), _dev(f))
         rk, _ = orc.reduce_by_key(keys, f.astype(np.float64), textbook=True)
         assert m == rk.size and np.array_equal(ko[:m].cpu().numpy(), rk), n
+    # runs longer than a tile: tiles with no head carry the open run
+    for n, run in ((300001, 50000), (1 << 20, 1 << 20)):
+        keys = (np.arange(n) // run).astype(np.int32)
+        vals = orc.gen_uniform_i64(n + 7, -(1 << 20), 1 << 20, n)
+        ko, vo, m = api.my_reduce_by_key(_dev(keys), _dev(vals))
+        rk, rv = orc.reduce_by_key(keys, vals, textbook=True)
+        assert m == rk.size and np.array_equal(ko[:m].cpu().numpy(), rk) and \
+            np.array_equal(vo[:m].cpu().numpy(), rv), n
+        f = orc.gen_uniform_f32(n + 9, n, -1.0, 1.0).astype(np.float64)
+        ko, vo, m = api.my_reduce_by_key(_dev(keys), _dev(f.astype(np.float32)))
+        starts = np.flatnonzero(np.r_[True, keys[1:] != keys[:-1]])
+        exact = np.add.reduceat(f, starts)
+        absum = np.add.reduceat(np.abs(f), starts)
+        lens = np.diff(np.r_[starts, n])
+        # the float tolerance 4 * ceil(log2 len) * 2^-24 * sum|x| per run
+        bound = 4 * np.maximum(1, np.ceil(np.log2(np.maximum(lens, 2)))) * 2.0 ** -24 * absum
+        assert m == starts.size
+        assert np.all(np.abs(vo[:m].cpu().numpy().astype(np.float64) - exact) <= bound), n
