@@ -130,12 +130,25 @@ struct RbkSmem {
 
 // One tile of reduce_by_key (shared by the device-wide kernel and the batch
 // executor).
+// Aggregates scanned per workgroup of the three-pass form's pass 2: 256
+// threads x 8 (x 4 for double-double) -- all of a thread's loads in flight at
+// once, no spills.
+template <class S>
+constexpr int rbk_scan_items() {
+  return sizeof(S) <= 16 ? 8 : 4;
+}
+template <class S>
+constexpr int rbk_scan_seg() {
+  return 256 * rbk_scan_items<S>();
+}
+
 template <class K, class V, class Op, int BLOCK, int ITEMS>
 __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, const K* keys,
                                          const V* vals, int64_t n, K* kout, V* vout,
                                          int64_t* num_runs, int64_t tile, uint64_t* words,
                                          uint32_t* err, bool single_tile,
-                                         const Seg<typename Acc<V, Op>::type>* prefixes = nullptr) {
+                                         const Seg<typename Acc<V, Op>::type>* prefixes = nullptr,
+                                         const Seg<typename Acc<V, Op>::type>* group_prefixes = nullptr) {
   using AccP = Acc<V, Op>;
   using A = typename AccP::type;
   using AOp = typename AccP::op;
@@ -319,8 +332,13 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
     }
   }
   __syncthreads();  // the tile's prefix is in sm.prefix
+  // three-pass form: prefix of the tile's scan segment, then the tile's
+  // exclusive prefix inside it
   const S tile_prefix =
-      prefixes ? prefixes[tile] : chained ? sm.prefix : SOp::template identity<S>();
+      prefixes ? (group_prefixes
+                      ? SOp::apply(group_prefixes[tile / rbk_scan_seg<S>()], prefixes[tile])
+                      : prefixes[tile])
+               : chained ? sm.prefix : SOp::template identity<S>();
   // The tile holding element n-1 reports the run count.
   if (threadIdx.x == 0 && tile * TILE + TILE >= n)
     *num_runs = int64_t(tile_prefix.c + tile_agg.c);
@@ -388,29 +406,79 @@ __global__ __launch_bounds__(BLOCK) void rbk_upsweep_kernel(
   using AccP = Acc<V, Op>;
   using A = typename AccP::type;
   using AOp = typename AccP::op;
+  using WK = WaveTile<K, ITEMS>;
   constexpr int TILE = BLOCK * ITEMS;
   constexpr int NW = BLOCK / kWave;
-  static_assert(ITEMS <= 32, "head flags are one 32-bit mask per thread");
+  constexpr int VE = WK::VE;         // keys per 16-byte vector
+  constexpr int NV = ITEMS / VE;     // vectors per thread
+  constexpr int NVEC = BLOCK * NV;   // vectors per tile
+  // Striped layout: thread t holds vectors j*BLOCK + t (coalesced 16-byte
+  // loads, 1 KB per wavefront instruction).  A vector's first key is
+  // compared with the last key of the vector before it, exchanged in LDS.
+  __shared__ K s_tail[NVEC];
   __shared__ uint32_t s_cnt[NW];
   __shared__ int s_last[NW];
   __shared__ A s_val[NW];
   const int wid = threadIdx.x / kWave;
   const int64_t base = int64_t(blockIdx.x) * TILE;
   const int cnt = int(n - base < TILE ? n - base : TILE);
-  const int mine = threadIdx.x * ITEMS;
-  int vt = cnt - mine;
-  vt = vt < 0 ? 0 : (vt > ITEMS ? ITEMS : vt);
-  const bool full = vt == ITEMS && aligned16(keys + base + mine);
-  K k[ITEMS];
-  load_blocked<K, ITEMS>(keys + base + mine, vt, k, K(), full);
-  const K kp = (vt > 0 && base + mine > 0) ? keys[base + mine - 1] : K();
-  uint32_t h = 0;
+  const bool full = cnt == TILE && aligned16(keys + base);
+  K k[NV][VE];
+  if (full) {
+    const u32x4* q = reinterpret_cast<const u32x4*>(keys + base);
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i) h |= uint32_t(!(k[i] == (i ? k[i - 1] : kp))) << i;
-  if (base + mine == 0) h |= 1u;  // the array's first element
-  h &= vt >= 32 ? ~0u : ((1u << vt) - 1);
-  const int last = h ? mine + 31 - __builtin_clz(h) : -1;
-  const uint32_t wc = wave_reduce<OpPlus>(uint32_t(__builtin_popcount(h)));
+    for (int j = 0; j < NV; ++j) {
+      const u32x4 t = __builtin_nontemporal_load(q + j * BLOCK + threadIdx.x);
+#pragma unroll
+      for (int c = 0; c < VE; ++c) k[j][c] = reinterpret_cast<const K*>(&t)[c];
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
+#pragma unroll
+      for (int c = 0; c < VE; ++c) {
+        const int e = (j * BLOCK + int(threadIdx.x)) * VE + c;
+        k[j][c] = e < cnt ? keys[base + e] : K();
+      }
+  }
+  // The last vectors of the tile (the last 64*VE keys) load their values
+  // with the keys: the tile's last head is nearly always among them, which
+  // saves a second memory round trip after the head search.
+  // (whole 16-byte value vectors only: not for 8-byte keys with 4-byte values)
+  constexpr bool kSpecOk = (VE * sizeof(V)) % 16 == 0;
+  const bool spec =
+      kSpecOk && full && int(threadIdx.x) >= BLOCK - kWave && aligned16(vals + base);
+  constexpr int SV = kSpecOk ? int(VE * sizeof(V)) / 16 : 1;  // value vectors per key vector
+  V sv[VE];
+  if (spec) {
+    const int e0 = ((NV - 1) * BLOCK + int(threadIdx.x)) * VE;
+    const u32x4* q = reinterpret_cast<const u32x4*>(vals + base + e0);
+#pragma unroll
+    for (int u = 0; u < SV; ++u) {
+      const u32x4 t = __builtin_nontemporal_load(q + u);
+#pragma unroll
+      for (int c = 0; c < VE / SV; ++c) sv[u * (VE / SV) + c] = reinterpret_cast<const V*>(&t)[c];
+    }
+  }
+  const K gprev = (threadIdx.x == 0 && base > 0) ? keys[base - 1] : K();
+#pragma unroll
+  for (int j = 0; j < NV; ++j) s_tail[j * BLOCK + threadIdx.x] = k[j][VE - 1];
+  __syncthreads();
+  uint32_t heads = 0;
+  int last = -1;
+#pragma unroll
+  for (int j = 0; j < NV; ++j) {
+    const int vi = j * BLOCK + int(threadIdx.x);
+    const K before = vi > 0 ? s_tail[vi - 1] : gprev;
+#pragma unroll
+    for (int c = 0; c < VE; ++c) {
+      const int e = vi * VE + c;
+      const bool head = e < cnt && (base + e == 0 || !(k[j][c] == (c ? k[j][c - 1] : before)));
+      heads += head;
+      last = head ? e : last;  // e grows with j and c
+    }
+  }
+  const uint32_t wc = wave_reduce<OpPlus>(heads);
   const int wl = wave_reduce<OpMax>(last);
   if (lane_id() == 0) {
     s_cnt[wid] = wc;
@@ -425,20 +493,33 @@ __global__ __launch_bounds__(BLOCK) void rbk_upsweep_kernel(
     from = s_last[w] > from ? s_last[w] : from;
   }
   from = from < 0 ? 0 : from;  // no head: the whole tile continues the open run
-  // fold of values [from, cnt) in element order: per thread, then across lanes
-  // and waves in order
+  // fold of values [from, cnt): each thread folds its elements of that range
+  // in element order, then lanes and waves combine in a fixed order
   A part = AOp::template identity<A>();
-  if (mine + vt > from) {
-    if (mine >= from && vt == ITEMS && aligned16(vals + base + mine)) {
-      V v[ITEMS];
-      load_blocked<V, ITEMS>(vals + base + mine, ITEMS, v, V(), true);
+  const int spec0 = ((NV - 1) * BLOCK + int(threadIdx.x)) * VE;
+  if (spec && from <= spec0) {
+    // this thread's whole last vector is in range; earlier ones only if from is
 #pragma unroll
-      for (int i = 0; i < ITEMS; ++i) part = AOp::apply(part, AccP::from(v[i]));
-    } else {
+    for (int j = 0; j < NV - 1; ++j)
 #pragma unroll
-      for (int i = 0; i < ITEMS; ++i)
-        if (i < vt && mine + i >= from) part = AOp::apply(part, AccP::from(vals[base + mine + i]));
-    }
+      for (int c = 0; c < VE; ++c) {
+        const int e = (j * BLOCK + int(threadIdx.x)) * VE + c;
+        if (e >= from) part = AOp::apply(part, AccP::from(vals[base + e]));
+      }
+#pragma unroll
+    for (int c = 0; c < VE; ++c) part = AOp::apply(part, AccP::from(sv[c]));
+  } else if (spec) {
+#pragma unroll
+    for (int c = 0; c < VE; ++c)
+      if (spec0 + c >= from) part = AOp::apply(part, AccP::from(sv[c]));
+  } else {
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
+#pragma unroll
+      for (int c = 0; c < VE; ++c) {
+        const int e = (j * BLOCK + int(threadIdx.x)) * VE + c;
+        if (e >= from && e < cnt) part = AOp::apply(part, AccP::from(vals[base + e]));
+      }
   }
   const A wv = wave_reduce<AOp>(part);
   if (lane_id() == 0) s_val[wid] = wv;
@@ -454,30 +535,35 @@ __global__ __launch_bounds__(BLOCK) void rbk_upsweep_kernel(
   }
 }
 
-// Pass 2: exclusive scan of the tile aggregates in place, by one
-// workgroup, ITEMS aggregates per thread per step.
+// Pass 2: exclusive scan of the tile aggregates in place.  With one
+// workgroup it scans all `count` of them, rbk_scan_seg() per step; with
+// several, workgroup b scans segment b only and writes the segment's total
+// to totals[b] (then scanned by one workgroup, and added by pass 3).
 template <class V, class Op>
-__global__ __launch_bounds__(1024) void rbk_scan_aggs_kernel(
-    Seg<typename Acc<V, Op>::type>* __restrict__ aggs, int64_t tiles) {
+__global__ __launch_bounds__(256) void rbk_scan_aggs_kernel(
+    Seg<typename Acc<V, Op>::type>* __restrict__ aggs, int64_t count,
+    Seg<typename Acc<V, Op>::type>* __restrict__ totals) {
   using AccP = Acc<V, Op>;
   using A = typename AccP::type;
   using S = Seg<A>;
   using SOp = SegOp<typename AccP::op, A>;
-  constexpr int ITEMS = 4;
-  constexpr int NW = 1024 / kWave;
+  constexpr int ITEMS = rbk_scan_items<S>();
+  constexpr int SEG = rbk_scan_seg<S>();
+  constexpr int NW = 256 / kWave;
   __shared__ S s_wave[NW];
   const int wid = threadIdx.x / kWave;
   const S id = SOp::template identity<S>();
   S carry = id;
-  for (int64_t c0 = 0; c0 < tiles; c0 += 1024 * ITEMS) {
+  const int64_t begin = int64_t(blockIdx.x) * SEG;
+  const int64_t end = gridDim.x == 1 ? count : (begin + SEG < count ? begin + SEG : count);
+  for (int64_t c0 = begin; c0 < end; c0 += SEG) {
     const int64_t i0 = c0 + int64_t(threadIdx.x) * ITEMS;
     S x[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; ++j) x[j] = i0 + j < end ? aggs[i0 + j] : id;
     S t = id;
 #pragma unroll
-    for (int j = 0; j < ITEMS; ++j) {
-      x[j] = i0 + j < tiles ? aggs[i0 + j] : id;
-      t = SOp::apply(t, x[j]);
-    }
+    for (int j = 0; j < ITEMS; ++j) t = SOp::apply(t, x[j]);
     const S inc = wave_inclusive_scan<SOp>(t);
     const S lane_excl = wave_shift_right<SOp>(inc);
     if (lane_id() == kWave - 1) s_wave[wid] = inc;
@@ -491,12 +577,13 @@ __global__ __launch_bounds__(1024) void rbk_scan_aggs_kernel(
     S pre = SOp::apply(carry, SOp::apply(wave_excl, lane_excl));
 #pragma unroll
     for (int j = 0; j < ITEMS; ++j) {
-      if (i0 + j < tiles) aggs[i0 + j] = pre;
+      if (i0 + j < end) aggs[i0 + j] = pre;
       pre = SOp::apply(pre, x[j]);
     }
     carry = SOp::apply(carry, total);
     __syncthreads();  // s_wave is rewritten by the next step
   }
+  if (totals && threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
 // Pass 3: one tile per workgroup, in any order.
@@ -504,10 +591,11 @@ template <class K, class V, class Op, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void rbk_downsweep_kernel(
     const K* __restrict__ keys, const V* __restrict__ vals, int64_t n, K* __restrict__ kout,
     V* __restrict__ vout, int64_t* __restrict__ num_runs,
-    const Seg<typename Acc<V, Op>::type>* __restrict__ prefixes) {
+    const Seg<typename Acc<V, Op>::type>* __restrict__ prefixes,
+    const Seg<typename Acc<V, Op>::type>* __restrict__ group_prefixes) {
   __shared__ RbkSmem<K, V, Op, BLOCK, ITEMS> sm;
   rbk_tile<K, V, Op, BLOCK, ITEMS>(sm, keys, vals, n, kout, vout, num_runs, blockIdx.x, nullptr,
-                                   nullptr, false, prefixes);
+                                   nullptr, false, prefixes, group_prefixes);
 }
 
 __global__ void rbk_empty_kernel(int64_t* num_runs);
@@ -537,16 +625,26 @@ int rbk_launch(const K* keys, const V* vals, int64_t n, K* kout, V* vout, int64_
     return BULKX_SUCCESS;
   }
   if constexpr (THREE_PASS) {
-    // the aggregates take half the room of the status words
-    static_assert(sizeof(S) <= TileStatus<S>::W * sizeof(uint64_t), "aggregates fit");
+    // the aggregates and the segment totals take at most the room of the
+    // status words (2 * sizeof(S) per tile)
+    static_assert(2 * sizeof(S) <= TileStatus<S>::W * sizeof(uint64_t), "aggregates fit");
     S* aggs = reinterpret_cast<S*>(ws + kRbkHeaderBytes);
     rbk_upsweep_kernel<K, V, Op, C::BLOCK, C::ITEMS><<<dim3(uint32_t(tiles)), C::BLOCK, 0, stream>>>(
         keys, vals, n, aggs);
     BULKX_LAUNCHED();
-    rbk_scan_aggs_kernel<V, Op><<<1, 1024, 0, stream>>>(aggs, tiles);
+    // pass 2: segments of rbk_scan_seg() aggregates in parallel, then their
+    // totals in one workgroup (the totals follow the aggregates)
+    const int64_t groups = (tiles + rbk_scan_seg<S>() - 1) / rbk_scan_seg<S>();
+    S* totals = groups > 1 ? aggs + tiles : nullptr;
+    rbk_scan_aggs_kernel<V, Op><<<dim3(uint32_t(groups)), 256, 0, stream>>>(aggs, tiles, totals);
     BULKX_LAUNCHED();
+    if (groups > 1) {
+      rbk_scan_aggs_kernel<V, Op><<<1, 256, 0, stream>>>(totals, groups, nullptr);
+      BULKX_LAUNCHED();
+    }
     rbk_downsweep_kernel<K, V, Op, C::BLOCK, C::ITEMS>
-        <<<dim3(uint32_t(tiles)), C::BLOCK, 0, stream>>>(keys, vals, n, kout, vout, num_runs, aggs);
+        <<<dim3(uint32_t(tiles)), C::BLOCK, 0, stream>>>(keys, vals, n, kout, vout, num_runs, aggs,
+                                                          totals);
     BULKX_LAUNCHED();
     return BULKX_SUCCESS;
   }
