@@ -393,6 +393,15 @@ static size_t per_call_workspace(const bulkx_job& j) {
 
 using namespace bulkx;
 
+// A recorded launch sequence and its kernel nodes whose grid follows the job
+// list.
+struct BatchGraph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<std::pair<hipGraphNode_t, int>> sized_nodes;  // (node, grid kind)
+  unsigned graph_grid[kGridKinds] = {};
+};
+
 struct bulkx_batch {
   hipStream_t stream = nullptr;
   hipStream_t capture_stream = nullptr;  // graphs are recorded here (capture is not
@@ -404,7 +413,7 @@ struct bulkx_batch {
   int max_jobs = 0;
   int64_t max_elements = 0;
   int64_t max_tiles = 0;
-  int max_passes = 0;
+  int max_passes = 0;  // rounds the largest allowed job could need
   int64_t fam_tiles[FAM_COUNT] = {};  // phase-0 grid of each family = its tile count
   // device
   BatchDesc* d_desc = nullptr;
@@ -439,11 +448,11 @@ struct bulkx_batch {
   hipEvent_t rev[2 * kBatchMaxPasses + 1] = {};
   int timed_passes = 0;
   bool timed = false;
-  // graph, and its kernel nodes whose grid follows the job list
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  std::vector<std::pair<hipGraphNode_t, int>> sized_nodes;  // (node, grid kind)
-  unsigned graph_grid[kGridKinds] = {};
+  // merge rounds the current job list needs (its largest sort job's)
+  int rounds = 0;
+  // one graph per number of merge rounds, recorded on first use, so a launch
+  // never runs a round with nothing to merge
+  BatchGraph graphs[kBatchMaxPasses + 1];
 };
 
 // Grid of each job-list-sized launch: the phase-0 families (one workgroup per
@@ -479,7 +488,7 @@ static int batch_enqueue_kernels(bulkx_batch* b, hipStream_t s, bool timed) {
     BULKX_LAUNCHED();
   }
   if (timed) BULKX_HIP(hipEventRecord(b->ev[FAM_COUNT], s));
-  for (int p = 0; p < b->max_passes; ++p) {
+  for (int p = 0; p < b->rounds; ++p) {
     if (timed) BULKX_HIP(hipEventRecord(b->rev[2 * p], s));
     batch_partition_kernel<<<batch_grid(b, grid_kind_partition(p)), 256, 0, s>>>(
         b->d_desc, b->d_jobs, b->d_sort_tile_job, b->d_splits, p);
@@ -491,7 +500,7 @@ static int batch_enqueue_kernels(bulkx_batch* b, hipStream_t s, bool timed) {
     if (timed) BULKX_HIP(hipEventRecord(b->rev[2 * p + 2], s));
   }
   if (timed) BULKX_HIP(hipEventRecord(b->ev[FAM_COUNT + 1], s));
-  b->timed_passes = b->max_passes;
+  b->timed_passes = b->rounds;
   return BULKX_SUCCESS;
 }
 
@@ -512,7 +521,7 @@ static int batch_enqueue_graph(bulkx_batch* b, hipStream_t s) {
   };
   family(FAM_SORT, b->side[0]);
   BULKX_LAUNCHED();
-  for (int p = 0; p < b->max_passes; ++p) {
+  for (int p = 0; p < b->rounds; ++p) {
     batch_partition_kernel<<<batch_grid(b, grid_kind_partition(p)), 256, 0, b->side[0]>>>(
         b->d_desc, b->d_jobs, b->d_sort_tile_job, b->d_splits, p);
     BULKX_LAUNCHED();
@@ -783,8 +792,10 @@ extern "C" int bulkx_batch_set_jobs(bulkx_batch* b, const bulkx_job* jobs, int n
   std::stable_sort(sort_jobs.begin(), sort_jobs.end(),
                    [&](int x, int y) { return b->h_jobs[x].passes > b->h_jobs[y].passes; });
   for (int r = 0; r < kBatchMaxPasses; ++r) b->active_sort_tiles[r] = 0;
+  b->rounds = 0;
   for (int i : sort_jobs) {
     BatchJobDev& dj = b->h_jobs[i];
+    b->rounds = dj.passes > b->rounds ? dj.passes : b->rounds;
     dj.sort_tile_begin = ts;
     for (int64_t k = 0; k < dj.tiles0; ++k) b->h_sort_tile_job[ts + k] = uint32_t(i);
     ts += dj.tiles0;
@@ -810,17 +821,21 @@ extern "C" int bulkx_batch_set_jobs(bulkx_batch* b, const bulkx_job* jobs, int n
   return BULKX_SUCCESS;
 }
 
-static void batch_drop_graph(bulkx_batch* b) {
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
-  b->exec = nullptr;
-  b->graph = nullptr;
-  b->sized_nodes.clear();
+static void batch_drop_one(BatchGraph& G) {
+  if (G.exec) (void)hipGraphExecDestroy(G.exec);
+  if (G.graph) (void)hipGraphDestroy(G.graph);
+  G.exec = nullptr;
+  G.graph = nullptr;
+  G.sized_nodes.clear();
 }
 
-// Records clear + phase-0 + merge rounds once on the private capture stream
-// and finds the phase-0 kernel nodes, whose grids follow the job list.
-static int batch_capture(bulkx_batch* b) {
+static void batch_drop_graph(bulkx_batch* b) {
+  for (BatchGraph& G : b->graphs) batch_drop_one(G);
+}
+
+// Records clear + phase-0 + the job list's merge rounds once on the private
+// capture stream and finds the kernel nodes whose grids follow the job list.
+static int batch_capture(bulkx_batch* b, BatchGraph& G) {
   BULKX_HIP(hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal));
   int st = batch_enqueue_graph(b, b->capture_stream);
   hipGraph_t g = nullptr;
@@ -833,7 +848,7 @@ static int batch_capture(bulkx_batch* b) {
     set_last_error("hipStreamEndCapture", e);
     return BULKX_ERR_HIP;
   }
-  b->graph = g;
+  G.graph = g;
   size_t num = 0;
   BULKX_HIP(hipGraphGetNodes(g, nullptr, &num));
   std::vector<hipGraphNode_t> nodes(num);
@@ -855,10 +870,10 @@ static int batch_capture(bulkx_batch* b) {
       kind = kp.func == reinterpret_cast<void*>(batch_pass_kernel) ? grid_kind_pass(round)
                                                                    : grid_kind_partition(round);
     }
-    if (kind >= 0) b->sized_nodes.emplace_back(node, kind);
+    if (kind >= 0) G.sized_nodes.emplace_back(node, kind);
   }
-  BULKX_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
-  for (int k = 0; k < kGridKinds; ++k) b->graph_grid[k] = batch_grid(b, k);
+  BULKX_HIP(hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0));
+  for (int k = 0; k < kGridKinds; ++k) G.graph_grid[k] = batch_grid(b, k);
   return BULKX_SUCCESS;
 }
 
@@ -868,33 +883,34 @@ extern "C" int bulkx_batch_launch(bulkx_batch* b, int use_graph) {
     b->timed = true;
     return batch_enqueue_kernels(b, b->stream, true);
   }
-  if (b->exec) {  // resize the job-list-sized grids in place
+  BatchGraph& G = b->graphs[b->rounds];
+  if (G.exec) {  // resize the job-list-sized grids in place
     bool stale[kGridKinds];
     bool any = false;
     for (int k = 0; k < kGridKinds; ++k) {
-      stale[k] = b->graph_grid[k] != batch_grid(b, k);
+      stale[k] = G.graph_grid[k] != batch_grid(b, k);
       any |= stale[k];
     }
     hipError_t e = hipSuccess;
-    for (const auto& nk : b->sized_nodes) {
+    for (const auto& nk : G.sized_nodes) {
       if (!any || e != hipSuccess) break;
       if (!stale[nk.second]) continue;
       hipKernelNodeParams kp;
       e = hipGraphKernelNodeGetParams(nk.first, &kp);
       if (e == hipSuccess) {
         kp.gridDim = dim3(batch_grid(b, nk.second));
-        e = hipGraphExecKernelNodeSetParams(b->exec, nk.first, &kp);
+        e = hipGraphExecKernelNodeSetParams(G.exec, nk.first, &kp);
       }
     }
     if (e != hipSuccess) {  // a node could not be updated: record the graph again
       (void)hipGetLastError();
-      batch_drop_graph(b);
+      batch_drop_one(G);
     } else {
-      for (int k = 0; k < kGridKinds; ++k) b->graph_grid[k] = batch_grid(b, k);
+      for (int k = 0; k < kGridKinds; ++k) G.graph_grid[k] = batch_grid(b, k);
     }
   }
-  if (!b->exec) BULKX_TRY(batch_capture(b));
-  BULKX_HIP(hipGraphLaunch(b->exec, b->stream));
+  if (!G.exec) BULKX_TRY(batch_capture(b, G));
+  BULKX_HIP(hipGraphLaunch(G.exec, b->stream));
   return BULKX_SUCCESS;
 }
 

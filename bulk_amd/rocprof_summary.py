@@ -2,7 +2,11 @@
 sources.
 
   python -m bulk_amd.rocprof_summary stats <dir> [--command "..."]
-      per-kernel calls / average / total from *_kernel_stats.csv
+      per-kernel calls / average / total from *_kernel_stats.csv, and for
+      the library's kernels the average over the dispatches that ran alone
+      (no other dispatch overlapping them in time, from *_kernel_trace.csv):
+      graph replays run independent branches concurrently, eager launches
+      (the benchmark's per-stage HIP-event timing) one kernel at a time
 
   python -m bulk_amd.rocprof_summary traffic <fetch_dir> <write_dir> --jobs J --seed S
       HBM bytes per dispatch from a FETCH_SIZE run and a WRITE_SIZE run
@@ -47,6 +51,26 @@ def stats(d: str, command: str | None, top: int) -> str:
     for r in rows[:top]:
         out.append(f"| {_short(r['Name'])} | {int(r['Calls'])} | {float(r['AverageNs']) / 1e3:.1f} | "
                    f"{float(r['TotalDurationNs']) / 1e6:.2f} | {float(r['Percentage']):.2f} |")
+    hits = glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True)
+    if hits:
+        disp = [(int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"])
+                for r in csv.DictReader(open(hits[0]))]
+        disp.sort()
+        alone = collections.defaultdict(list)
+        reach = -1  # latest end among the dispatches before i
+        for i, (s, e, n) in enumerate(disp):
+            nxt = disp[i + 1][0] if i + 1 < len(disp) else None
+            if s >= reach and (nxt is None or nxt >= e) and "bulkx" in n:
+                alone[_short(n)].append(e - s)
+            reach = max(reach, e)
+        if alone:
+            out.append("")
+            out.append("Library kernels, dispatches that ran alone:")
+            out.append("")
+            out.append("| kernel | alone | avg us |")
+            out.append("|---|---:|---:|")
+            for n, v in sorted(alone.items(), key=lambda kv: -sum(kv[1])):
+                out.append(f"| {n} | {len(v)} | {sum(v) / len(v) / 1e3:.1f} |")
     return "\n".join(out) + "\n"
 
 
