@@ -74,14 +74,20 @@ enum BatchFamily : int32_t {
 };
 constexpr int kBatchStages = FAM_COUNT + 1;  // the family kernels, then the sort merge rounds
 // Launch-grid kinds: one per phase-0 family, then (partition, pass) for
-// every merge round.
-constexpr int kGridKinds = FAM_COUNT + 2 * kBatchMaxPasses;
+// every merge round, then the aggregate scans of the three reduce_by_key
+// families.
+constexpr int kRbkFamilies = 3;
+constexpr int kGridKinds = FAM_COUNT + 2 * kBatchMaxPasses + kRbkFamilies;
 constexpr int kBatchBranches = 4;
 constexpr int grid_kind_partition(int round) { return FAM_COUNT + 2 * round; }
 constexpr int grid_kind_pass(int round) { return FAM_COUNT + 2 * round + 1; }
+constexpr int grid_kind_rbk_scan(int fam) { return FAM_COUNT + 2 * kBatchMaxPasses + fam - FAM_RBK_I32; }
 
 struct BatchDesc {
   int64_t fam_begin[FAM_COUNT + 1];  // family f owns tiles [fam_begin[f], fam_begin[f+1])
+  // multi-tile reduce_by_key jobs of family FAM_RBK_I32 + r:
+  // rbk_jobs[rbk_begin[r], rbk_begin[r + 1])
+  int64_t rbk_begin[kRbkFamilies + 1];
   int64_t total_sort_tiles;
   int64_t num_status_words;
   int64_t num_jobs;
@@ -147,6 +153,14 @@ __device__ __forceinline__ void sort_block_job(SM& sm, const BatchJobDev& job, i
                                                                              dk, dv, lt);
 }
 
+// Value type of a reduce_by_key family, and its tile aggregates (kept in the
+// job's status words).
+template <int FAM>
+using RbkValue = typename std::conditional<
+    FAM == FAM_RBK_I32, int32_t, typename std::conditional<FAM == FAM_RBK_F32, float, int64_t>::type>::type;
+template <int FAM>
+using RbkAgg = Seg<typename Acc<RbkValue<FAM>, OpPlus>::type>;
+
 // One phase-0 tile of a job of family FAM.
 template <int FAM>
 __device__ __forceinline__ void run_tile(FamilySmem<FAM>& sm, BatchDesc* d, const BatchJobDev& job,
@@ -170,21 +184,15 @@ __device__ __forceinline__ void run_tile(FamilySmem<FAM>& sm, BatchDesc* d, cons
       scan_tile<float, OpPlus, kBatchBlock, kBatchScanItems, false, false>(
           sm.s, static_cast<const float*>(job.in0), nullptr, job.n0, 0.f, false, lt, job.status,
           &d->err, single, static_cast<float*>(job.out0));
-  } else if constexpr (FAM == FAM_RBK_I32) {
-    rbk_tile<int32_t, int32_t, OpPlus, kBatchBlock, kBatchRbkItems>(
-        sm.s, static_cast<const int32_t*>(job.in0), static_cast<const int32_t*>(job.in1), job.n0,
-        static_cast<int32_t*>(job.out0), static_cast<int32_t*>(job.out1),
-        static_cast<int64_t*>(job.out2), lt, job.status, &d->err, single);
-  } else if constexpr (FAM == FAM_RBK_F32) {
-    rbk_tile<int32_t, float, OpPlus, kBatchBlock, kBatchRbkItems>(
-        sm.s, static_cast<const int32_t*>(job.in0), static_cast<const float*>(job.in1), job.n0,
-        static_cast<int32_t*>(job.out0), static_cast<float*>(job.out1),
-        static_cast<int64_t*>(job.out2), lt, job.status, &d->err, single);
-  } else if constexpr (FAM == FAM_RBK_I64) {
-    rbk_tile<int32_t, int64_t, OpPlus, kBatchBlock, kBatchRbkItems>(
-        sm.s, static_cast<const int32_t*>(job.in0), static_cast<const int64_t*>(job.in1), job.n0,
-        static_cast<int32_t*>(job.out0), static_cast<int64_t*>(job.out1),
-        static_cast<int64_t*>(job.out2), lt, job.status, &d->err, single);
+  } else if constexpr (FAM == FAM_RBK_I32 || FAM == FAM_RBK_F32 || FAM == FAM_RBK_I64) {
+    // the third pass: the tile's exclusive prefix was scanned by
+    // batch_rbk_scan_kernel into the job's status words
+    using V = RbkValue<FAM>;
+    rbk_tile<int32_t, V, OpPlus, kBatchBlock, kBatchRbkItems>(
+        sm.s, static_cast<const int32_t*>(job.in0), static_cast<const V*>(job.in1), job.n0,
+        static_cast<int32_t*>(job.out0), static_cast<V*>(job.out1),
+        static_cast<int64_t*>(job.out2), lt, nullptr, &d->err, single,
+        single ? nullptr : reinterpret_cast<const RbkAgg<FAM>*>(job.status));
   } else if constexpr (FAM == FAM_MERGE) {
     if (job.op == OP_MERGE_I32)
       merge_two_tile<CmpLess, int32_t, uint32_t, true, kBatchBlock, kBatchMergeItems>(
@@ -229,7 +237,8 @@ __global__ __launch_bounds__(kBatchBlock) void batch_phase0_kernel(
   const uint32_t j = tile_job[begin + g];
   const BatchJobDev& job = jobs[j];
   int64_t lt;
-  if constexpr (FAM == FAM_MERGE || FAM == FAM_SORT) {
+  if constexpr (FAM == FAM_MERGE || FAM == FAM_SORT || FAM == FAM_RBK_I32 ||
+                FAM == FAM_RBK_F32 || FAM == FAM_RBK_I64) {
     lt = begin + g - job.tile_begin0;
   } else {
     __shared__ uint32_t s_t;
@@ -238,6 +247,38 @@ __global__ __launch_bounds__(kBatchBlock) void batch_phase0_kernel(
     lt = wave_uniform(s_t);
   }
   run_tile<FAM>(sm, d, job, lt);
+}
+
+// Three-pass reduce_by_key in the batch (the device-wide three-pass form,
+// rbk_upsweep_kernel): pass 1 writes the aggregate of every tile of the
+// family's multi-tile jobs into the job's status words, pass 2 scans each
+// job's aggregates in place (one workgroup per job), pass 3 is the family's
+// phase-0 kernel with the tile's prefix read, not looked back.  No tile ever
+// waits on another, and the tiles need no claim counter.
+template <int FAM>
+__global__ __launch_bounds__(kBatchBlock) void batch_rbk_upsweep_kernel(
+    BatchDesc* d, const BatchJobDev* __restrict__ jobs, const uint32_t* __restrict__ tile_job) {
+  const int64_t begin = d->fam_begin[FAM];
+  const int64_t g = int64_t(blockIdx.x);
+  if (g >= d->fam_begin[FAM + 1] - begin) return;
+  const BatchJobDev& job = jobs[tile_job[begin + g]];
+  if (job.tiles0 <= 1) return;
+  using V = RbkValue<FAM>;
+  rbk_upsweep_tile<int32_t, V, OpPlus, kBatchBlock, kBatchRbkItems>(
+      static_cast<const int32_t*>(job.in0), static_cast<const V*>(job.in1), job.n0,
+      reinterpret_cast<RbkAgg<FAM>*>(job.status), begin + g - job.tile_begin0);
+}
+
+template <int FAM>
+__global__ __launch_bounds__(256) void batch_rbk_scan_kernel(BatchDesc* d,
+                                                             const BatchJobDev* __restrict__ jobs,
+                                                             const uint32_t* __restrict__ rbk_jobs) {
+  const int r = FAM - FAM_RBK_I32;
+  const int64_t b = d->rbk_begin[r] + int64_t(blockIdx.x);
+  if (b >= d->rbk_begin[r + 1]) return;
+  const BatchJobDev& job = jobs[rbk_jobs[b]];
+  (void)rbk_scan_range<RbkValue<FAM>, OpPlus>(reinterpret_cast<RbkAgg<FAM>*>(job.status), 0,
+                                              job.tiles0);
 }
 
 template <class K>
@@ -420,6 +461,7 @@ struct bulkx_batch {
   BatchJobDev* d_jobs = nullptr;
   uint32_t* d_tile_job = nullptr;
   uint32_t* d_sort_tile_job = nullptr;
+  uint32_t* d_rbk_jobs = nullptr;  // multi-tile reduce_by_key jobs, by family
   int64_t* d_splits = nullptr;  // merge-path splits of the current merge round
   uint32_t* d_job_ctr = nullptr;  // per-job tile counters of the look-back families
   uint64_t* d_status = nullptr;
@@ -433,6 +475,8 @@ struct bulkx_batch {
   BatchJobDev* h_jobs = nullptr;
   uint32_t* h_tile_job = nullptr;
   uint32_t* h_sort_tile_job = nullptr;
+  uint32_t* h_rbk_jobs = nullptr;
+  int64_t rbk_jobs[kRbkFamilies] = {};  // multi-tile jobs of each reduce_by_key family
   // current job list
   std::vector<bulkx_job> jobs;
   int num_jobs = 0;
@@ -463,6 +507,8 @@ static unsigned batch_grid(const bulkx_batch* b, int kind) {
   int64_t g;
   if (kind < FAM_COUNT) {
     g = b->fam_tiles[kind];
+  } else if (kind >= grid_kind_rbk_scan(FAM_RBK_I32)) {
+    g = b->rbk_jobs[kind - grid_kind_rbk_scan(FAM_RBK_I32)];
   } else {
     const int r = (kind - FAM_COUNT) / 2;
     const int64_t active = b->active_sort_tiles[r];
@@ -478,14 +524,39 @@ static const Phase0Kernel kPhase0[FAM_COUNT] = {
     batch_phase0_kernel<FAM_RBK_I64>,  batch_phase0_kernel<FAM_MERGE>,
     batch_phase0_kernel<FAM_SORT>};
 
+using RbkUpKernel = void (*)(BatchDesc*, const BatchJobDev*, const uint32_t*);
+static const RbkUpKernel kRbkUp[kRbkFamilies] = {batch_rbk_upsweep_kernel<FAM_RBK_I32>,
+                                                 batch_rbk_upsweep_kernel<FAM_RBK_F32>,
+                                                 batch_rbk_upsweep_kernel<FAM_RBK_I64>};
+static const RbkUpKernel kRbkScan[kRbkFamilies] = {batch_rbk_scan_kernel<FAM_RBK_I32>,
+                                                   batch_rbk_scan_kernel<FAM_RBK_F32>,
+                                                   batch_rbk_scan_kernel<FAM_RBK_I64>};
+
+static bool is_rbk_family(int f) { return f == FAM_RBK_I32 || f == FAM_RBK_F32 || f == FAM_RBK_I64; }
+
+// The passes of family f on stream q: for reduce_by_key the aggregates and
+// their per-job scans come first.
+static int batch_enqueue_family(bulkx_batch* b, int f, hipStream_t q) {
+  if (is_rbk_family(f)) {
+    const int r = f - FAM_RBK_I32;
+    kRbkUp[r]<<<batch_grid(b, f), kBatchBlock, 0, q>>>(b->d_desc, b->d_jobs, b->d_tile_job);
+    BULKX_LAUNCHED();
+    kRbkScan[r]<<<batch_grid(b, grid_kind_rbk_scan(f)), 256, 0, q>>>(b->d_desc, b->d_jobs,
+                                                                     b->d_rbk_jobs);
+    BULKX_LAUNCHED();
+  }
+  kPhase0[f]<<<batch_grid(b, f), kBatchBlock, 0, q>>>(b->d_desc, b->d_jobs, b->d_tile_job,
+                                                      b->d_job_ctr);
+  BULKX_LAUNCHED();
+  return BULKX_SUCCESS;
+}
+
 static int batch_enqueue_kernels(bulkx_batch* b, hipStream_t s, bool timed) {
   batch_clear_kernel<<<256, 256, 0, s>>>(b->d_desc, b->d_status, b->d_job_ctr);
   BULKX_LAUNCHED();
   for (int f = 0; f < FAM_COUNT; ++f) {
     if (timed) BULKX_HIP(hipEventRecord(b->ev[f], s));
-    kPhase0[f]<<<batch_grid(b, f), kBatchBlock, 0, s>>>(b->d_desc, b->d_jobs, b->d_tile_job,
-                                                        b->d_job_ctr);
-    BULKX_LAUNCHED();
+    BULKX_TRY(batch_enqueue_family(b, f, s));
   }
   if (timed) BULKX_HIP(hipEventRecord(b->ev[FAM_COUNT], s));
   for (int p = 0; p < b->rounds; ++p) {
@@ -515,12 +586,7 @@ static int batch_enqueue_graph(bulkx_batch* b, hipStream_t s) {
   BULKX_LAUNCHED();
   BULKX_HIP(hipEventRecord(b->fork_ev, s));
   for (int br = 0; br < kBatchBranches; ++br) BULKX_HIP(hipStreamWaitEvent(b->side[br], b->fork_ev, 0));
-  auto family = [&](int f, hipStream_t q) {
-    kPhase0[f]<<<batch_grid(b, f), kBatchBlock, 0, q>>>(b->d_desc, b->d_jobs, b->d_tile_job,
-                                                        b->d_job_ctr);
-  };
-  family(FAM_SORT, b->side[0]);
-  BULKX_LAUNCHED();
+  BULKX_TRY(batch_enqueue_family(b, FAM_SORT, b->side[0]));
   for (int p = 0; p < b->rounds; ++p) {
     batch_partition_kernel<<<batch_grid(b, grid_kind_partition(p)), 256, 0, b->side[0]>>>(
         b->d_desc, b->d_jobs, b->d_sort_tile_job, b->d_splits, p);
@@ -529,16 +595,11 @@ static int batch_enqueue_graph(bulkx_batch* b, hipStream_t s) {
         b->d_desc, b->d_jobs, b->d_sort_tile_job, b->d_splits, p);
     BULKX_LAUNCHED();
   }
-  family(FAM_MERGE, b->side[1]);
-  BULKX_LAUNCHED();
-  for (int f : {int(FAM_RBK_I32), int(FAM_RBK_F32), int(FAM_RBK_I64)}) {
-    family(f, b->side[2]);
-    BULKX_LAUNCHED();
-  }
-  for (int f : {int(FAM_SCAN_I32), int(FAM_SCAN_F32)}) {
-    family(f, b->side[3]);
-    BULKX_LAUNCHED();
-  }
+  BULKX_TRY(batch_enqueue_family(b, FAM_MERGE, b->side[1]));
+  for (int f : {int(FAM_RBK_I32), int(FAM_RBK_F32), int(FAM_RBK_I64)})
+    BULKX_TRY(batch_enqueue_family(b, f, b->side[2]));
+  for (int f : {int(FAM_SCAN_I32), int(FAM_SCAN_F32)})
+    BULKX_TRY(batch_enqueue_family(b, f, b->side[3]));
   for (int br = 0; br < kBatchBranches; ++br) {
     BULKX_HIP(hipEventRecord(b->join_ev[br], b->side[br]));
     BULKX_HIP(hipStreamWaitEvent(s, b->join_ev[br], 0));
@@ -588,6 +649,7 @@ extern "C" int bulkx_batch_create(int max_jobs, int64_t max_elements, void* stre
   dmalloc(reinterpret_cast<void**>(&b->d_jobs), sizeof(BatchJobDev) * max_jobs);
   dmalloc(reinterpret_cast<void**>(&b->d_tile_job), sizeof(uint32_t) * b->max_tiles);
   dmalloc(reinterpret_cast<void**>(&b->d_sort_tile_job), sizeof(uint32_t) * b->max_tiles);
+  dmalloc(reinterpret_cast<void**>(&b->d_rbk_jobs), sizeof(uint32_t) * max_jobs);
   dmalloc(reinterpret_cast<void**>(&b->d_splits), sizeof(int64_t) * b->max_tiles);
   dmalloc(reinterpret_cast<void**>(&b->d_job_ctr), sizeof(uint32_t) * max_jobs);
   dmalloc(reinterpret_cast<void**>(&b->d_status), sizeof(uint64_t) * b->status_words_cap);
@@ -596,6 +658,7 @@ extern "C" int bulkx_batch_create(int max_jobs, int64_t max_elements, void* stre
   hmalloc(reinterpret_cast<void**>(&b->h_jobs), sizeof(BatchJobDev) * max_jobs);
   hmalloc(reinterpret_cast<void**>(&b->h_tile_job), sizeof(uint32_t) * b->max_tiles);
   hmalloc(reinterpret_cast<void**>(&b->h_sort_tile_job), sizeof(uint32_t) * b->max_tiles);
+  hmalloc(reinterpret_cast<void**>(&b->h_rbk_jobs), sizeof(uint32_t) * max_jobs);
   if (st != BULKX_SUCCESS) {
     bulkx_batch_destroy(b);
     return st;
@@ -642,10 +705,11 @@ extern "C" int bulkx_batch_destroy(bulkx_batch* b) {
                   static_cast<void*>(b->d_tile_job), static_cast<void*>(b->d_sort_tile_job),
                   static_cast<void*>(b->d_splits), static_cast<void*>(b->d_job_ctr),
                   static_cast<void*>(b->d_status), static_cast<void*>(b->d_tmp),
-                  static_cast<void*>(b->d_ws)})
+                  static_cast<void*>(b->d_ws), static_cast<void*>(b->d_rbk_jobs)})
     if (p) (void)hipFree(p);
   for (void* p : {static_cast<void*>(b->h_desc), static_cast<void*>(b->h_jobs),
-                  static_cast<void*>(b->h_tile_job), static_cast<void*>(b->h_sort_tile_job)})
+                  static_cast<void*>(b->h_tile_job), static_cast<void*>(b->h_sort_tile_job),
+                  static_cast<void*>(b->h_rbk_jobs)})
     if (p) (void)hipHostFree(p);
   delete b;
   return BULKX_SUCCESS;
@@ -792,6 +856,24 @@ extern "C" int bulkx_batch_set_jobs(bulkx_batch* b, const bulkx_job* jobs, int n
   std::stable_sort(sort_jobs.begin(), sort_jobs.end(),
                    [&](int x, int y) { return b->h_jobs[x].passes > b->h_jobs[y].passes; });
   for (int r = 0; r < kBatchMaxPasses; ++r) b->active_sort_tiles[r] = 0;
+  // multi-tile reduce_by_key jobs by family, for the aggregate scans
+  {
+    int64_t c = 0;
+    for (int r = 0; r < kRbkFamilies; ++r) {
+      b->h_desc->rbk_begin[r] = c;
+      for (int i = 0; i < num_jobs; ++i) {
+        const BatchJobDev& dj = b->h_jobs[i];
+        if (dj.tiles0 > 1 && host_family(dj.op) == FAM_RBK_I32 + r &&
+            dj.op >= OP_RBK_I32_I32 && dj.op <= OP_RBK_I32_I64)
+          b->h_rbk_jobs[c++] = uint32_t(i);
+      }
+      b->rbk_jobs[r] = c - b->h_desc->rbk_begin[r];
+    }
+    b->h_desc->rbk_begin[kRbkFamilies] = c;
+    if (c > 0)
+      BULKX_HIP(hipMemcpyAsync(b->d_rbk_jobs, b->h_rbk_jobs, sizeof(uint32_t) * c,
+                               hipMemcpyHostToDevice, b->stream));
+  }
   b->rounds = 0;
   for (int i : sort_jobs) {
     BatchJobDev& dj = b->h_jobs[i];
@@ -862,6 +944,10 @@ static int batch_capture(bulkx_batch* b, BatchGraph& G) {
     int kind = -1;
     for (int f = 0; f < FAM_COUNT; ++f)
       if (kp.func == reinterpret_cast<void*>(kPhase0[f])) kind = f;
+    for (int r = 0; r < kRbkFamilies; ++r) {
+      if (kp.func == reinterpret_cast<void*>(kRbkUp[r])) kind = FAM_RBK_I32 + r;
+      if (kp.func == reinterpret_cast<void*>(kRbkScan[r])) kind = grid_kind_rbk_scan(FAM_RBK_I32 + r);
+    }
     // the merge-round kernels' last argument is the round
     if (kp.func == reinterpret_cast<void*>(batch_partition_kernel) ||
         kp.func == reinterpret_cast<void*>(batch_pass_kernel)) {

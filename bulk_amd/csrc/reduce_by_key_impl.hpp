@@ -400,9 +400,10 @@ __global__ __launch_bounds__(BLOCK) void reduce_by_key_kernel(
 // no look-back, whose waits on the slowest predecessor of a window bound
 // the single-pass kernel on large inputs.
 template <class K, class V, class Op, int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void rbk_upsweep_kernel(
-    const K* __restrict__ keys, const V* __restrict__ vals, int64_t n,
-    Seg<typename Acc<V, Op>::type>* __restrict__ aggs) {
+__device__ __forceinline__ void rbk_upsweep_tile(const K* __restrict__ keys,
+                                                 const V* __restrict__ vals, int64_t n,
+                                                 Seg<typename Acc<V, Op>::type>* __restrict__ aggs,
+                                                 int64_t tile) {
   using AccP = Acc<V, Op>;
   using A = typename AccP::type;
   using AOp = typename AccP::op;
@@ -420,7 +421,7 @@ __global__ __launch_bounds__(BLOCK) void rbk_upsweep_kernel(
   __shared__ int s_last[NW];
   __shared__ A s_val[NW];
   const int wid = threadIdx.x / kWave;
-  const int64_t base = int64_t(blockIdx.x) * TILE;
+  const int64_t base = tile * TILE;
   const int cnt = int(n - base < TILE ? n - base : TILE);
   const bool full = cnt == TILE && aligned16(keys + base);
   K k[NV][VE];
@@ -531,18 +532,26 @@ __global__ __launch_bounds__(BLOCK) void rbk_upsweep_kernel(
     Seg<A> g;
     g.c = tc;
     g.v = v;
-    aggs[blockIdx.x] = g;
+    aggs[tile] = g;
   }
+}
+
+template <class K, class V, class Op, int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void rbk_upsweep_kernel(
+    const K* __restrict__ keys, const V* __restrict__ vals, int64_t n,
+    Seg<typename Acc<V, Op>::type>* __restrict__ aggs) {
+  rbk_upsweep_tile<K, V, Op, BLOCK, ITEMS>(keys, vals, n, aggs, blockIdx.x);
 }
 
 // Pass 2: exclusive scan of the tile aggregates in place.  With one
 // workgroup it scans all `count` of them, rbk_scan_seg() per step; with
 // several, workgroup b scans segment b only and writes the segment's total
 // to totals[b] (then scanned by one workgroup, and added by pass 3).
+// Exclusive scan of aggs[begin, end) in place by one 256-thread workgroup;
+// returns the total (in every thread).
 template <class V, class Op>
-__global__ __launch_bounds__(256) void rbk_scan_aggs_kernel(
-    Seg<typename Acc<V, Op>::type>* __restrict__ aggs, int64_t count,
-    Seg<typename Acc<V, Op>::type>* __restrict__ totals) {
+__device__ __forceinline__ Seg<typename Acc<V, Op>::type> rbk_scan_range(
+    Seg<typename Acc<V, Op>::type>* __restrict__ aggs, int64_t begin, int64_t end) {
   using AccP = Acc<V, Op>;
   using A = typename AccP::type;
   using S = Seg<A>;
@@ -554,8 +563,6 @@ __global__ __launch_bounds__(256) void rbk_scan_aggs_kernel(
   const int wid = threadIdx.x / kWave;
   const S id = SOp::template identity<S>();
   S carry = id;
-  const int64_t begin = int64_t(blockIdx.x) * SEG;
-  const int64_t end = gridDim.x == 1 ? count : (begin + SEG < count ? begin + SEG : count);
   for (int64_t c0 = begin; c0 < end; c0 += SEG) {
     const int64_t i0 = c0 + int64_t(threadIdx.x) * ITEMS;
     S x[ITEMS];
@@ -583,7 +590,19 @@ __global__ __launch_bounds__(256) void rbk_scan_aggs_kernel(
     carry = SOp::apply(carry, total);
     __syncthreads();  // s_wave is rewritten by the next step
   }
-  if (totals && threadIdx.x == 0) totals[blockIdx.x] = carry;
+  return carry;
+}
+
+template <class V, class Op>
+__global__ __launch_bounds__(256) void rbk_scan_aggs_kernel(
+    Seg<typename Acc<V, Op>::type>* __restrict__ aggs, int64_t count,
+    Seg<typename Acc<V, Op>::type>* __restrict__ totals) {
+  using S = Seg<typename Acc<V, Op>::type>;
+  constexpr int SEG = rbk_scan_seg<S>();
+  const int64_t begin = int64_t(blockIdx.x) * SEG;
+  const int64_t end = gridDim.x == 1 ? count : (begin + SEG < count ? begin + SEG : count);
+  const S total = rbk_scan_range<V, Op>(aggs, begin, end);
+  if (totals && threadIdx.x == 0) totals[blockIdx.x] = total;
 }
 
 // Pass 3: one tile per workgroup, in any order.
