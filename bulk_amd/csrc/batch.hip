@@ -29,11 +29,14 @@ namespace bulkx {
 
 constexpr int kBatchBlock = 512;
 constexpr int kBatchScanItems = 32;   // 16384 elements per scan / reduce tile
-constexpr int kBatchRbkItems = 8;     // 4096 elements per reduce_by_key tile
+constexpr int kBatchRbkItems = 16;    // 8192 elements per reduce_by_key tile
 constexpr int kBatchMergeItems = 8;   // 4096 outputs per merge / sort tile
 constexpr int64_t kBatchScanTile = int64_t(kBatchBlock) * kBatchScanItems;
 constexpr int64_t kBatchRbkTile = int64_t(kBatchBlock) * kBatchRbkItems;
 constexpr int64_t kBatchMergeTile = int64_t(kBatchBlock) * kBatchMergeItems;
+constexpr int64_t kBatchMinTile = kBatchScanTile < kBatchRbkTile
+                                      ? (kBatchScanTile < kBatchMergeTile ? kBatchScanTile : kBatchMergeTile)
+                                      : (kBatchRbkTile < kBatchMergeTile ? kBatchRbkTile : kBatchMergeTile);
 constexpr int kBatchMaxPasses = 24;
 constexpr int kStatusWordsPerTile = 6;  // widest look-back record used here (24 bytes)
 
@@ -619,7 +622,7 @@ extern "C" int bulkx_batch_create(int max_jobs, int64_t max_elements, void* stre
   b->max_jobs = max_jobs;
   b->max_elements = max_elements;
   // every job can add one partial tile
-  b->max_tiles = max_elements / kBatchRbkTile + 2 * int64_t(max_jobs) + 1;
+  b->max_tiles = max_elements / kBatchMinTile + 2 * int64_t(max_jobs) + 1;
   b->max_passes = passes_for((max_elements + kBatchMergeTile - 1) / kBatchMergeTile);
   if (b->max_passes > kBatchMaxPasses) {
     delete b;
