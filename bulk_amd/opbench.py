@@ -182,8 +182,9 @@ def run(iters: int = 10, log=print, large: int = 28) -> list[dict]:
         api.stable_merge_sort_by_key(k, v, workspace=wst)
 
     ms = _timeit(srt, iters) - t_copy
-    tiles = (s + 4095) // 4096
-    passes = max(0, math.ceil(math.log2(tiles)))
+    _, gs, gr = api.choose_sizes(C.ALGO_SORT_BY_KEY, api.dtype_code(k), s)  # block-sorted run length
+    runs = (s + gs * gr - 1) // (gs * gr)
+    passes = max(0, math.ceil(math.log2(runs)))
     rows.append(_row(f"stable_sort_by_key u32+u32 2^{large - 2}", ms, 2 * s * 8, s, merge_passes=passes,
                      pass_bytes=(1 + passes) * 2 * s * 8,
                      note="time excludes restoring the unsorted input (timed separately)"))
