@@ -61,6 +61,7 @@ struct BatchJobDev {
   void* out1;
   void* out2;
   uint64_t* status;  // look-back words (null for single-tile jobs)
+  int64_t* splits;   // merge: the merge path of every tile boundary (tiles0 + 1)
   void* tmp_k;       // sort ping-pong buffers
   void* tmp_v;
   int64_t tiles0;       // phase-0 tiles
@@ -80,11 +81,12 @@ constexpr int kBatchStages = FAM_COUNT + 1;  // the family kernels, then the sor
 // every merge round, then the aggregate scans of the three reduce_by_key
 // families.
 constexpr int kRbkFamilies = 3;
-constexpr int kGridKinds = FAM_COUNT + 2 * kBatchMaxPasses + kRbkFamilies;
+constexpr int kGridKinds = FAM_COUNT + 2 * kBatchMaxPasses + kRbkFamilies + 1;
 constexpr int kBatchBranches = 4;
 constexpr int grid_kind_partition(int round) { return FAM_COUNT + 2 * round; }
 constexpr int grid_kind_pass(int round) { return FAM_COUNT + 2 * round + 1; }
 constexpr int grid_kind_rbk_scan(int fam) { return FAM_COUNT + 2 * kBatchMaxPasses + fam - FAM_RBK_I32; }
+constexpr int kGridMergePartition = FAM_COUNT + 2 * kBatchMaxPasses + kRbkFamilies;
 
 struct BatchDesc {
   int64_t fam_begin[FAM_COUNT + 1];  // family f owns tiles [fam_begin[f], fam_begin[f+1])
@@ -201,12 +203,14 @@ __device__ __forceinline__ void run_tile(FamilySmem<FAM>& sm, BatchDesc* d, cons
       merge_two_tile<CmpLess, int32_t, uint32_t, true, kBatchBlock, kBatchMergeItems>(
           sm.i, static_cast<const int32_t*>(job.in0), static_cast<const uint32_t*>(job.in1),
           job.n0, static_cast<const int32_t*>(job.in2), static_cast<const uint32_t*>(job.in3),
-          job.n1, static_cast<int32_t*>(job.out0), static_cast<uint32_t*>(job.out1), lt);
+          job.n1, static_cast<int32_t*>(job.out0), static_cast<uint32_t*>(job.out1), lt,
+          job.splits);
     else
       merge_two_tile<CmpLess, float, uint32_t, true, kBatchBlock, kBatchMergeItems>(
           sm.f, static_cast<const float*>(job.in0), static_cast<const uint32_t*>(job.in1),
           job.n0, static_cast<const float*>(job.in2), static_cast<const uint32_t*>(job.in3),
-          job.n1, static_cast<float*>(job.out0), static_cast<uint32_t*>(job.out1), lt);
+          job.n1, static_cast<float*>(job.out0), static_cast<uint32_t*>(job.out1), lt,
+          job.splits);
   } else {
     switch (job.op) {
       case OP_SORT_I32: sort_block_job<int32_t>(sm.i, job, lt); break;
@@ -250,6 +254,29 @@ __global__ __launch_bounds__(kBatchBlock) void batch_phase0_kernel(
     lt = wave_uniform(s_t);
   }
   run_tile<FAM>(sm, d, job, lt);
+}
+
+// The merge path of every merge tile's first diagonal, one lane per tile
+// (as merge_split_kernel); the last tile of a job also writes the end
+// boundary (all of A).  Searched once here instead of by each tile.
+__global__ __launch_bounds__(256) void batch_merge_partition_kernel(
+    BatchDesc* d, const BatchJobDev* __restrict__ jobs, const uint32_t* __restrict__ tile_job) {
+  const int64_t begin = d->fam_begin[FAM_MERGE];
+  const int64_t g = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (g >= d->fam_begin[FAM_MERGE + 1] - begin) return;
+  const BatchJobDev& job = jobs[tile_job[begin + g]];
+  const int64_t lt = begin + g - job.tile_begin0;
+  const int64_t n = job.n0 + job.n1;
+  const int64_t diag = lt * kBatchMergeTile < n ? lt * kBatchMergeTile : n;
+  int64_t r;
+  if (job.op == OP_MERGE_I32)
+    r = lane_merge_path<CmpLess>(static_cast<const int32_t*>(job.in0), job.n0,
+                                 static_cast<const int32_t*>(job.in2), job.n1, diag);
+  else
+    r = lane_merge_path<CmpLess>(static_cast<const float*>(job.in0), job.n0,
+                                 static_cast<const float*>(job.in2), job.n1, diag);
+  job.splits[lt] = r;
+  if (lt == job.tiles0 - 1) job.splits[lt + 1] = job.n0;
 }
 
 // Three-pass reduce_by_key in the batch (the device-wide three-pass form,
@@ -465,6 +492,7 @@ struct bulkx_batch {
   uint32_t* d_tile_job = nullptr;
   uint32_t* d_sort_tile_job = nullptr;
   uint32_t* d_rbk_jobs = nullptr;  // multi-tile reduce_by_key jobs, by family
+  int64_t* d_merge_splits = nullptr;  // tile boundaries of the merge jobs
   int64_t* d_splits = nullptr;  // merge-path splits of the current merge round
   uint32_t* d_job_ctr = nullptr;  // per-job tile counters of the look-back families
   uint64_t* d_status = nullptr;
@@ -510,6 +538,8 @@ static unsigned batch_grid(const bulkx_batch* b, int kind) {
   int64_t g;
   if (kind < FAM_COUNT) {
     g = b->fam_tiles[kind];
+  } else if (kind == kGridMergePartition) {
+    g = (b->fam_tiles[FAM_MERGE] + 255) / 256;
   } else if (kind >= grid_kind_rbk_scan(FAM_RBK_I32)) {
     g = b->rbk_jobs[kind - grid_kind_rbk_scan(FAM_RBK_I32)];
   } else {
@@ -540,6 +570,11 @@ static bool is_rbk_family(int f) { return f == FAM_RBK_I32 || f == FAM_RBK_F32 |
 // The passes of family f on stream q: for reduce_by_key the aggregates and
 // their per-job scans come first.
 static int batch_enqueue_family(bulkx_batch* b, int f, hipStream_t q) {
+  if (f == FAM_MERGE) {
+    batch_merge_partition_kernel<<<batch_grid(b, kGridMergePartition), 256, 0, q>>>(
+        b->d_desc, b->d_jobs, b->d_tile_job);
+    BULKX_LAUNCHED();
+  }
   if (is_rbk_family(f)) {
     const int r = f - FAM_RBK_I32;
     kRbkUp[r]<<<batch_grid(b, f), kBatchBlock, 0, q>>>(b->d_desc, b->d_jobs, b->d_tile_job);
@@ -653,6 +688,8 @@ extern "C" int bulkx_batch_create(int max_jobs, int64_t max_elements, void* stre
   dmalloc(reinterpret_cast<void**>(&b->d_tile_job), sizeof(uint32_t) * b->max_tiles);
   dmalloc(reinterpret_cast<void**>(&b->d_sort_tile_job), sizeof(uint32_t) * b->max_tiles);
   dmalloc(reinterpret_cast<void**>(&b->d_rbk_jobs), sizeof(uint32_t) * max_jobs);
+  dmalloc(reinterpret_cast<void**>(&b->d_merge_splits),
+          sizeof(int64_t) * (b->max_tiles + int64_t(max_jobs)));
   dmalloc(reinterpret_cast<void**>(&b->d_splits), sizeof(int64_t) * b->max_tiles);
   dmalloc(reinterpret_cast<void**>(&b->d_job_ctr), sizeof(uint32_t) * max_jobs);
   dmalloc(reinterpret_cast<void**>(&b->d_status), sizeof(uint64_t) * b->status_words_cap);
@@ -708,7 +745,8 @@ extern "C" int bulkx_batch_destroy(bulkx_batch* b) {
                   static_cast<void*>(b->d_tile_job), static_cast<void*>(b->d_sort_tile_job),
                   static_cast<void*>(b->d_splits), static_cast<void*>(b->d_job_ctr),
                   static_cast<void*>(b->d_status), static_cast<void*>(b->d_tmp),
-                  static_cast<void*>(b->d_ws), static_cast<void*>(b->d_rbk_jobs)})
+                  static_cast<void*>(b->d_ws), static_cast<void*>(b->d_rbk_jobs),
+                  static_cast<void*>(b->d_merge_splits)})
     if (p) (void)hipFree(p);
   for (void* p : {static_cast<void*>(b->h_desc), static_cast<void*>(b->h_jobs),
                   static_cast<void*>(b->h_tile_job), static_cast<void*>(b->h_sort_tile_job),
@@ -808,7 +846,7 @@ extern "C" int bulkx_batch_set_jobs(bulkx_batch* b, const bulkx_job* jobs, int n
     cursor[f] = b->h_desc->fam_begin[f];
     b->h_desc->fam_begin[f + 1] = b->h_desc->fam_begin[f] + fam_tiles[f];
   }
-  int64_t ts = 0;
+  int64_t ts = 0, ms = 0;
   size_t sw = 0, tb = 0;
   std::vector<int> sort_jobs;
   for (int i = 0; i < num_jobs; ++i) {
@@ -839,6 +877,10 @@ extern "C" int bulkx_batch_set_jobs(bulkx_batch* b, const bulkx_job* jobs, int n
     if (chained && t > 1) {
       dj.status = b->d_status + sw;
       sw += size_t(t) * kStatusWordsPerTile;
+    }
+    if (op == OP_MERGE_I32 || op == OP_MERGE_F32) {
+      dj.splits = b->d_merge_splits + ms;
+      ms += t + 1;
     }
     if (op >= OP_SORT_I32) {
       dj.passes = passes_for(t);
@@ -947,6 +989,7 @@ static int batch_capture(bulkx_batch* b, BatchGraph& G) {
     int kind = -1;
     for (int f = 0; f < FAM_COUNT; ++f)
       if (kp.func == reinterpret_cast<void*>(kPhase0[f])) kind = f;
+    if (kp.func == reinterpret_cast<void*>(batch_merge_partition_kernel)) kind = kGridMergePartition;
     for (int r = 0; r < kRbkFamilies; ++r) {
       if (kp.func == reinterpret_cast<void*>(kRbkUp[r])) kind = FAM_RBK_I32 + r;
       if (kp.func == reinterpret_cast<void*>(kRbkScan[r])) kind = grid_kind_rbk_scan(FAM_RBK_I32 + r);

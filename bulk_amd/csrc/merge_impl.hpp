@@ -242,13 +242,22 @@ template <class CMP, class K, class V, bool HAS_V, int BLOCK, int ITEMS>
 __device__ __forceinline__ void merge_two_tile(MergeSmem<K, V, HAS_V, BLOCK, ITEMS>& sm,
                                                const K* a, const V* va, int64_t n1, const K* b,
                                                const V* vb, int64_t n2, K* out, V* vout,
-                                               int64_t tile) {
+                                               int64_t tile, const int64_t* splits = nullptr) {
   constexpr int TILE = BLOCK * ITEMS;
   const int64_t n = n1 + n2;
   const int64_t d0 = tile * TILE;
   const int64_t d1 = d0 + TILE < n ? d0 + TILE : n;
-  tile_splits<CMP>(a, n1, b, n2, d0, d1, sm.split);
-  const int64_t a0 = sm.split[0], a1 = sm.split[1];
+  // the tile's two boundaries: precomputed (splits[tile], splits[tile + 1])
+  // or searched here
+  int64_t a0, a1;
+  if (splits) {
+    a0 = splits[tile];
+    a1 = splits[tile + 1];
+  } else {
+    tile_splits<CMP>(a, n1, b, n2, d0, d1, sm.split);
+    a0 = sm.split[0];
+    a1 = sm.split[1];
+  }
   const int64_t b0 = d0 - a0, b1 = d1 - a1;
   merge_tile<CMP, K, V, HAS_V, BLOCK, ITEMS>(a + a0, HAS_V ? va + a0 : nullptr, int(a1 - a0),
                                              b + b0, HAS_V ? vb + b0 : nullptr, int(b1 - b0),
