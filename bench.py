@@ -57,13 +57,6 @@ def _pct(xs, q):
     return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if len(xs) else None
 
 
-def _sort_passes(n, tile=4096):
-    t = (n + tile - 1) // tile
-    p = 0
-    while (1 << p) < t:
-        p += 1
-    return p
-
 
 def _read_traffic(path, kernel):
     """HBM bytes per launch of `kernel` from a committed rocprofv3 --pmc
@@ -245,9 +238,9 @@ def main():
             stage_bytes["merge_by_key"] += j.algorithmic_bytes()
         else:
             stage_bytes["sort_block"] += j.algorithmic_bytes()
-            stage_bytes["sort_rounds"] += _sort_passes(n) * 16 * n
+            stage_bytes["sort_rounds"] += Batch.sort_rounds(n) * 16 * n
     stage_bytes["sort_pass_kernels"] = stage_bytes["sort_rounds"]
-    max_rounds = max([_sort_passes(j.spec.n) for j in jobs if j.spec.kind == C.JOB_SORT_BY_KEY] + [0])
+    max_rounds = max([Batch.sort_rounds(j.spec.n) for j in jobs if j.spec.kind == C.JOB_SORT_BY_KEY] + [0])
     stages = {k: {"ms": acc[k], "bytes": stage_bytes[k],
                   "gbs": stage_bytes[k] / (acc[k] * 1e-3) / 1e9 if acc[k] > 0 else None}
               for k in Batch.STAGES}

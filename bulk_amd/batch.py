@@ -97,6 +97,18 @@ class Batch:
               "reduce_by_key_i64", "merge_by_key", "sort_block", "sort_rounds",
               "sort_pass_kernels", "sort_partition_kernels")
 
+    @staticmethod
+    def sort_rounds(n: int) -> int:
+        """Merge rounds the batch runs for a sort job of n elements."""
+        run, tile = ctypes.c_int64(), ctypes.c_int64()
+        C.check(C.lib().bulkx_batch_sort_geometry(ctypes.byref(run), ctypes.byref(tile)),
+                "bulkx_batch_sort_geometry")
+        runs = (n + run.value - 1) // run.value
+        p = 0
+        while (1 << p) < runs:
+            p += 1
+        return p
+
     def stage_timing(self) -> dict[str, float]:
         """Per-stage ms of the last eager launch (HIP events on the batch's stream)."""
         arr = (ctypes.c_float * len(self.STAGES))()

@@ -283,6 +283,12 @@ int bulkx_batch_last_timing(bulkx_batch* batch, float* phase0_ms, float* passes_
  * pass kernels, [9] the partition kernels.  Writes num_stages entries
  * (zeros beyond 10). */
 int bulkx_batch_stage_timing(bulkx_batch* batch, float* stage_ms, int num_stages);
+/* Tile geometry of the batch's sort jobs: the block sort writes sorted runs
+ * of *run_elements; each merge round then doubles them in tiles of
+ * *merge_tile outputs, so a sort job of n elements takes
+ * ceil(log2(ceil(n / run_elements))) rounds.  No counterpart in the
+ * reference (its merge_sort_by_key.cu:163-171 block size is fixed). */
+int bulkx_batch_sort_geometry(int64_t* run_elements, int64_t* merge_tile);
 
 /* ------------------------------------------------------------------------
  * Serving: a request loop that admits jobs under an HBM budget, keeps their
