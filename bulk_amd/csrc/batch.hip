@@ -34,10 +34,12 @@ constexpr int kBatchMergeItems = 8;   // 4096 outputs per merge / sort tile
 constexpr int64_t kBatchScanTile = int64_t(kBatchBlock) * kBatchScanItems;
 constexpr int64_t kBatchRbkTile = int64_t(kBatchBlock) * kBatchRbkItems;
 constexpr int64_t kBatchMergeTile = int64_t(kBatchBlock) * kBatchMergeItems;
-// The sort family's block sort writes runs of 16384 elements (1024 x 16 in
-// LDS): two merge rounds fewer than runs of one merge tile, for ~12 % less
-// sort time at 2^22 elements on MI355X.
-constexpr int kBatchSortBlock = 1024;
+// The sort family's block sort writes runs of 8192 elements (512 x 16 in
+// LDS, two workgroups per CU): one merge round fewer than runs of one merge
+// tile.  Measured on the mixed stream: block sort + rounds 7.9 ms with these
+// runs, 8.5 ms with 4096-element runs, 8.1 ms with 16384 (1024 x 16, one
+// workgroup per CU).
+constexpr int kBatchSortBlock = 512;
 constexpr int kBatchSortItems = 16;
 constexpr int64_t kBatchSortRun = int64_t(kBatchSortBlock) * kBatchSortItems;
 static_assert(kBatchSortRun % kBatchMergeTile == 0, "runs are whole merge tiles");
