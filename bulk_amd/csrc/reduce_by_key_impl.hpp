@@ -243,9 +243,9 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
   for (int i = 0; i < ITEMS; ++i) h |= uint32_t(!(k[i] == (i ? k[i - 1] : kprev))) << i;
   uint32_t t = (h >> 1) | (uint32_t(!(k[ITEMS - 1] == knext)) << (ITEMS - 1));
   if (chunk_base == 0 && lane == 0) h |= 1u;  // the array's first element
-  if (valid > 0 && valid < ITEMS) t |= 1u << (valid - 1);  // last element of a partial chunk
-  if (valid == ITEMS && count == N && chunk_base + N == n && lane == kWave - 1)
-    t |= 1u << (ITEMS - 1);  // last element of a full last chunk
+  // The array's last element ends a run wherever it falls in the lane (the
+  // element after it in the lane's slice is fill, which may equal its key).
+  if (valid > 0 && chunk_base + int64_t(lane) * ITEMS + valid == n) t |= 1u << (valid - 1);
   h &= vmask;
   t &= vmask;
 

@@ -152,3 +152,48 @@ def test_batch_rejects_over_capacity(bx, orc):
     with pytest.raises(BulkError):
         b.set_jobs([B.scan_job(x[:10], x[:10])] * 3)
     b.close()
+
+
+@pytest.mark.parametrize("mode", ["graph", "eager"])
+def test_batch_reduce_by_key_long_runs(bx, orc, mode):
+    """Runs longer than a tile (tiles with no head carry the open run through
+    the aggregates), a single run over a whole job, and float values within
+    the bound, in one batch."""
+    from bulk_amd import batch as B
+    jobs, checks = [], []
+    for i, (n, run) in enumerate([(100003, 20000), (70000, 70000), (300001, 9000), (5000, 1)]):
+        keys = (np.arange(n) // run).astype(np.int32)
+        vals = orc.gen_uniform_i64(40 + i, -(1 << 20), 1 << 20, n)
+        dk, dv = _dev(keys), _dev(vals)
+        ko = torch.empty(n, dtype=torch.int32, device="cuda")
+        vo = torch.empty(n, dtype=torch.int64, device="cuda")
+        cnt = torch.full((1,), -1, dtype=torch.int64, device="cuda")
+        jobs.append(B.reduce_by_key_job(dk, dv, ko, vo, cnt))
+        rk, rv = orc.reduce_by_key(keys, vals, textbook=True)
+        checks.append(lambda ko=ko, vo=vo, cnt=cnt, rk=rk, rv=rv:
+                      int(cnt.item()) == rk.size and np.array_equal(ko.cpu().numpy()[:rk.size], rk)
+                      and np.array_equal(vo.cpu().numpy()[:rk.size], rv))
+        f = orc.gen_uniform_f32(50 + i, n, -1.0, 1.0)
+        fo = torch.empty(n, dtype=torch.float32, device="cuda")
+        cnt2 = torch.full((1,), -1, dtype=torch.int64, device="cuda")
+        ko2 = torch.empty(n, dtype=torch.int32, device="cuda")
+        jobs.append(B.reduce_by_key_job(dk, _dev(f), ko2, fo, cnt2))
+        f64 = f.astype(np.float64)
+        starts = np.flatnonzero(np.r_[True, keys[1:] != keys[:-1]])
+        exact = np.add.reduceat(f64, starts)
+        lens = np.diff(np.r_[starts, n])
+        # 4 * ceil(log2 len) * 2^-24 * sum|x| per run
+        bound = 4 * np.maximum(1, np.ceil(np.log2(np.maximum(lens, 2)))) * 2.0 ** -24 * \
+            np.add.reduceat(np.abs(f64), starts)
+        checks.append(lambda fo=fo, cnt2=cnt2, exact=exact, bound=bound:
+                      int(cnt2.item()) == exact.size and
+                      bool(np.all(np.abs(fo.cpu().numpy()[:exact.size].astype(np.float64) - exact) <= bound)))
+    b = B.Batch(max_jobs=16, max_elements=1 << 21)
+    b.set_jobs(jobs)
+    print(f"batch long runs mode={mode}", flush=True)
+    for rep in range(2):
+        b.launch(use_graph=(mode == "graph"))
+        torch.cuda.synchronize()
+        bad = [k for k, c in enumerate(checks) if not c()]
+        assert bad == [], f"replay {rep}: jobs {bad} differ"
+    b.close()

@@ -133,3 +133,20 @@ def test_reduce_by_key_variants(force, orc, v):
         bound = 4 * np.maximum(1, np.ceil(np.log2(np.maximum(lens, 2)))) * 2.0 ** -24 * absum
         assert m == starts.size
         assert np.all(np.abs(vo[:m].cpu().numpy().astype(np.float64) - exact) <= bound), n
+
+
+@pytest.mark.parametrize("v", list(range(9)))
+def test_reduce_by_key_last_run_key_zero(force, orc, v):
+    """A run ending the array whose key equals the loads' fill value (0),
+    with the array's end on a lane boundary inside a partial chunk and
+    elsewhere: the last run is still written and counted."""
+    from bulk_amd import api
+    for code in (C.INT32, C.INT64, C.FLOAT32):
+        force(C.ALGO_REDUCE_BY_KEY, code, v)
+    for n in (16, 100, 70000, 16 * 1000, 4096 * 3 + 16, 8192 + 32, 8192 * 5 + 16 * 23):
+        for keys in (np.zeros(n, np.int32), np.r_[np.full(n // 2, 5), np.zeros(n - n // 2)].astype(np.int32)):
+            vals = orc.gen_uniform_i64(n, -(1 << 20), 1 << 20, n)
+            ko, vo, m = api.my_reduce_by_key(_dev(keys), _dev(vals))
+            rk, rv = orc.reduce_by_key(keys, vals, textbook=True)
+            assert m == rk.size and np.array_equal(ko[:m].cpu().numpy(), rk) and \
+                np.array_equal(vo[:m].cpu().numpy(), rv), (n, v)
