@@ -92,6 +92,7 @@ SIGNATURES = {
     "bulkx_batch_last_timing": (_I, [_P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
     "bulkx_batch_stage_timing": (_I, [_P, ctypes.POINTER(ctypes.c_float), _I]),
     "bulkx_batch_sort_geometry": (_I, [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
+    "bulkx_batch_prepare_graphs": (_I, [_P]),
     "bulkx_server_create": (_I, [_L, _I, _I, _L, _I, ctypes.POINTER(_P)]),
     "bulkx_server_destroy": (_I, [_P]),
     "bulkx_server_submit": (_I, [_P, ctypes.POINTER(Job), _L, ctypes.POINTER(_L)]),

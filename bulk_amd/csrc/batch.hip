@@ -1106,6 +1106,19 @@ extern "C" int bulkx_batch_last_timing(bulkx_batch* b, float* phase0_ms, float* 
   return BULKX_SUCCESS;
 }
 
+extern "C" int bulkx_batch_prepare_graphs(bulkx_batch* b) {
+  if (!b) return BULKX_ERR_INVALID_ARGUMENT;
+  const int keep = b->rounds;
+  int st = BULKX_SUCCESS;
+  for (int r = 0; r <= b->max_passes && st == BULKX_SUCCESS; ++r) {
+    if (b->graphs[r].exec) continue;
+    b->rounds = r;
+    st = batch_capture(b, b->graphs[r]);
+  }
+  b->rounds = keep;
+  return st;
+}
+
 extern "C" int bulkx_batch_sort_geometry(int64_t* run_elements, int64_t* merge_tile) {
   if (!run_elements || !merge_tile) return BULKX_ERR_INVALID_ARGUMENT;
   *run_elements = kBatchSortRun;

@@ -177,6 +177,9 @@ extern "C" int bulkx_server_create(int64_t hbm_budget_bytes, int max_inflight, i
     int st = e == hipSuccess ? BULKX_SUCCESS : BULKX_ERR_HIP;
     if (st == BULKX_SUCCESS)
       st = bulkx_batch_create(max_batch_jobs, max_batch_elements, sl.stream, &sl.batch);
+    // graphs of every merge-round count up front: no request waits on a capture
+    if (st == BULKX_SUCCESS && !(flags & (BULKX_SERVE_PER_CALL | BULKX_SERVE_NO_GRAPH)))
+      st = bulkx_batch_prepare_graphs(sl.batch);
     if (st != BULKX_SUCCESS) {
       if (e != hipSuccess) set_last_error("server slot", e);
       bulkx_server_destroy(s);

@@ -271,6 +271,11 @@ int bulkx_batch_set_jobs(bulkx_batch* batch, const bulkx_job* jobs, int num_jobs
 int bulkx_batch_launch(bulkx_batch* batch, int use_graph);
 /* Run each job as its own plain call on the batch's stream (the per-call baseline). */
 int bulkx_batch_launch_per_call(bulkx_batch* batch);
+/* Record ahead of time the graph of every number of merge rounds the batch
+ * can need (one graph per count is recorded on first use otherwise), so no
+ * later graph launch pays a capture.  No counterpart in the reference; for
+ * latency-sensitive callers (the server calls it for each of its slots). */
+int bulkx_batch_prepare_graphs(bulkx_batch* batch);
 /* Device time in ms of the phases of the last eager launch (use_graph = 0):
  * the persistent first-phase kernels (one per op family) and all sort merge
  * rounds together.  Waits for that launch to finish.  BULKX_ERR_NOT_READY if

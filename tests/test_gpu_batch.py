@@ -120,6 +120,8 @@ def test_batch_graph_replays_and_new_job_lists(bx, orc):
     job list uploaded before the last replays, all match the oracle."""
     from bulk_amd.batch import Batch
     b = Batch(max_jobs=64, max_elements=1 << 23)
+    # every round count's graph recorded up front; launches then only replay
+    C.check(C.lib().bulkx_batch_prepare_graphs(b._h), "bulkx_batch_prepare_graphs")
     jobs, checks = _make_jobs(orc, SIZES, seed=2)
     b.set_jobs(jobs)
     for rep in range(3):
