@@ -22,6 +22,7 @@ import torch
 
 from . import _capi as C
 from . import api
+from . import opbench
 
 SIZES = {0: 1 << 14, 1: 1 << 18, 2: 1 << 22, 3: 1 << 26}
 DTYPES = {C.INT32: torch.int32, C.FLOAT32: torch.float32, C.INT64: torch.int64,
@@ -103,7 +104,8 @@ def _cases(g):
                 k.copy_(k0)
                 api.stable_merge_sort_by_key(k, v, workspace=ws)
             yield C.ALGO_SORT_BY_KEY, code, sc, srt
-        keys = (torch.arange(n, device=dev) // 8).to(torch.int32)
+        # runs of geometric length, mean 8 (the single-op benchmark's keys)
+        keys = opbench._runs(n, "geometric", g, torch.device(dev))
         for code in (C.INT32, C.FLOAT32, C.INT64):
             dt = DTYPES[code]
             vals = torch.ones(n, dtype=dt, device=dev)
