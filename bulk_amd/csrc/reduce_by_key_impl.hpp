@@ -149,7 +149,8 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
                                          int64_t* num_runs, int64_t tile, uint64_t* words,
                                          uint32_t* err, bool single_tile,
                                          const Seg<typename Acc<V, Op>::type>* prefixes = nullptr,
-                                         const Seg<typename Acc<V, Op>::type>* group_prefixes = nullptr) {
+                                         const Seg<typename Acc<V, Op>::type>* group_prefixes = nullptr,
+                                         const uint8_t* skip = nullptr, int64_t skip_tiles = 0) {
   using AccP = Acc<V, Op>;
   using A = typename AccP::type;
   using AOp = typename AccP::op;
@@ -183,6 +184,11 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
 #pragma unroll
     for (int j = 0; j < WK::NV; ++j) tk[j] = __builtin_nontemporal_load(q + j * kWave + lane);
   }
+  // Three-pass form with `skip` (the upsweep's flags): a tile in which no
+  // run ends writes nothing, so it stops here, while its key loads are in
+  // flight and before its values are read.
+  if (skip != nullptr && tile + 1 < skip_tiles && !(skip[tile] & 2u) && !(skip[tile + 1] & 1u))
+    return;
   if (fullv) {
     const u32x4* q = reinterpret_cast<const u32x4*>(vals + cb);
 #pragma unroll
@@ -500,22 +506,40 @@ __device__ __forceinline__ void rbk_upsweep_tile(const K* __restrict__ keys,
   // fold of values [from, cnt): each thread folds its elements of that range
   // in element order, then lanes and waves combine in a fixed order
   A part = AOp::template identity<A>();
-  const int spec0 = ((NV - 1) * BLOCK + int(threadIdx.x)) * VE;
-  if (spec && from <= spec0) {
-    // this thread's whole last vector is in range; earlier ones only if from is
+  // Values of the key vectors that reach [from, cnt): in a full aligned tile
+  // as whole 16-byte vectors striped like the keys (the last wave's last
+  // vector is already loaded), all in flight before the fold; runs longer
+  // than a tile make this the whole tile's values.
+  const bool vec = kSpecOk && full && aligned16(vals + base);
+  if (vec) {
+    V w[NV][VE];
 #pragma unroll
-    for (int j = 0; j < NV - 1; ++j)
+    for (int j = 0; j < NV; ++j) {
+      const int e0 = (j * BLOCK + int(threadIdx.x)) * VE;
+      if (j == NV - 1 && spec) {
+#pragma unroll
+        for (int c = 0; c < VE; ++c) w[j][c] = sv[c];
+      } else if (e0 + VE > from) {
+        const u32x4* q = reinterpret_cast<const u32x4*>(vals + base + e0);
+#pragma unroll
+        for (int u = 0; u < SV; ++u) {
+          const u32x4 t = __builtin_nontemporal_load(q + u);
+#pragma unroll
+          for (int c = 0; c < VE / SV; ++c)
+            w[j][u * (VE / SV) + c] = reinterpret_cast<const V*>(&t)[c];
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < VE; ++c) w[j][c] = V();  // before the range: not folded
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
 #pragma unroll
       for (int c = 0; c < VE; ++c) {
         const int e = (j * BLOCK + int(threadIdx.x)) * VE + c;
-        if (e >= from) part = AOp::apply(part, AccP::from(vals[base + e]));
+        if (e >= from) part = AOp::apply(part, AccP::from(w[j][c]));
       }
-#pragma unroll
-    for (int c = 0; c < VE; ++c) part = AOp::apply(part, AccP::from(sv[c]));
-  } else if (spec) {
-#pragma unroll
-    for (int c = 0; c < VE; ++c)
-      if (spec0 + c >= from) part = AOp::apply(part, AccP::from(sv[c]));
   } else {
 #pragma unroll
     for (int j = 0; j < NV; ++j)
@@ -625,14 +649,16 @@ __global__ __launch_bounds__(BLOCK) void rbk_downsweep_kernel(
   // tiles through the aggregates): with 8-byte values it is skipped without
   // reading its keys or values, which runs spanning many tiles (Zipf-like
   // lengths) make the common case (measured at 2^28: -27 % time for Zipf
-  // runs, unchanged for runs of ~8).  With 4-byte values the flag read
-  // ahead of the tile's loads cost 9 % on runs of ~8, so it is not done.
-  // The last tile always holds a run's end.
+  // runs, unchanged for runs of ~8).  With 4-byte values a flag read ahead
+  // of the tile's loads cost 9 % on runs of ~8, so the tile reads its flags
+  // behind its key loads and skips only its values.  The last tile always
+  // holds a run's end.
   if constexpr (sizeof(V) >= 8) {
     if (tile + 1 < tiles && !(flags[tile] & 2u) && !(flags[tile + 1] & 1u)) return;
   }
   rbk_tile<K, V, Op, BLOCK, ITEMS>(sm, keys, vals, n, kout, vout, num_runs, tile, nullptr,
-                                   nullptr, false, prefixes, group_prefixes);
+                                   nullptr, false, prefixes, group_prefixes,
+                                   sizeof(V) >= 8 ? nullptr : flags, tiles);
 }
 
 __global__ void rbk_empty_kernel(int64_t* num_runs);
