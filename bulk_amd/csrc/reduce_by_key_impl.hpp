@@ -166,6 +166,13 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
   const TileStatus<S> st{words};
   const int wid = threadIdx.x / kWave;
   const int lane = lane_id();
+  // Three-pass form: prefix of the tile's scan segment, then the tile's
+  // exclusive prefix inside it, read before the tile's loads so that the
+  // round trip overlaps them.
+  S known = SOp::template identity<S>();
+  if (prefixes)
+    known = group_prefixes ? SOp::apply(group_prefixes[tile / rbk_scan_seg<S>()], prefixes[tile])
+                           : prefixes[tile];
   const int64_t chunk_base = tile * TILE + int64_t(wid) * N;
   const int64_t rem = n - chunk_base;
   const int count = rem <= 0 ? 0 : (rem >= N ? N : int(rem));
@@ -305,57 +312,54 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
   // consecutive slots from base (the run open at the chunk's first element),
   // so slot - base < 64*ITEMS is a 32-bit stage index.  Slots are kept
   // relative to the tile (base_rel) until the tile's prefix arrives.  The
-  // walk stages the output keys and leaves each tail's output value in v[i].
+  // walk stages the output keys and leaves each element's fold since its
+  // run's head (the output value, for a tail) in v[i].  It has no branches:
+  // an element that stages nothing writes to a spare slot past the N staged
+  // ones (one per lane, or lane pair), and the lane's one tail that may
+  // precede its first head -- it closes the run carried into the lane, whose
+  // value needs the prefix -- is patched once after the walk.
   const S q = SOp::apply(wave_prefix, lift(lane_prefix));  // in-tile prefix of the lane
   const int64_t base_rel = readlane(int64_t(q.c) + int64_t(h & 1u) - 1, 0);
   const uint32_t ntails = wave_reduce<OpPlus>(uint32_t(__builtin_popcount(t)));
   K* sk = lk;
   V* sv = lv;
+  constexpr uint32_t KSPARE = uint32_t(RbkLds<K, V, ITEMS>::BYTES / int(sizeof(K)) - N);
+  constexpr uint32_t VSPARE = uint32_t(RbkLds<K, V, ITEMS>::BYTES / int(sizeof(V)) - N);
+  static_assert(KSPARE >= 1 && VSPARE >= 1, "the region has spare slots past the stage");
+  const uint32_t kspare = uint32_t(N) + uint32_t(lane) % KSPARE;
+  const uint32_t vspare = uint32_t(N) + uint32_t(lane) % VSPARE;
   const uint32_t e0 = uint32_t(int64_t(q.c) - base_rel);
   uint32_t e = e0;
   V local = Op::template identity<V>();
   bool started = false;  // a run began inside this lane's slice
-  bool fix = false;      // this lane holds the tail of the run carried into the tile
-  int fix_i = 0;
-  A fix_local = AOp::template identity<A>();
+  bool pre = false;      // a tail before the lane's first head (its stage slot is e0 - 1)
+  V pre_local = Op::template identity<V>();
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     const bool hi = (h >> i) & 1u;
+    const bool ti = (t >> i) & 1u;
     e += hi;
     local = hi ? v[i] : Op::apply(local, v[i]);
     started |= hi;
-    if ((t >> i) & 1u) {
-      const uint32_t slot = e - 1;
-      if (slot < uint32_t(N)) sk[slot] = k[i];
-      if (started) {
-        v[i] = local;
-      } else if (q.c > 0) {
-        v[i] = AccP::to(AOp::apply(q.v, AccP::from(local)));
-      } else {
-        fix = true;
-        fix_i = i;
-        fix_local = AccP::from(local);
-      }
-    }
+    const uint32_t slot = e - 1;
+    sk[ti && slot < uint32_t(N) ? slot : kspare] = k[i];
+    const bool p = ti && !started;
+    pre_local = p ? local : pre_local;
+    pre |= p;
+    v[i] = local;
   }
   __syncthreads();  // the tile's prefix is in sm.prefix
-  // three-pass form: prefix of the tile's scan segment, then the tile's
-  // exclusive prefix inside it
-  const S tile_prefix =
-      prefixes ? (group_prefixes
-                      ? SOp::apply(group_prefixes[tile / rbk_scan_seg<S>()], prefixes[tile])
-                      : prefixes[tile])
-               : chained ? sm.prefix : SOp::template identity<S>();
+  const S tile_prefix = prefixes ? known : chained ? sm.prefix : SOp::template identity<S>();
   // The tile holding element n-1 reports the run count.
   if (threadIdx.x == 0 && tile * TILE + TILE >= n)
     *num_runs = int64_t(tile_prefix.c + tile_agg.c);
-  // same association as folding the full prefix first: (tile_prefix . q) . local
-  if (fix) {
-    const V fixed = AccP::to(AOp::apply(AOp::apply(tile_prefix.v, q.v), fix_local));
-#pragma unroll
-    for (int i = 0; i < ITEMS; ++i)
-      if (i == fix_i) v[i] = fixed;
-  }
+  // The pre-head tail's value: the lane's in-tile prefix folded in, and the
+  // tile's prefix too when no head precedes the lane in the tile (same
+  // association as folding the full prefix first: (tile_prefix . q) . local).
+  V pre_val = Op::template identity<V>();
+  if (pre)
+    pre_val = q.c > 0 ? AccP::to(AOp::apply(q.v, AccP::from(pre_local)))
+                      : AccP::to(AOp::apply(AOp::apply(tile_prefix.v, q.v), AccP::from(pre_local)));
   const int64_t base = int64_t(tile_prefix.c) + base_rel;
   // bounds guard: slots stay below n when the scan is consistent
   uint32_t cnt = ntails < uint32_t(N) ? ntails : uint32_t(N);
@@ -367,11 +371,11 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     e += (h >> i) & 1u;
-    if ((t >> i) & 1u) {
-      const uint32_t slot = e - 1;
-      if (slot < uint32_t(N)) sv[slot] = v[i];
-    }
+    const uint32_t slot = e - 1;
+    sv[((t >> i) & 1u) && slot < uint32_t(N) ? slot : vspare] = v[i];
   }
+  // (e0 - 1 wraps to a huge value when e0 is 0, so the guard holds)
+  if (pre && e0 - 1 < uint32_t(N)) sv[e0 - 1] = pre_val;
   wave_lds_sync();
   for (uint32_t x = lane; x < cnt; x += kWave) vout[base + x] = sv[x];
 }
