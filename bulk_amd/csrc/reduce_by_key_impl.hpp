@@ -266,11 +266,17 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
   // Lane aggregate (count of heads, value since the last head), DPP scan
   // across the wave, LDS combine across waves.  Values past the end of the
   // array only extend the last run's aggregate, which nothing reads.
+  // The fold since each element's run head replaces v[i] (for a tail it is
+  // the run's output value, once the lane's prefix is folded in where no
+  // head precedes it in the lane).
   RS a;
   a.c = uint32_t(__builtin_popcount(h));
   a.v = Op::template identity<V>();
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i) a.v = (h >> i) & 1u ? v[i] : Op::apply(a.v, v[i]);
+  for (int i = 0; i < ITEMS; ++i) {
+    a.v = (h >> i) & 1u ? v[i] : Op::apply(a.v, v[i]);
+    v[i] = a.v;
+  }
   const RS inc = wave_inclusive_scan<RSOp>(a);
   const RS lane_prefix = wave_shift_right<RSOp>(inc);
   const RS wave_total = readlane(inc, kWave - 1);
@@ -312,12 +318,12 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
   // consecutive slots from base (the run open at the chunk's first element),
   // so slot - base < 64*ITEMS is a 32-bit stage index.  Slots are kept
   // relative to the tile (base_rel) until the tile's prefix arrives.  The
-  // walk stages the output keys and leaves each element's fold since its
-  // run's head (the output value, for a tail) in v[i].  It has no branches:
-  // an element that stages nothing writes to a spare slot past the N staged
-  // ones (one per lane, or lane pair), and the lane's one tail that may
-  // precede its first head -- it closes the run carried into the lane, whose
-  // value needs the prefix -- is patched once after the walk.
+  // walk stages the output keys.  It has no branches and no chain: a tail's
+  // slot is sbase + (heads up to it in the lane), masked into the stage; an
+  // element that is no tail writes to a spare slot past the N staged ones
+  // (one per lane, or lane pair).  The lane's one tail that may precede its
+  // first head closes the run carried into the lane; its value needs the
+  // prefix and is patched once, after the walk.
   const S q = SOp::apply(wave_prefix, lift(lane_prefix));  // in-tile prefix of the lane
   const int64_t base_rel = readlane(int64_t(q.c) + int64_t(h & 1u) - 1, 0);
   const uint32_t ntails = wave_reduce<OpPlus>(uint32_t(__builtin_popcount(t)));
@@ -326,27 +332,22 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
   constexpr uint32_t KSPARE = uint32_t(RbkLds<K, V, ITEMS>::BYTES / int(sizeof(K)) - N);
   constexpr uint32_t VSPARE = uint32_t(RbkLds<K, V, ITEMS>::BYTES / int(sizeof(V)) - N);
   static_assert(KSPARE >= 1 && VSPARE >= 1, "the region has spare slots past the stage");
+  static_assert((N & (N - 1)) == 0, "the stage index is masked with N - 1");
   const uint32_t kspare = uint32_t(N) + uint32_t(lane) % KSPARE;
   const uint32_t vspare = uint32_t(N) + uint32_t(lane) % VSPARE;
   const uint32_t e0 = uint32_t(int64_t(q.c) - base_rel);
-  uint32_t e = e0;
-  V local = Op::template identity<V>();
-  bool started = false;  // a run began inside this lane's slice
-  bool pre = false;      // a tail before the lane's first head (its stage slot is e0 - 1)
+  const uint32_t sbase = e0 - 1;  // slot of a tail with no head before it in the lane
+  bool pre = false;               // the lane has such a tail
   V pre_local = Op::template identity<V>();
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
-    const bool hi = (h >> i) & 1u;
+    const uint32_t upto = h & (i == 31 ? ~0u : ((2u << i) - 1u));  // heads up to element i
+    const uint32_t slot = (uint32_t(__builtin_popcount(upto)) + sbase) & uint32_t(N - 1);
     const bool ti = (t >> i) & 1u;
-    e += hi;
-    local = hi ? v[i] : Op::apply(local, v[i]);
-    started |= hi;
-    const uint32_t slot = e - 1;
-    sk[ti && slot < uint32_t(N) ? slot : kspare] = k[i];
-    const bool p = ti && !started;
-    pre_local = p ? local : pre_local;
+    sk[ti ? slot : kspare] = k[i];
+    const bool p = ti && upto == 0;
+    pre_local = p ? v[i] : pre_local;
     pre |= p;
-    v[i] = local;
   }
   __syncthreads();  // the tile's prefix is in sm.prefix
   const S tile_prefix = prefixes ? known : chained ? sm.prefix : SOp::template identity<S>();
@@ -367,15 +368,13 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
   else if (base + int64_t(cnt) > n) cnt = uint32_t(n - base);
   for (uint32_t x = lane; x < cnt; x += kWave) kout[base + x] = sk[x];
   wave_lds_sync();  // key stage read before the value stage overwrites it
-  e = e0;
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
-    e += (h >> i) & 1u;
-    const uint32_t slot = e - 1;
-    sv[((t >> i) & 1u) && slot < uint32_t(N) ? slot : vspare] = v[i];
+    const uint32_t upto = h & (i == 31 ? ~0u : ((2u << i) - 1u));
+    const uint32_t slot = (uint32_t(__builtin_popcount(upto)) + sbase) & uint32_t(N - 1);
+    sv[(t >> i) & 1u ? slot : vspare] = v[i];
   }
-  // (e0 - 1 wraps to a huge value when e0 is 0, so the guard holds)
-  if (pre && e0 - 1 < uint32_t(N)) sv[e0 - 1] = pre_val;
+  if (pre) sv[sbase & uint32_t(N - 1)] = pre_val;
   wave_lds_sync();
   for (uint32_t x = lane; x < cnt; x += kWave) vout[base + x] = sv[x];
 }
