@@ -53,15 +53,18 @@ def stats(d: str, command: str | None, top: int) -> str:
                    f"{float(r['TotalDurationNs']) / 1e6:.2f} | {float(r['Percentage']):.2f} |")
     hits = glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True)
     if hits:
-        disp = [(int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"])
+        disp = [(int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"],
+                 int(r.get("Grid_Size_X") or r.get("Grid_Size") or 0))
                 for r in csv.DictReader(open(hits[0]))]
         disp.sort()
         alone = collections.defaultdict(list)
+        by_grid = collections.defaultdict(lambda: collections.defaultdict(list))
         reach = -1  # latest end among the dispatches before i
-        for i, (s, e, n) in enumerate(disp):
+        for i, (s, e, n, g) in enumerate(disp):
             nxt = disp[i + 1][0] if i + 1 < len(disp) else None
             if s >= reach and (nxt is None or nxt >= e) and "bulkx" in n:
                 alone[_short(n)].append(e - s)
+                by_grid[_short(n)][g].append(e - s)
             reach = max(reach, e)
         if alone:
             out.append("")
@@ -71,6 +74,25 @@ def stats(d: str, command: str | None, top: int) -> str:
             out.append("|---|---:|---:|")
             for n, v in sorted(alone.items(), key=lambda kv: -sum(kv[1])):
                 out.append(f"| {n} | {len(v)} | {sum(v) / len(v) / 1e3:.1f} |")
+        # A kernel launched with several grid sizes (one per merge round)
+        # does different work per launch: its plain average depends on which
+        # launches ran alone, so the average per grid size and their mean
+        # (one launch of each size, as a step makes them) are listed too.
+        for n, grids in sorted(by_grid.items(), key=lambda kv: -sum(map(sum, kv[1].values()))):
+            if len(grids) < 2:
+                continue
+            out.append("")
+            out.append(f"{n}, dispatches that ran alone, by grid size (workitems):")
+            out.append("")
+            out.append("| grid | alone | avg us |")
+            out.append("|---:|---:|---:|")
+            means = []
+            for g in sorted(grids, reverse=True):
+                v = grids[g]
+                means.append(sum(v) / len(v))
+                out.append(f"| {g} | {len(v)} | {means[-1] / 1e3:.1f} |")
+            out.append("")
+            out.append(f"mean over the {len(means)} grid sizes: {sum(means) / len(means) / 1e3:.1f} us")
     return "\n".join(out) + "\n"
 
 
