@@ -115,14 +115,17 @@ def test_reduce_by_key_variants(force, orc, v):
         ko, vo, m = api.my_reduce_by_key(_dev(keys), _dev(f))
         rk, _ = orc.reduce_by_key(keys, f.astype(np.float64), textbook=True)
         assert m == rk.size and np.array_equal(ko[:m].cpu().numpy(), rk), n
-    # runs longer than a tile: tiles with no head carry the open run
+    # runs longer than a tile: tiles with no head carry the open run (and,
+    # in the three-pass form, tiles where no run ends are skipped: whole
+    # tiles with 8-byte values, the value loads with 4-byte ones)
     for n, run in ((300001, 50000), (1 << 20, 1 << 20)):
         keys = (np.arange(n) // run).astype(np.int32)
-        vals = orc.gen_uniform_i64(n + 7, -(1 << 20), 1 << 20, n)
-        ko, vo, m = api.my_reduce_by_key(_dev(keys), _dev(vals))
-        rk, rv = orc.reduce_by_key(keys, vals, textbook=True)
-        assert m == rk.size and np.array_equal(ko[:m].cpu().numpy(), rk) and \
-            np.array_equal(vo[:m].cpu().numpy(), rv), n
+        for vals in (orc.gen_uniform_i64(n + 7, -(1 << 20), 1 << 20, n),
+                     orc.gen_uniform_i32(n + 8, -(1 << 10), 1 << 10, n)):
+            ko, vo, m = api.my_reduce_by_key(_dev(keys), _dev(vals))
+            rk, rv = orc.reduce_by_key(keys, vals, textbook=True)
+            assert m == rk.size and np.array_equal(ko[:m].cpu().numpy(), rk) and \
+                np.array_equal(vo[:m].cpu().numpy(), rv), (n, vals.dtype)
         f = orc.gen_uniform_f32(n + 9, n, -1.0, 1.0).astype(np.float64)
         ko, vo, m = api.my_reduce_by_key(_dev(keys), _dev(f.astype(np.float32)))
         starts = np.flatnonzero(np.r_[True, keys[1:] != keys[:-1]])
