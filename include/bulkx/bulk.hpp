@@ -19,19 +19,27 @@
 //   * bulk::malloc(g, bytes)           -> a bump arena in the workgroup's dynamic LDS
 //                                         (heap_size bytes, up to 160 KB per CU on gfx950),
 //                                         falling back to device malloc when exhausted
-// Group algorithms (copy_n, reduce, inclusive_scan, exclusive_scan,
-// merge_path, merge) run on a concurrent_group with wavefront shuffles and
-// LDS; see bulk/algorithm.hpp of the reference for their contracts.
+// Group algorithms (copy_n, reduce, accumulate, inclusive_scan,
+// exclusive_scan, merge_path, merge, merge_by_key, inplace_merge,
+// stable_sort_by_key, reduce_by_key, scatter_if, gather) run on a
+// concurrent_group with wavefront shuffles and LDS; the same names on a
+// bounded agent (bulk::bound<N>(self)) are the sequential bodies of one
+// work-item.  head_flags / tail_flags are index-computed flag ranges.
+// Contracts: bulk/algorithm/*.hpp of the reference.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
 #include <cstdint>
+#include <iterator>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
 #include <utility>
+
+#include <thrust/pair.h>
+#include <thrust/tuple.h>
 
 namespace bulk {
 
@@ -737,6 +745,587 @@ __device__ It3 merge(concurrent_group<Agent, S>& g, It1 first1, It1 last1, It2 f
       result[d + k] = take_a ? first1[i++] : first2[j++];
     }
   }
+  g.wait();
+  return result + n;
+}
+
+// ------------------------------------------------------ bounded-agent bodies
+// The sequential bodies of one work-item whose work is statically bounded
+// (bulk::bound<N>(self)); each touches at most N elements.
+
+// inclusive_scan: init = op(init, x[i]); result[i] = init (scan.hpp:35-56).
+template <std::size_t B, std::size_t G, class It1, class It2, class T, class Op>
+__device__ It2 inclusive_scan(const bounded<B, agent<G>>& exec, It1 first, It1 last, It2 result,
+                              T init, Op op) {
+  const auto n = last - first;
+  for (int i = 0; i < exec.bound(); ++i) {
+    if (i < n) {
+      init = op(init, first[i]);
+      result[i] = init;
+    }
+  }
+  return result + n;
+}
+
+// exclusive_scan: result[i] = init; init = op(init, x[i]) (scan.hpp:59-78).
+template <std::size_t B, std::size_t G, class It1, class It2, class T, class Op>
+__device__ It2 exclusive_scan(const bounded<B, agent<G>>& exec, It1 first, It1 last, It2 result,
+                              T init, Op op) {
+  const auto n = last - first;
+  for (int i = 0; i < exec.bound(); ++i) {
+    if (i < n) {
+      result[i] = init;
+      init = op(init, first[i]);
+    }
+  }
+  return result + n;
+}
+
+// reduce / accumulate: left fold from init (reduce.hpp:33-58, accumulate.hpp:30-55).
+template <std::size_t B, std::size_t G, class It, class T, class Op>
+__device__ T accumulate(const bounded<B, agent<G>>& exec, It first, It last, T init, Op op) {
+  const auto n = last - first;
+  for (int i = 0; i < exec.bound(); ++i)
+    if (i < n) init = op(init, first[i]);
+  return init;
+}
+
+template <std::size_t B, std::size_t G, class It, class T, class Op>
+__device__ T reduce(const bounded<B, agent<G>>& exec, It first, It last, T init, Op op) {
+  return bulk::accumulate(exec, first, last, init, op);
+}
+
+// copy_n of at most the bound's elements.
+template <std::size_t B, std::size_t G, class It1, class Size, class It2>
+__device__ It2 copy_n(const bounded<B, agent<G>>& exec, It1 first, Size n, It2 result) {
+  for (int i = 0; i < exec.bound(); ++i)
+    if (Size(i) < n) result[i] = first[i];
+  return result + n;
+}
+
+// merge: the first min(bound, n1 + n2) outputs of the stable merge; a is
+// taken unless b's head is strictly less (merge.hpp:63-170).
+template <std::size_t B, std::size_t G, class It1, class It2, class It3, class Compare>
+__device__ It3 merge(const bounded<B, agent<G>>& exec, It1 first1, It1 last1, It2 first2,
+                     It2 last2, It3 result, Compare comp) {
+  const auto n1 = last1 - first1;
+  const auto n2 = last2 - first2;
+  decltype(n1 + n2) i = 0, j = 0, k = 0;
+  for (; k < exec.bound() && i + j < n1 + n2; ++k) {
+    const bool take_a = j >= n2 || (i < n1 && !comp(first2[j], first1[i]));
+    result[k] = take_a ? first1[i++] : first2[j++];
+  }
+  return result + k;
+}
+
+// merge_by_key: as merge, each key's value travelling with it (merge.hpp:173-305).
+template <std::size_t B, std::size_t G, class It1, class It2, class It3, class It4, class It5,
+          class It6, class Compare>
+__device__ thrust::pair<It5, It6> merge_by_key(const bounded<B, agent<G>>& exec, It1 keys_first1,
+                                               It1 keys_last1, It2 keys_first2, It2 keys_last2,
+                                               It3 values_first1, It4 values_first2,
+                                               It5 keys_result, It6 values_result, Compare comp) {
+  const auto n1 = keys_last1 - keys_first1;
+  const auto n2 = keys_last2 - keys_first2;
+  decltype(n1 + n2) i = 0, j = 0, k = 0;
+  for (; k < exec.bound() && i + j < n1 + n2; ++k) {
+    const bool take_a = j >= n2 || (i < n1 && !comp(keys_first2[j], keys_first1[i]));
+    if (take_a) {
+      keys_result[k] = keys_first1[i];
+      values_result[k] = values_first1[i++];
+    } else {
+      keys_result[k] = keys_first2[j];
+      values_result[k] = values_first2[j++];
+    }
+  }
+  return thrust::make_pair(keys_result + k, values_result + k);
+}
+
+// stable_sort_by_key / stable_sort: odd-even transposition over at most the
+// bound's elements; a pair swaps only when the later key is strictly less,
+// so equal keys keep their order (sort.hpp:33-149).
+template <std::size_t B, std::size_t G, class It1, class It2, class Compare>
+__device__ void stable_sort_by_key(const bounded<B, agent<G>>& exec, It1 keys_first, It1 keys_last,
+                                   It2 values_first, Compare comp) {
+  const int n = int(keys_last - keys_first);
+  for (int r = 0; r < exec.bound() && r < n; ++r) {
+    for (int j = r & 1; j + 1 < n; j += 2) {
+      if (comp(keys_first[j + 1], keys_first[j])) {
+        auto k = keys_first[j];
+        keys_first[j] = keys_first[j + 1];
+        keys_first[j + 1] = k;
+        auto v = values_first[j];
+        values_first[j] = values_first[j + 1];
+        values_first[j + 1] = v;
+      }
+    }
+  }
+}
+
+template <std::size_t B, std::size_t G, class It, class Compare>
+__device__ void stable_sort(const bounded<B, agent<G>>& exec, It first, It last, Compare comp) {
+  const int n = int(last - first);
+  for (int r = 0; r < exec.bound() && r < n; ++r) {
+    for (int j = r & 1; j + 1 < n; j += 2) {
+      if (comp(first[j + 1], first[j])) {
+        auto k = first[j];
+        first[j] = first[j + 1];
+        first[j + 1] = k;
+      }
+    }
+  }
+}
+
+// scatter_if: result[map[i]] = first[i] where stencil[i] (scatter.hpp:27-33).
+template <std::size_t B, std::size_t G, class It1, class It2, class It3, class It4>
+__device__ void scatter_if(const bounded<B, agent<G>>& exec, It1 first, It1 last, It2 map,
+                           It3 stencil, It4 result) {
+  const auto n = last - first;
+  for (int i = 0; i < exec.bound(); ++i)
+    if (i < n && stencil[i]) result[map[i]] = first[i];
+}
+
+// gather: result[i] = input[map[i]] (gather.hpp:31-64).
+template <std::size_t B, std::size_t G, class It1, class It2, class It3>
+__device__ It3 gather(const bounded<B, agent<G>>& exec, It1 map_first, It1 map_last,
+                      It2 input_first, It3 result) {
+  const auto n = map_last - map_first;
+  for (int i = 0; i < exec.bound(); ++i)
+    if (i < n) result[i] = input_first[map_first[i]];
+  return result + n;
+}
+
+// --------------------------------------------------------- flag ranges
+namespace detail {
+
+// A random-access iterator whose element d is f(i + d): the flag ranges
+// below compute each flag from the keys when it is read.
+template <class F, class Value, class Index>
+class index_iterator {
+ public:
+  typedef std::random_access_iterator_tag iterator_category;
+  typedef Value value_type;
+  typedef Index difference_type;
+  typedef const Value* pointer;
+  typedef Value reference;
+
+  __host__ __device__ index_iterator(F f = F(), Index i = 0) : m_f(f), m_i(i) {}
+  __host__ __device__ Value operator*() const { return m_f(m_i); }
+  __host__ __device__ Value operator[](Index d) const { return m_f(m_i + d); }
+  __host__ __device__ index_iterator operator+(Index d) const { return index_iterator(m_f, m_i + d); }
+  __host__ __device__ index_iterator operator-(Index d) const { return index_iterator(m_f, m_i - d); }
+  __host__ __device__ Index operator-(const index_iterator& o) const { return m_i - o.m_i; }
+  __host__ __device__ index_iterator& operator+=(Index d) { m_i += d; return *this; }
+  __host__ __device__ index_iterator& operator-=(Index d) { m_i -= d; return *this; }
+  __host__ __device__ index_iterator& operator++() { ++m_i; return *this; }
+  __host__ __device__ index_iterator& operator--() { --m_i; return *this; }
+  __host__ __device__ bool operator==(const index_iterator& o) const { return m_i == o.m_i; }
+  __host__ __device__ bool operator!=(const index_iterator& o) const { return m_i != o.m_i; }
+  __host__ __device__ bool operator<(const index_iterator& o) const { return m_i < o.m_i; }
+
+ private:
+  F m_f;
+  Index m_i;
+};
+
+struct equal_to_ {
+  template <class A, class B>
+  __host__ __device__ bool operator()(const A& a, const B& b) const { return a == b; }
+};
+
+}  // namespace detail
+
+// head_flags: flag i = (i == 0 || !pred(x[i], x[i-1])) (head_flags.hpp:118-196).
+template <class It, class Pred = detail::equal_to_, class ValueType = bool,
+          class IndexType = typename std::iterator_traits<It>::difference_type>
+class head_flags {
+  struct flag {
+    It first;
+    Pred pred;
+    __host__ __device__ ValueType operator()(IndexType i) const {
+      return ValueType(i == 0 || !pred(first[i], first[i - 1]));
+    }
+  };
+
+ public:
+  typedef detail::index_iterator<flag, ValueType, IndexType> iterator;
+  __host__ __device__ head_flags(It first, It last, Pred pred = Pred())
+      : m_begin(flag{first, pred}, 0), m_end(flag{first, pred}, IndexType(last - first)) {}
+  __host__ __device__ iterator begin() const { return m_begin; }
+  __host__ __device__ iterator end() const { return m_end; }
+  __host__ __device__ ValueType operator[](IndexType i) const { return m_begin[i]; }
+
+ private:
+  iterator m_begin, m_end;
+};
+
+// head_flags_with_init: flag 0 compares with init instead of being set:
+// flag 0 = !pred(init, x[0]) (head_flags.hpp:26-110, bulk/detail/head_flags.hpp:34-122).
+template <class It, class Pred = detail::equal_to_, class ValueType = bool,
+          class IndexType = typename std::iterator_traits<It>::difference_type>
+class head_flags_with_init {
+  typedef typename std::iterator_traits<It>::value_type init_type;
+  struct flag {
+    It first;
+    init_type init;
+    Pred pred;
+    __host__ __device__ ValueType operator()(IndexType i) const {
+      return ValueType(i == 0 ? !pred(init, first[0]) : !pred(first[i], first[i - 1]));
+    }
+  };
+
+ public:
+  typedef detail::index_iterator<flag, ValueType, IndexType> iterator;
+  __host__ __device__ head_flags_with_init(It first, It last, init_type init, Pred pred = Pred())
+      : m_begin(flag{first, init, pred}, 0),
+        m_end(flag{first, init, pred}, IndexType(last - first)) {}
+  __host__ __device__ iterator begin() const { return m_begin; }
+  __host__ __device__ iterator end() const { return m_end; }
+  __host__ __device__ ValueType operator[](IndexType i) const { return m_begin[i]; }
+
+ private:
+  iterator m_begin, m_end;
+};
+
+// tail_flags: flag i = (i == n-1 || !pred(x[i], x[i+1])) (tail_flags.hpp:26-109).
+template <class It, class Pred = detail::equal_to_, class ValueType = bool,
+          class IndexType = typename std::iterator_traits<It>::difference_type>
+class tail_flags {
+  struct flag {
+    It first;
+    IndexType n;
+    Pred pred;
+    __host__ __device__ ValueType operator()(IndexType i) const {
+      return ValueType(i == n - 1 || !pred(first[i], first[i + 1]));
+    }
+  };
+
+ public:
+  typedef detail::index_iterator<flag, ValueType, IndexType> iterator;
+  __host__ __device__ tail_flags(It first, It last, Pred pred = Pred())
+      : m_begin(flag{first, IndexType(last - first), pred}, 0),
+        m_end(flag{first, IndexType(last - first), pred}, IndexType(last - first)) {}
+  __host__ __device__ iterator begin() const { return m_begin; }
+  __host__ __device__ iterator end() const { return m_end; }
+  __host__ __device__ ValueType operator[](IndexType i) const { return m_begin[i]; }
+
+ private:
+  iterator m_begin, m_end;
+};
+
+template <class It>
+__host__ __device__ head_flags<It> make_head_flags(It first, It last) {
+  return head_flags<It>(first, last);
+}
+template <class It, class Pred>
+__host__ __device__ head_flags<It, Pred> make_head_flags(It first, It last, Pred pred) {
+  return head_flags<It, Pred>(first, last, pred);
+}
+template <class It>
+__host__ __device__ tail_flags<It> make_tail_flags(It first, It last) {
+  return tail_flags<It>(first, last);
+}
+template <class It, class Pred>
+__host__ __device__ tail_flags<It, Pred> make_tail_flags(It first, It last, Pred pred) {
+  return tail_flags<It, Pred>(first, last, pred);
+}
+
+// ------------------------------------------------- more group algorithms
+
+// accumulate: init op x[0] op ... op x[n-1] in order, so the op need only be
+// associative (accumulate.hpp:198-217; the reference reduces when the op is
+// commutative, which gives the same value for exact arithmetic).  Each agent
+// folds a blocked grain of each tile; the grains combine in agent order.
+// Every agent receives the result.
+template <class Agent, std::size_t S, class It, class T, class Op>
+__device__ T accumulate(concurrent_group<Agent, S>& g, It first, It last, T init, Op op) {
+  const auto n = last - first;
+  constexpr int G = Agent::static_grainsize;
+  const int tid = g.this_exec.index();
+  const auto tile = decltype(n + 0)(g.size()) * G;
+  struct P {
+    T v;
+    int has;
+  };
+  auto pop = [op](P a, P b) { return b.has ? (a.has ? P{op(a.v, b.v), 1} : b) : a; };
+  for (decltype(n + 0) base = 0; base < n; base += tile) {
+    P p{init, 0};
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+      const auto i = base + decltype(n + 0)(tid) * G + j;
+      if (i < n) p = pop(p, P{T(first[i]), 1});
+    }
+    P total;
+    (void)detail::group_inclusive(g, p, pop, &total);
+    if (total.has) init = op(init, total.v);
+  }
+  return init;
+}
+
+// merge_by_key: the stable merge of two key ranges, each value travelling
+// with its key (merge.hpp:507-607); each agent finds the merge path of its
+// grain and merges it sequentially.  Outputs must not overlap the inputs.
+template <class Agent, std::size_t S, class It1, class It2, class It3, class It4, class It5,
+          class It6, class Compare>
+__device__ thrust::pair<It5, It6> merge_by_key(concurrent_group<Agent, S>& g, It1 keys_first1,
+                                               It1 keys_last1, It2 keys_first2, It2 keys_last2,
+                                               It3 values_first1, It4 values_first2,
+                                               It5 keys_result, It6 values_result, Compare comp) {
+  typedef long long size_type;
+  const size_type n1 = keys_last1 - keys_first1, n2 = keys_last2 - keys_first2, n = n1 + n2;
+  constexpr int G = Agent::static_grainsize;
+  for (size_type d = size_type(g.this_exec.index()) * G; d < n; d += size_type(g.size()) * G) {
+    size_type i = merge_path(keys_first1, n1, keys_first2, n2, d, comp);
+    size_type j = d - i;
+    for (int k = 0; k < G && d + k < n; ++k) {
+      const bool take_a = j >= n2 || (i < n1 && !comp(keys_first2[j], keys_first1[i]));
+      if (take_a) {
+        keys_result[d + k] = keys_first1[i];
+        values_result[d + k] = values_first1[i++];
+      } else {
+        keys_result[d + k] = keys_first2[j];
+        values_result[d + k] = values_first2[j++];
+      }
+    }
+  }
+  g.wait();
+  return thrust::make_pair(keys_result + n, values_result + n);
+}
+
+namespace detail {
+
+// One merge step of a bounded group: agent t produces outputs [t*G, t*G+G)
+// of the stable merge of [a, a+n1) and [a+n1, a+n1+n2) (keys, and values when
+// VALUES), holds them in registers until every agent has read its inputs,
+// then writes them back in place.
+template <bool VALUES, int G, class G_, class It1, class It2, class Compare>
+__device__ void inplace_merge_step(G_& g, It1 keys, It2 vals, long long pair_start, long long n1,
+                                   long long n2, long long p, long long count, Compare comp) {
+  typedef typename std::iterator_traits<It1>::value_type K;
+  typedef typename std::conditional<VALUES, typename std::iterator_traits<It2>::value_type,
+                                    int>::type V;
+  K k_out[G];
+  V v_out[G];
+  It1 a = keys + pair_start, b = a + n1;
+  long long i = 0, j = 0;
+  if (count > 0) {
+    const long long d = p - pair_start;
+    i = merge_path(a, n1, b, n2, d, comp);
+    j = d - i;
+#pragma unroll
+    for (int k = 0; k < G; ++k) {
+      if (k < count) {
+        const bool take_a = j >= n2 || (i < n1 && !comp(b[j], a[i]));
+        const long long src = take_a ? pair_start + i : pair_start + n1 + j;
+        k_out[k] = keys[src];
+        if constexpr (VALUES) v_out[k] = vals[src];
+        if (take_a) ++i;
+        else ++j;
+      }
+    }
+  }
+  g.wait();
+#pragma unroll
+  for (int k = 0; k < G; ++k) {
+    if (k < count) {
+      keys[p + k] = k_out[k];
+      if constexpr (VALUES) vals[p + k] = v_out[k];
+    }
+  }
+  g.wait();
+}
+
+}  // namespace detail
+
+// inplace_merge on a bounded group: [first, middle) and [middle, last), each
+// sorted, become one sorted range; n <= bound <= groupsize * grainsize
+// (merge.hpp:308-353).
+template <std::size_t B, std::size_t S, std::size_t G, class It, class Compare>
+__device__ typename std::enable_if<(B <= S * G)>::type inplace_merge(
+    bounded<B, concurrent_group<agent<G>, S>>& g, It first, It middle, It last, Compare comp) {
+  const long long n1 = middle - first, n2 = last - middle, n = n1 + n2;
+  const long long p = (long long)(g.this_exec.index()) * G;
+  const long long count = p < n ? (n - p < (long long)G ? n - p : (long long)G) : 0;
+  detail::inplace_merge_step<false, int(G)>(g, first, (int*)nullptr, 0, n1, n2, p, count, comp);
+}
+
+// stable_sort_by_key on a bounded group: each agent sorts its grain by odd-even
+// transposition, then rounds of in-place merges of adjacent sorted runs
+// double the run length until one run remains (sort.hpp:152-166,
+// stable_merge_sort.hpp:125-206).  n <= bound <= groupsize * grainsize.
+template <std::size_t B, std::size_t S, std::size_t G, class It1, class It2, class Compare>
+__device__ typename std::enable_if<(B <= S * G)>::type stable_sort_by_key(
+    bounded<B, concurrent_group<agent<G>, S>>& g, It1 keys_first, It1 keys_last, It2 values_first,
+    Compare comp) {
+  typedef typename std::iterator_traits<It1>::value_type K;
+  typedef typename std::iterator_traits<It2>::value_type V;
+  const long long n = keys_last - keys_first;
+  const long long p = (long long)(g.this_exec.index()) * G;
+  const int cnt = p < n ? int(n - p < (long long)G ? n - p : (long long)G) : 0;
+  {
+    K k[G];
+    V v[G];
+#pragma unroll
+    for (int j = 0; j < int(G); ++j) {
+      if (j < cnt) {
+        k[j] = keys_first[p + j];
+        v[j] = values_first[p + j];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < int(G); ++r) {
+#pragma unroll
+      for (int j = r & 1; j + 1 < int(G); j += 2) {
+        if (j + 1 < cnt && comp(k[j + 1], k[j])) {
+          K tk = k[j];
+          k[j] = k[j + 1];
+          k[j + 1] = tk;
+          V tv = v[j];
+          v[j] = v[j + 1];
+          v[j + 1] = tv;
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < int(G); ++j) {
+      if (j < cnt) {
+        keys_first[p + j] = k[j];
+        values_first[p + j] = v[j];
+      }
+    }
+  }
+  g.wait();
+  for (long long w = G; w < n; w *= 2) {
+    // 2w is a multiple of the grain, so an agent's outputs lie in one pair of runs
+    const long long pair_start = p < n ? (p / (2 * w)) * (2 * w) : 0;
+    const long long rest = n - pair_start;
+    const long long n1 = rest < w ? rest : w;
+    const long long n2 = rest - n1 < w ? rest - n1 : w;
+    detail::inplace_merge_step<true, int(G)>(g, keys_first, values_first, pair_start, n1, n2, p,
+                                             cnt, comp);
+  }
+}
+
+// reduce_by_key on a group (reduce_by_key.hpp:115-216): runs of keys equal
+// under pred are reduced with op, continuing the run (init_key, init_value)
+// carried in.  Per tile of groupsize * grainsize elements, element i is a
+// head when !pred(x[i], x[i-1]) (element 0 compares with the carried key as
+// !pred(init_key, x[0])); every run that ends inside the tile writes its last
+// key and its sum to the next output slot, and the carried run is written
+// first when the tile's first element starts a new run.  The tile's last run
+// is carried to the next tile; the returned tuple holds the output ends and
+// the final carry (key, value), which is not written.  Outputs may alias the
+// inputs' start (each output slot is at or before the element that fills it).
+template <std::size_t S, std::size_t G, class It1, class It2, class It3, class It4, class T1,
+          class T2, class Pred, class Op>
+__device__ thrust::tuple<It3, It4, typename std::iterator_traits<It1>::value_type,
+                         typename std::iterator_traits<It4>::value_type>
+reduce_by_key(concurrent_group<agent<G>, S>& g, It1 keys_first, It1 keys_last, It2 values_first,
+              It3 keys_result, It4 values_result, T1 init_key, T2 init_value, Pred pred, Op op) {
+  typedef typename std::iterator_traits<It1>::value_type K;
+  typedef typename std::iterator_traits<It2>::value_type V;
+  typedef typename std::iterator_traits<It4>::value_type VOut;
+  // (heads so far, fold of the values since the last head); `has` marks a
+  // non-empty part; the combine is the scan_head_flags operator
+  // (reduce_by_key.hpp:41-62)
+  struct P {
+    long long cnt;
+    V v;
+    int has;
+  };
+  auto pop = [op](P a, P b) {
+    if (!b.has) return a;
+    if (!a.has) return b;
+    return P{a.cnt + b.cnt, b.cnt ? b.v : V(op(a.v, b.v)), 1};
+  };
+  const int tid = g.this_exec.index();
+  const long long tile = (long long)(g.size()) * G;
+  K carry_key = K(init_key);
+  V carry_val = V(init_value);
+  for (; keys_first < keys_last; keys_first += tile, values_first += tile) {
+    const long long rem = keys_last - keys_first;
+    const long long n = rem < tile ? rem : tile;
+    const long long base = (long long)(tid) * G;
+    K k[G];
+    V v[G];
+    bool head[G];
+    bool tail[G];
+    P mine{0, V(), 0};
+#pragma unroll
+    for (int j = 0; j < int(G); ++j) {
+      const long long i = base + j;
+      head[j] = false;
+      tail[j] = false;
+      if (i < n) {
+        k[j] = keys_first[i];
+        v[j] = values_first[i];
+        if (i == 0) head[j] = !pred(carry_key, k[j]);
+        else head[j] = !pred(k[j], j > 0 ? k[j > 0 ? j - 1 : 0] : K(keys_first[i - 1]));
+        mine = pop(mine, P{head[j] ? 1 : 0, v[j], 1});
+      }
+    }
+    // element i ends a run written out when i < n - 1 and i + 1 is a head
+#pragma unroll
+    for (int j = 0; j < int(G); ++j) {
+      const long long i = base + j;
+      if (i + 1 < n)
+        tail[j] = j + 1 < int(G) ? head[j + 1 < int(G) ? j + 1 : j]
+                                 : !pred(K(keys_first[i + 1]), k[j]);
+    }
+    const K last_key = keys_first[n - 1];
+    P total;
+    const P inc = detail::group_inclusive(g, mine, pop, &total);  // waits: all loads are done
+    // exclusive prefix of this agent, seeded with the carried run (1 slot, carry_val)
+    P excl = detail::shfl_up(inc, 1);
+    __shared__ P s_last[16];
+    const int lane = tid & 63, wid = tid >> 6;
+    if (lane == 63) s_last[wid] = inc;
+    g.wait();
+    if (lane == 0) excl = wid > 0 ? s_last[wid - 1] : P{0, V(), 0};
+    P r = pop(P{1, carry_val, 1}, excl);
+#pragma unroll
+    for (int j = 0; j < int(G); ++j) {
+      const long long i = base + j;
+      if (i < n) {
+        r = pop(r, P{head[j] ? 1 : 0, v[j], 1});
+        if (tail[j]) {
+          keys_result[r.cnt - 1] = k[j];
+          values_result[r.cnt - 1] = VOut(r.v);
+        }
+      }
+    }
+    if (tid == 0 && head[0]) {
+      keys_result[0] = carry_key;
+      values_result[0] = VOut(carry_val);
+    }
+    const P fin = pop(P{1, carry_val, 1}, total);
+    keys_result += fin.cnt - 1;
+    values_result += fin.cnt - 1;
+    carry_key = last_key;
+    carry_val = fin.v;
+    g.wait();  // s_last is rewritten by the next tile
+  }
+  return thrust::make_tuple(keys_result, values_result, carry_key, VOut(carry_val));
+}
+
+// scatter_if: result[map[i]] = first[i] where stencil[i], striped over the
+// group (scatter.hpp:96-178).
+template <class Agent, std::size_t S, class It1, class It2, class It3, class It4>
+__device__ void scatter_if(concurrent_group<Agent, S>& g, It1 first, It1 last, It2 map,
+                           It3 stencil, It4 result) {
+  const auto n = last - first;
+  for (decltype(n + 0) i = g.this_exec.index(); i < n; i += g.size())
+    if (stencil[i]) result[map[i]] = first[i];
+  g.wait();
+}
+
+// gather: result[i] = input[map[i]], striped over the group (gather.hpp:67-78).
+template <class Agent, std::size_t S, class It1, class It2, class It3>
+__device__ It3 gather(concurrent_group<Agent, S>& g, It1 map_first, It1 map_last,
+                      It2 input_first, It3 result) {
+  const auto n = map_last - map_first;
+  for (decltype(n + 0) i = g.this_exec.index(); i < n; i += g.size())
+    result[i] = input_first[map_first[i]];
   g.wait();
   return result + n;
 }

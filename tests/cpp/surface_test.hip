@@ -106,6 +106,109 @@ struct one_group {
   }
 };
 
+// bounded-agent bodies: each agent works on its own grain of at most 8 elements
+struct lcg_op {  // sequential fold that is neither commutative nor associative
+  __host__ __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a * 3u + b; }
+};
+
+struct agent_bodies {
+  __device__ void operator()(bulk::agent<>& self, const int* in, int n, int* incl, int* excl,
+                             int* sums, uint32_t* folds, int* skeys, int* svals, const int* map,
+                             const int* stencil, int* gathered, int* scattered) {
+    const int t = self.index();
+    const int lo = t * 8;
+    const int cnt = n - lo < 8 ? n - lo : 8;
+    auto& b = bulk::bound<8>(self);
+    bulk::inclusive_scan(b, in + lo, in + lo + cnt, incl + lo, 3, add_op());
+    bulk::exclusive_scan(b, in + lo, in + lo + cnt, excl + lo, 3, add_op());
+    sums[t] = bulk::reduce(b, in + lo, in + lo + cnt, 0, add_op());
+    folds[t] = bulk::accumulate(b, in + lo, in + lo + cnt, 7u, lcg_op());
+    bulk::stable_sort_by_key(b, skeys + lo, skeys + lo + cnt, svals + lo,
+                             [](int u, int v) { return u < v; });
+    bulk::gather(b, map + lo, map + lo + cnt, in, gathered + lo);
+    bulk::scatter_if(b, in + lo, in + lo + cnt, map + lo, stencil + lo, scattered);
+  }
+};
+
+// bounded-agent merge and merge_by_key of two sorted 4- and 5-element grains
+struct agent_merges {
+  __device__ void operator()(bulk::agent<>& self, const int* a, const int* b, const int* va,
+                             const int* vb, int* m, int* mk, int* mv) {
+    const int t = self.index();
+    auto& e = bulk::bound<9>(self);
+    auto less = [](int u, int v) { return u < v; };
+    bulk::merge(e, a + 4 * t, a + 4 * t + 4, b + 5 * t, b + 5 * t + 5, m + 9 * t, less);
+    bulk::merge_by_key(e, a + 4 * t, a + 4 * t + 4, b + 5 * t, b + 5 * t + 5, va + 4 * t,
+                       vb + 5 * t, mk + 9 * t, mv + 9 * t, less);
+  }
+};
+
+// bounded group: stable_sort_by_key staged in the LDS arena, inplace_merge in global memory
+struct group_sort_merge {
+  template <std::size_t G, std::size_t S>
+  __device__ void operator()(bulk::concurrent_group<bulk::agent<G>, S>& g, int* keys, int* vals,
+                             int n, int* x, int mid, int nx) {
+    int* sk = static_cast<int*>(bulk::malloc(g, n * sizeof(int)));
+    int* sv = static_cast<int*>(bulk::malloc(g, n * sizeof(int)));
+    bulk::copy_n(g, keys, n, sk);
+    bulk::copy_n(g, vals, n, sv);
+    auto less = [](int u, int v) { return u < v; };
+    bulk::stable_sort_by_key(bulk::bound<S * G>(g), sk, sk + n, sv, less);
+    bulk::copy_n(g, sk, n, keys);
+    bulk::copy_n(g, sv, n, vals);
+    bulk::free(g, sk);
+    bulk::inplace_merge(bulk::bound<S * G>(g), x, x + mid, x + nx, less);
+  }
+};
+
+struct affine {
+  uint32_t a, b;
+};
+struct compose {  // apply f, then h: associative, not commutative
+  __host__ __device__ affine operator()(affine f, affine h) const {
+    return affine{f.a * h.a, f.b * h.a + h.b};
+  }
+};
+
+struct rbk_result {
+  long long nout;
+  int carry_key;
+  int carry_val;
+};
+
+// group: merge_by_key, accumulate, reduce_by_key, scatter_if, gather, flag ranges
+struct group_more {
+  template <std::size_t G, std::size_t S>
+  __device__ void operator()(bulk::concurrent_group<bulk::agent<G>, S>& g, const int* a, int na,
+                             const int* va, const int* b, int nb, const int* vb, int* mk, int* mv,
+                             const affine* f, int nf, affine* acc, const int* keys,
+                             const int* vals, int n, int init_key, int* ko, int* vo,
+                             rbk_result* res, const int* map, const int* stencil, int* gathered,
+                             int* scattered, int* flag_counts) {
+    auto less = [](int u, int v) { return u < v; };
+    bulk::merge_by_key(g, a, a + na, b, b + nb, va, vb, mk, mv, less);
+    const affine r = bulk::accumulate(g, f, f + nf, affine{1u, 0u}, compose());
+    if (g.this_exec.index() == 0) *acc = r;
+    auto eq = [](int u, int v) { return u == v; };
+    auto t = bulk::reduce_by_key(g, keys, keys + n, vals, ko, vo, init_key, 100, eq, add_op());
+    if (g.this_exec.index() == 0)
+      *res = rbk_result{(long long)(thrust::get<0>(t) - ko), thrust::get<2>(t), thrust::get<3>(t)};
+    bulk::gather(g, map, map + n, vals, gathered);
+    bulk::scatter_if(g, vals, vals + n, map, stencil, scattered);
+    auto hf = bulk::make_head_flags(keys, keys + n);
+    auto tf = bulk::make_tail_flags(keys, keys + n, eq);
+    bulk::head_flags_with_init<const int*> hfi(keys, keys + n, init_key);
+    const int heads = bulk::reduce(g, hf.begin(), hf.end(), 0, add_op());
+    const int tails = bulk::reduce(g, tf.begin(), tf.end(), 0, add_op());
+    const int heads_init = bulk::reduce(g, hfi.begin(), hfi.end(), 0, add_op());
+    if (g.this_exec.index() == 0) {
+      flag_counts[0] = heads;
+      flag_counts[1] = tails;
+      flag_counts[2] = heads_init;
+    }
+  }
+};
+
 int main() {
   // hello: par(32), bulk::root is the group
   {
@@ -179,6 +282,208 @@ int main() {
     }
     CHECK(dm.get() == m, "con<512,3> merge");
     CHECK(de.get() == e, "con<512,3> exclusive_scan");
+  }
+  // bounded-agent bodies over grains of 8 (the last one ragged)
+  {
+    const int n = 8 * 123 + 5, agents = (n + 7) / 8;
+    std::vector<int> in(n), skeys(n), svals(n), map(n), stencil(n);
+    for (int i = 0; i < n; ++i) {
+      in[i] = int((i * 2654435761u) % 201) - 100;
+      skeys[i] = in[i] & 3;
+      svals[i] = i;
+      map[i] = int((uint64_t(i) * 7919u) % uint64_t(n));  // a permutation (7919 is prime, n is not a multiple)
+      stencil[i] = i % 3 != 0;
+    }
+    dvec<int> din(in), dincl(n), dexcl(n), dsums(agents), dsk(skeys), dsv(svals), dmap(map),
+        dst(stencil), dga(n), dsc(std::vector<int>(n, -7));
+    dvec<uint32_t> dfold(agents);
+    bulk::async(bulk::par(agents), agent_bodies(), bulk::root.this_exec, din.p, n, dincl.p, dexcl.p,
+                dsums.p, dfold.p, dsk.p, dsv.p, dmap.p, dst.p, dga.p, dsc.p)
+        .wait();
+    auto hi = dincl.get(), he = dexcl.get(), hs = dsums.get(), hk = dsk.get(), hv = dsv.get(),
+         hg = dga.get(), hsc = dsc.get();
+    auto hf = dfold.get();
+    bool ok_scan = true, ok_red = true, ok_sort = true;
+    for (int t = 0; t < agents; ++t) {
+      const int lo = t * 8, cnt = std::min(8, n - lo);
+      int run = 3, sum = 0;
+      uint32_t fold = 7u;
+      std::vector<std::pair<int, int>> grain;
+      for (int j = 0; j < cnt; ++j) {
+        ok_scan &= he[lo + j] == run;
+        run += in[lo + j];
+        ok_scan &= hi[lo + j] == run;
+        sum += in[lo + j];
+        fold = fold * 3u + uint32_t(in[lo + j]);
+        grain.push_back({skeys[lo + j], svals[lo + j]});
+      }
+      ok_red &= hs[t] == sum && hf[t] == fold;
+      std::stable_sort(grain.begin(), grain.end(),
+                       [](const auto& u, const auto& v) { return u.first < v.first; });
+      for (int j = 0; j < cnt; ++j)
+        ok_sort &= hk[lo + j] == grain[j].first && hv[lo + j] == grain[j].second;
+    }
+    CHECK(ok_scan, "bounded agent inclusive_scan / exclusive_scan");
+    CHECK(ok_red, "bounded agent reduce / accumulate (sequential left fold)");
+    CHECK(ok_sort, "bounded agent stable_sort_by_key (odd-even transposition, ties keep order)");
+    bool ok_g = true, ok_s = true;
+    for (int i = 0; i < n; ++i) ok_g &= hg[i] == in[map[i]];
+    std::vector<int> sc(n, -7);
+    for (int i = 0; i < n; ++i)
+      if (stencil[i]) sc[map[i]] = in[i];
+    for (int i = 0; i < n; ++i) ok_s &= hsc[i] == sc[i];
+    CHECK(ok_g, "bounded agent gather");
+    CHECK(ok_s, "bounded agent scatter_if");
+  }
+  // bounded-agent merge / merge_by_key with ties (the first range wins)
+  {
+    const int agents = 200;
+    std::vector<int> a(4 * agents), b(5 * agents), va(a.size()), vb(b.size());
+    for (int t = 0; t < agents; ++t) {
+      for (int j = 0; j < 4; ++j) a[4 * t + j] = (t % 7) + j * (t % 3);
+      for (int j = 0; j < 5; ++j) b[5 * t + j] = (t % 5) + j * (t % 2);
+    }
+    for (size_t i = 0; i < a.size(); ++i) va[i] = int(i);
+    for (size_t i = 0; i < b.size(); ++i) vb[i] = 100000 + int(i);
+    dvec<int> da(a), db(b), dva(va), dvb(vb), dm(9 * agents), dmk(9 * agents), dmv(9 * agents);
+    bulk::async(bulk::par(agents), agent_merges(), bulk::root.this_exec, da.p, db.p, dva.p, dvb.p,
+                dm.p, dmk.p, dmv.p)
+        .wait();
+    auto hm = dm.get(), hk = dmk.get(), hv = dmv.get();
+    bool ok = true;
+    for (int t = 0; t < agents; ++t) {
+      std::vector<std::pair<int, int>> pa, pb, pm(9);
+      for (int j = 0; j < 4; ++j) pa.push_back({a[4 * t + j], va[4 * t + j]});
+      for (int j = 0; j < 5; ++j) pb.push_back({b[5 * t + j], vb[5 * t + j]});
+      std::merge(pa.begin(), pa.end(), pb.begin(), pb.end(), pm.begin(),
+                 [](const auto& u, const auto& v) { return u.first < v.first; });
+      for (int j = 0; j < 9; ++j)
+        ok &= hm[9 * t + j] == pm[j].first && hk[9 * t + j] == pm[j].first &&
+              hv[9 * t + j] == pm[j].second;
+    }
+    CHECK(ok, "bounded agent merge / merge_by_key");
+  }
+  // con<256, 4>: bounded-group stable_sort_by_key in LDS, inplace_merge
+  {
+    const int n = 1000, nx = 990, mid = 400;
+    std::vector<int> keys(n), vals(n), x(nx);
+    for (int i = 0; i < n; ++i) {
+      keys[i] = int((i * 2654435761u) % 37);
+      vals[i] = i;
+    }
+    for (int i = 0; i < mid; ++i) x[i] = i / 3;
+    for (int i = mid; i < nx; ++i) x[i] = (i - mid) / 5;
+    dvec<int> dk(keys), dv(vals), dx(x);
+    bulk::async(bulk::con<256, 4>(2 * n * sizeof(int)), group_sort_merge(), bulk::root, dk.p, dv.p,
+                n, dx.p, mid, nx)
+        .wait();
+    std::vector<size_t> order(n);
+    std::iota(order.begin(), order.end(), size_t(0));
+    std::stable_sort(order.begin(), order.end(), [&](size_t i, size_t j) { return keys[i] < keys[j]; });
+    auto hk = dk.get(), hv = dv.get();
+    bool ok = true;
+    for (int i = 0; i < n; ++i) ok &= hk[i] == keys[order[i]] && hv[i] == vals[order[i]];
+    CHECK(ok, "bounded group stable_sort_by_key (LDS arena)");
+    std::vector<int> xm(x);
+    std::inplace_merge(xm.begin(), xm.begin() + mid, xm.end());
+    CHECK(dx.get() == xm, "bounded group inplace_merge");
+  }
+  // con<256, 4>: group merge_by_key, accumulate, reduce_by_key, scatter_if, gather, flags
+  for (int variant = 0; variant < 2; ++variant) {
+    const int na = 3001, nb = 2500, nf = 5000, n = 5003;
+    std::vector<int> a(na), va(na), b(nb), vb(nb);
+    for (int i = 0; i < na; ++i) {
+      a[i] = i / 4;
+      va[i] = i;
+    }
+    for (int i = 0; i < nb; ++i) {
+      b[i] = i / 3;
+      vb[i] = 1 << 20 | i;
+    }
+    std::vector<affine> f(nf);
+    for (int i = 0; i < nf; ++i) f[i] = affine{uint32_t(i * 2654435761u) | 1u, uint32_t(i * 40503u)};
+    std::vector<int> keys(n), vals(n), map(n), stencil(n);
+    {
+      int key = 11, left = 1;
+      uint32_t r = 12345u;
+      for (int i = 0; i < n; ++i) {
+        if (--left == 0) {
+          ++key;
+          r = r * 1664525u + 1013904223u;
+          left = 1 + int((r >> 16) % 17);
+        }
+        keys[i] = key;
+        vals[i] = int((i * 2654435761u) % 1001) - 500;
+        map[i] = int((uint64_t(i) * 7919u) % uint64_t(n));
+        stencil[i] = (i % 5) != 2;
+      }
+    }
+    // variant 0: the carried run continues into the first keys; 1: it ends before them
+    const int init_key = variant == 0 ? keys[0] : -5;
+    dvec<int> da(a), dva(va), db(b), dvb(vb), dmk(na + nb), dmv(na + nb), dkeys(keys), dvals(vals),
+        dko(n + 1), dvo(n + 1), dmap(map), dst(stencil), dga(n), dsc(std::vector<int>(n, -9)),
+        dfc(3);
+    dvec<affine> df(f), dacc(1);
+    dvec<rbk_result> dres(1);
+    bulk::async(bulk::con<256, 4>(0), group_more(), bulk::root, da.p, na, dva.p, db.p, nb, dvb.p,
+                dmk.p, dmv.p, df.p, nf, dacc.p, dkeys.p, dvals.p, n, init_key, dko.p, dvo.p,
+                dres.p, dmap.p, dst.p, dga.p, dsc.p, dfc.p)
+        .wait();
+    if (variant == 0) {
+      std::vector<std::pair<int, int>> pa, pb, pm(na + nb);
+      for (int i = 0; i < na; ++i) pa.push_back({a[i], va[i]});
+      for (int i = 0; i < nb; ++i) pb.push_back({b[i], vb[i]});
+      std::merge(pa.begin(), pa.end(), pb.begin(), pb.end(), pm.begin(),
+                 [](const auto& u, const auto& v) { return u.first < v.first; });
+      auto hk = dmk.get(), hv = dmv.get();
+      bool ok = true;
+      for (int i = 0; i < na + nb; ++i) ok &= hk[i] == pm[i].first && hv[i] == pm[i].second;
+      CHECK(ok, "group merge_by_key (ties take the first range)");
+      affine e{1u, 0u};
+      for (int i = 0; i < nf; ++i) e = compose()(e, f[i]);
+      const affine got = dacc.get()[0];
+      CHECK(got.a == e.a && got.b == e.b, "group accumulate (in order, non-commutative op)");
+    }
+    // host restatement: the runs of (init_key, 100) followed by the input; all
+    // but the last are written (key = the run's last key), the last is the carry
+    std::vector<int> rk, rv;
+    int ck = init_key, cv = 100;
+    for (int i = 0; i < n; ++i) {
+      if (keys[i] != ck) {
+        rk.push_back(ck);
+        rv.push_back(cv);
+        cv = vals[i];
+      } else {
+        cv += vals[i];
+      }
+      ck = keys[i];
+    }
+    const rbk_result res = dres.get()[0];
+    const bool ok_rbk = res.nout == (long long)rk.size() && res.carry_key == ck &&
+                        res.carry_val == cv && dko.get(rk.size()) == rk && dvo.get(rv.size()) == rv;
+    CHECK(ok_rbk, variant == 0 ? "group reduce_by_key (carried run continues)"
+                               : "group reduce_by_key (carried run written first)");
+    if (variant == 0) {
+      auto hg = dga.get(), hsc = dsc.get();
+      bool ok_g = true, ok_s = true;
+      std::vector<int> sc(n, -9);
+      for (int i = 0; i < n; ++i) {
+        ok_g &= hg[i] == vals[map[i]];
+        if (stencil[i]) sc[map[i]] = vals[i];
+      }
+      for (int i = 0; i < n; ++i) ok_s &= hsc[i] == sc[i];
+      CHECK(ok_g, "group gather");
+      CHECK(ok_s, "group scatter_if");
+    }
+    int heads = 0, tails = 0;
+    for (int i = 0; i < n; ++i) {
+      heads += i == 0 || keys[i] != keys[i - 1];
+      tails += i == n - 1 || keys[i] != keys[i + 1];
+    }
+    const int heads_init = heads - 1 + (init_key != keys[0]);
+    auto hfc = dfc.get();
+    CHECK(hfc[0] == heads && hfc[1] == tails && hfc[2] == heads_init,
+          "head_flags / tail_flags / head_flags_with_init");
   }
   // driver facade over the C-ABI
   {
