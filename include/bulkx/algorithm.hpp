@@ -18,8 +18,9 @@
 // (std:: and thrust:: functors, or the ones below).  Every call is ordered
 // on `bulkx::default_stream()` (the null stream unless set); my_reduce and
 // my_reduce_by_key read their scalar back with a blocking copy, as the
-// reference does (reduce.cu:96, reduce_by_key.cu:269).  Errors are thrown
-// as bulkx::system_error, the counterpart of throw_on_error
+// reference does (reduce.cu:96, reduce_by_key.cu:269); async_reduce and
+// async_reduce_by_key return the scalar through a scalar_future instead.
+// Errors are thrown as bulkx::system_error, the counterpart of throw_on_error
 // (bulk/detail/throw_on_error.hpp:33-49).
 #pragma once
 
@@ -254,6 +255,164 @@ std::pair<KIt2, VIt2> my_reduce_by_key(KIt1 keys_first, KIt1 keys_last, VIt1 val
   (void)hipFreeAsync(d, default_stream());
   check(st, "my_reduce_by_key");
   return {keys_result + m, values_result + m};
+}
+
+// A scalar result that is still on its way to the host: the value is copied
+// from device memory into pinned host memory behind the producing kernel on
+// the stream, and an event marks its arrival.  get() waits for that event
+// only; the stream is never synchronised, so the caller keeps issuing work
+// while the value travels.  This is the non-blocking form of the scalar
+// read that my_reduce and my_reduce_by_key make (reduce.cu:96,
+// reduce_by_key.cu:269).  Move-only; the destructor waits for the copy
+// before releasing the pinned slot and swallows errors, as the reference's
+// future destructor does (bulk/future.hpp:53-75).
+template <class T>
+class scalar_future {
+ public:
+  scalar_future() = default;
+  scalar_future(const scalar_future&) = delete;
+  scalar_future& operator=(const scalar_future&) = delete;
+  scalar_future(scalar_future&& o) noexcept { swap(o); }
+  scalar_future& operator=(scalar_future&& o) noexcept {
+    release();
+    swap(o);
+    return *this;
+  }
+  ~scalar_future() { release(); }
+
+  bool valid() const { return m_host != nullptr; }
+  // true once the value has reached the host (never blocks)
+  bool ready() const {
+    if (!valid()) return false;
+    const hipError_t e = hipEventQuery(m_event);
+    if (e == hipErrorNotReady) return false;
+    check(e == hipSuccess ? BULKX_SUCCESS : BULKX_ERR_HIP, "scalar_future::ready");
+    return true;
+  }
+  void wait() const {
+    if (!valid()) throw system_error(BULKX_ERR_INVALID_ARGUMENT, "scalar_future::wait");
+    check(hipEventSynchronize(m_event) == hipSuccess ? BULKX_SUCCESS : BULKX_ERR_HIP,
+          "scalar_future::wait");
+  }
+  T get() const {
+    wait();
+    return *m_host;
+  }
+
+  // Enqueues the copy of *device_value on `stream` and records the event.
+  // The device scalar stays the caller's.
+  static scalar_future from_device(const T* device_value, hipStream_t stream) {
+    scalar_future f;
+    check(hipHostMalloc(reinterpret_cast<void**>(&f.m_host), sizeof(T), hipHostMallocDefault) ==
+                  hipSuccess
+              ? BULKX_SUCCESS
+              : BULKX_ERR_OUT_OF_MEMORY,
+          "scalar_future");
+    check(hipEventCreateWithFlags(&f.m_event, hipEventDisableTiming) == hipSuccess
+              ? BULKX_SUCCESS
+              : BULKX_ERR_HIP,
+          "scalar_future");
+    check(hipMemcpyAsync(f.m_host, device_value, sizeof(T), hipMemcpyDeviceToHost, stream) ==
+                      hipSuccess &&
+                  hipEventRecord(f.m_event, stream) == hipSuccess
+              ? BULKX_SUCCESS
+              : BULKX_ERR_HIP,
+          "scalar_future");
+    return f;
+  }
+
+ private:
+  void swap(scalar_future& o) noexcept {
+    std::swap(m_host, o.m_host);
+    std::swap(m_event, o.m_event);
+  }
+  void release() noexcept {
+    if (m_event) {
+      (void)hipEventSynchronize(m_event);
+      (void)hipEventDestroy(m_event);
+    }
+    if (m_host) (void)hipHostFree(m_host);
+    m_host = nullptr;
+    m_event = nullptr;
+  }
+  T* m_host = nullptr;
+  hipEvent_t m_event = nullptr;
+};
+
+namespace detail {
+// A stream-ordered device scalar of type T, freed behind the work queued so far.
+template <class T>
+struct stream_scalar {
+  T* p = nullptr;
+  hipStream_t s;
+  explicit stream_scalar(hipStream_t s_) : s(s_) {
+    check(hipMallocAsync(reinterpret_cast<void**>(&p), sizeof(T), s) == hipSuccess
+              ? BULKX_SUCCESS
+              : BULKX_ERR_OUT_OF_MEMORY,
+          "device scalar");
+  }
+  ~stream_scalar() { (void)hipFreeAsync(p, s); }
+};
+}  // namespace detail
+
+// async_reduce(first, last, init, op[, stream]): my_reduce without the
+// blocking read; the result arrives through the returned future.
+template <class It, class T, class Op>
+scalar_future<T> async_reduce(It first, It last, T init, Op op,
+                              hipStream_t stream = default_stream()) {
+  using V = detail::value_t<It>;
+  static_assert(std::is_same<V, T>::value, "init has the element type");
+  static_assert(detail::op_code<Op>() >= 0, "op must be plus, minimum or maximum");
+  const V iv = V(init);
+  detail::stream_scalar<V> d(stream);
+  check(bulkx_reduce(detail::dtype_of<V>::value, detail::op_code<Op>(), detail::raw(first),
+                     int64_t(last - first), &iv, d.p, nullptr, 0, stream),
+        "async_reduce");
+  return scalar_future<T>::from_device(d.p, stream);
+}
+
+// The result of async_reduce_by_key: get() gives the ends of the written
+// key and value ranges, as my_reduce_by_key returns them.
+template <class KIt, class VIt>
+class reduce_by_key_future {
+ public:
+  reduce_by_key_future(scalar_future<int64_t>&& count, KIt keys_result, VIt values_result)
+      : m_count(std::move(count)), m_keys(keys_result), m_values(values_result) {}
+  bool valid() const { return m_count.valid(); }
+  bool ready() const { return m_count.ready(); }
+  void wait() const { m_count.wait(); }
+  int64_t num_runs() const { return m_count.get(); }
+  std::pair<KIt, VIt> get() const {
+    const int64_t m = m_count.get();
+    return {m_keys + m, m_values + m};
+  }
+
+ private:
+  scalar_future<int64_t> m_count;
+  KIt m_keys;
+  VIt m_values;
+};
+
+// async_reduce_by_key(...): my_reduce_by_key without the blocking read of
+// the run count.
+template <class KIt1, class VIt1, class KIt2, class VIt2, class Pred, class Op>
+reduce_by_key_future<KIt2, VIt2> async_reduce_by_key(KIt1 keys_first, KIt1 keys_last,
+                                                     VIt1 values_first, KIt2 keys_result,
+                                                     VIt2 values_result, Pred pred, Op op,
+                                                     hipStream_t stream = default_stream()) {
+  using K = detail::value_t<KIt1>;
+  using V = detail::value_t<VIt1>;
+  static_assert(detail::pred_code<Pred>() >= 0, "pred must be equal_to");
+  static_assert(detail::op_code<Op>() >= 0, "op must be plus, minimum or maximum");
+  detail::stream_scalar<int64_t> d(stream);
+  check(bulkx_reduce_by_key(detail::dtype_of<K>::value, detail::dtype_of<V>::value,
+                            detail::pred_code<Pred>(), detail::op_code<Op>(),
+                            detail::raw(keys_first), detail::raw(values_first),
+                            int64_t(keys_last - keys_first), detail::raw(keys_result),
+                            detail::raw(values_result), d.p, nullptr, 0, stream),
+        "async_reduce_by_key");
+  return reduce_by_key_future<KIt2, VIt2>(scalar_future<int64_t>::from_device(d.p, stream),
+                                          keys_result, values_result);
 }
 
 // aligned_decomposition(n, num_partitions, tile): ceil(n / tile) tiles split

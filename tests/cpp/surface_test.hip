@@ -560,6 +560,21 @@ int main() {
     }
     CHECK(m == long(rk.size()) && ko.get(m) == rk && vo.get(m) == rv, "bulkx::my_reduce_by_key");
 
+    // the scalar results without the blocking read: a future per call on a stream
+    {
+      hipStream_t s;
+      HIP_OK(hipStreamCreate(&s));
+      bulkx::scalar_future<int> f = bulkx::async_reduce(d.p, d.p + n, 13, std::plus<int>(), s);
+      auto g = bulkx::async_reduce_by_key(dkk.p, dkk.p + n, dvv.p, ko.p, vo.p,
+                                          std::equal_to<int>(), std::plus<int>(), s);
+      bulkx::scalar_future<int> f2 = std::move(f);
+      CHECK(!f.valid() && f2.valid() && f2.get() == int(sum) && f2.ready(), "bulkx::async_reduce");
+      const auto e2 = g.get();
+      CHECK(g.ready() && g.num_runs() == m && e2.first - ko.p == m && e2.second - vo.p == m &&
+                ko.get(m) == rk && vo.get(m) == rv,
+            "bulkx::async_reduce_by_key");
+      HIP_OK(hipStreamDestroy(s));
+    }
     dvec<int> parts(37);
     bulkx::reduce_intervals(d.p, bulkx::make_aligned_decomposition<long>(n, 37, 896), parts.p,
                             std::plus<int>());
