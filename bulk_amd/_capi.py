@@ -97,6 +97,8 @@ SIGNATURES = {
     "bulkx_server_destroy": (_I, [_P]),
     "bulkx_server_submit": (_I, [_P, ctypes.POINTER(Job), _L, ctypes.POINTER(_L)]),
     "bulkx_server_submit_at": (_I, [_P, ctypes.POINTER(Job), _L, ctypes.c_double, ctypes.POINTER(_L)]),
+    "bulkx_server_submit_host": (_I, [_P, ctypes.POINTER(Job), ctypes.c_double, ctypes.POINTER(_L),
+                                      ctypes.POINTER(_L)]),
     "bulkx_server_now_us": (ctypes.c_double, [_P]),
     "bulkx_server_drain": (_I, [_P]),
     "bulkx_server_request_times": (_I, [_P, _L, ctypes.POINTER(ctypes.c_double),

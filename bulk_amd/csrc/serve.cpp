@@ -12,6 +12,13 @@
 // hipEvent per slot; every request gets arrival, launch and completion
 // timestamps (microseconds on one steady clock).  In per-call mode every
 // request is launched alone through the plain entry points.
+//
+// Host-staged requests (bulkx_server_submit_host) name host buffers: when
+// such a request is admitted, the slot allocates its device buffers
+// (stream-ordered), copies its inputs in ahead of the batch and its outputs
+// back behind it, and frees the buffers, all on the slot's stream; the copies
+// of one slot overlap the batches of the others.  Its footprint charged to
+// the budget is the size of those device buffers.
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -31,7 +38,14 @@ using Clock = std::chrono::steady_clock;
 struct Request {
   bulkx_job job;
   int64_t bytes = 0;
+  bool host = false;  // job names host buffers, staged through the device
   double t_arrival = 0, t_launch = 0, t_done = 0;
+};
+
+struct Copy {
+  void* dst;
+  const void* src;
+  size_t bytes;
 };
 
 struct Slot {
@@ -41,7 +55,122 @@ struct Slot {
   bool busy = false;
   int64_t bytes = 0;
   std::vector<int64_t> reqs;
+  std::vector<void*> staged;  // device buffers of the slot's host-staged requests
+  std::vector<Copy> back;     // their device -> host copies, after the batch
 };
+
+// The device buffers of a host-staged job, in bulkx_job order: for each of
+// in0..in3 and out0..out2, its size in bytes (0 when the job has none) and
+// whether it is copied in (input, or sorted in place) and back (output).
+struct Staging {
+  size_t bytes[7] = {0, 0, 0, 0, 0, 0, 0};
+  bool in[7] = {false, false, false, false, false, false, false};
+  bool out[7] = {false, false, false, false, false, false, false};
+  size_t total = 0;
+};
+
+constexpr size_t kStageAlign = 256;
+
+static bool staging_of(const bulkx_job& j, Staging* g) {
+  const size_t ks = bulkx_dtype_size(j.dtype);
+  const size_t vs = j.value_dtype >= 0 ? bulkx_dtype_size(j.value_dtype) : 0;
+  if (ks == 0 || j.n0 < 0 || j.n1 < 0) return false;
+  const size_t n0 = size_t(j.n0), n1 = size_t(j.n1);
+  auto set = [&](int i, size_t b, bool in, bool out) {
+    g->bytes[i] = b;
+    g->in[i] = in;
+    g->out[i] = out;
+  };
+  switch (j.kind) {
+    case BULKX_JOB_INCLUSIVE_SCAN:
+      set(0, n0 * ks, true, false);
+      set(4, n0 * ks, false, true);
+      break;
+    case BULKX_JOB_REDUCE:
+      set(0, n0 * ks, true, false);
+      set(4, ks, false, true);
+      break;
+    case BULKX_JOB_REDUCE_BY_KEY:  // outputs sized for n0 runs, copied back whole
+      if (vs == 0) return false;
+      set(0, n0 * ks, true, false);
+      set(1, n0 * vs, true, false);
+      set(4, n0 * ks, false, true);
+      set(5, n0 * vs, false, true);
+      set(6, sizeof(int64_t), false, true);
+      break;
+    case BULKX_JOB_MERGE_BY_KEY:
+      set(0, n0 * ks, true, false);
+      set(2, n1 * ks, true, false);
+      set(4, (n0 + n1) * ks, false, true);
+      if (j.in1 || j.in3 || j.out1) {
+        if (vs == 0) return false;
+        set(1, n0 * vs, true, false);
+        set(3, n1 * vs, true, false);
+        set(5, (n0 + n1) * vs, false, true);
+      }
+      break;
+    case BULKX_JOB_SORT_BY_KEY:  // in place
+      set(4, n0 * ks, true, true);
+      if (j.out1) {
+        if (vs == 0) return false;
+        set(5, n0 * vs, true, true);
+      }
+      break;
+    default:
+      return false;
+  }
+  g->total = 0;
+  for (size_t b : g->bytes) g->total += (b + kStageAlign - 1) / kStageAlign * kStageAlign;
+  if (g->total == 0) g->total = kStageAlign;
+  return true;
+}
+
+static const void* job_ptr(const bulkx_job& j, int i) {
+  const void* p[7] = {j.in0, j.in1, j.in2, j.in3, j.out0, j.out1, j.out2};
+  return p[i];
+}
+
+// Allocates the device buffers of host job `hj` on `stream`, enqueues the
+// copies of its inputs, and returns the device job in *dj; the copies back
+// are appended to `back` (enqueued after the batch).
+static int stage_in(const bulkx_job& hj, hipStream_t stream, bulkx_job* dj,
+                    std::vector<void*>& staged, std::vector<Copy>& back) {
+  Staging g;
+  if (!staging_of(hj, &g)) return BULKX_ERR_INVALID_ARGUMENT;
+  char* base = nullptr;
+  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&base), g.total, stream);
+  if (e != hipSuccess) {
+    bulkx::set_last_error("host-staged request", e);
+    return BULKX_ERR_OUT_OF_MEMORY;
+  }
+  staged.push_back(base);
+  void* dp[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  size_t off = 0;
+  for (int i = 0; i < 7; ++i) {
+    if (g.bytes[i] == 0) continue;
+    const void* hp = job_ptr(hj, i);
+    if (hp == nullptr) return BULKX_ERR_INVALID_ARGUMENT;
+    dp[i] = base + off;
+    off += (g.bytes[i] + kStageAlign - 1) / kStageAlign * kStageAlign;
+    if (g.in[i]) {
+      e = hipMemcpyAsync(dp[i], hp, g.bytes[i], hipMemcpyHostToDevice, stream);
+      if (e != hipSuccess) {
+        bulkx::set_last_error("host-staged request", e);
+        return BULKX_ERR_HIP;
+      }
+    }
+    if (g.out[i]) back.push_back(Copy{const_cast<void*>(hp), dp[i], g.bytes[i]});
+  }
+  *dj = hj;
+  dj->in0 = dp[0];
+  dj->in1 = dp[1];
+  dj->in2 = dp[2];
+  dj->in3 = dp[3];
+  dj->out0 = dp[4];
+  dj->out1 = dp[5];
+  dj->out2 = dp[6];
+  return BULKX_SUCCESS;
+}
 
 }  // namespace
 
@@ -94,6 +223,7 @@ static void serve_step(bulkx_server* s) {
     if (sl.busy || s->pending.empty() || s->error != BULKX_SUCCESS) continue;
     std::vector<bulkx_job> jobs;
     int64_t bytes = 0, elems = 0;
+    int st = BULKX_SUCCESS;
     const int cap = (s->flags & BULKX_SERVE_PER_CALL) ? 1 : s->max_batch_jobs;
     while (!s->pending.empty() && int(jobs.size()) < cap) {
       const Request& r = s->reqs[size_t(s->pending.front())];
@@ -101,18 +231,36 @@ static void serve_step(bulkx_server* s) {
       const int64_t e = job_elements(r.job);
       if (s->admitted_bytes + bytes + r.bytes > s->budget) break;  // wait for memory
       if (!jobs.empty() && elems + e > s->max_batch_elements) break;
-      jobs.push_back(r.job);
+      if (r.host) {
+        bulkx_job dj;
+        if (st == BULKX_SUCCESS) st = stage_in(r.job, sl.stream, &dj, sl.staged, sl.back);
+        jobs.push_back(st == BULKX_SUCCESS ? dj : r.job);
+      } else {
+        jobs.push_back(r.job);
+      }
       sl.reqs.push_back(s->pending.front());
       bytes += r.bytes;
       elems += e;
       s->pending.pop_front();
     }
     if (jobs.empty()) break;  // the head request waits for budget
-    int st = bulkx_batch_set_jobs(sl.batch, jobs.data(), int(jobs.size()));
+    if (st == BULKX_SUCCESS) st = bulkx_batch_set_jobs(sl.batch, jobs.data(), int(jobs.size()));
     if (st == BULKX_SUCCESS) {
       if (s->flags & BULKX_SERVE_PER_CALL) st = bulkx_batch_launch_per_call(sl.batch);
       else st = bulkx_batch_launch(sl.batch, (s->flags & BULKX_SERVE_NO_GRAPH) ? 0 : 1);
     }
+    // host-staged results back, then their device buffers released, in stream order
+    for (const Copy& c : sl.back) {
+      if (st != BULKX_SUCCESS) break;
+      const hipError_t e = hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyDeviceToHost, sl.stream);
+      if (e != hipSuccess) {
+        bulkx::set_last_error("host-staged request", e);
+        st = BULKX_ERR_HIP;
+      }
+    }
+    for (void* p : sl.staged) (void)hipFreeAsync(p, sl.stream);
+    sl.staged.clear();
+    sl.back.clear();
     if (st == BULKX_SUCCESS && hipEventRecord(sl.done, sl.stream) != hipSuccess) st = BULKX_ERR_HIP;
     const double t = s->now_us();
     for (int64_t id : sl.reqs) s->reqs[size_t(id)].t_launch = t;
@@ -211,7 +359,7 @@ extern "C" int bulkx_server_destroy(bulkx_server* s) {
 }
 
 static int submit_impl(bulkx_server* s, const bulkx_job* job, int64_t bytes, double arrival,
-                       int64_t* request_id) {
+                       int64_t* request_id, bool host = false) {
   if (!s || !job || bytes < 0) return BULKX_ERR_INVALID_ARGUMENT;
   if (bytes > s->budget) return BULKX_ERR_BUDGET;
   if (job_elements(*job) > s->max_batch_elements) return BULKX_ERR_BUDGET;
@@ -224,6 +372,7 @@ static int submit_impl(bulkx_server* s, const bulkx_job* job, int64_t bytes, dou
   Request r;
   r.job = *job;
   r.bytes = bytes;
+  r.host = host;
   r.t_arrival = arrival;
   const int64_t id = int64_t(s->reqs.size());
   s->reqs.push_back(r);
@@ -242,6 +391,18 @@ extern "C" int bulkx_server_submit_at(bulkx_server* s, const bulkx_job* job, int
                                       double arrival_us, int64_t* request_id) {
   if (arrival_us < 0) return BULKX_ERR_INVALID_ARGUMENT;
   return submit_impl(s, job, bytes, arrival_us, request_id);
+}
+
+extern "C" int bulkx_server_submit_host(bulkx_server* s, const bulkx_job* job, double arrival_us,
+                                        int64_t* request_id, int64_t* device_bytes) {
+  if (!s || !job) return BULKX_ERR_INVALID_ARGUMENT;
+  Staging g;
+  if (!staging_of(*job, &g)) return BULKX_ERR_INVALID_ARGUMENT;
+  for (int i = 0; i < 7; ++i)
+    if (g.bytes[i] != 0 && job_ptr(*job, i) == nullptr) return BULKX_ERR_INVALID_ARGUMENT;
+  if (device_bytes) *device_bytes = int64_t(g.total);
+  return submit_impl(s, job, int64_t(g.total), arrival_us < 0 ? -1.0 : arrival_us, request_id,
+                     true);
 }
 
 extern "C" double bulkx_server_now_us(bulkx_server* s) { return s ? s->now_us() : 0.0; }

@@ -37,6 +37,19 @@ class Server:
         self._keep.append(job)
         return rid.value
 
+    def submit_host(self, job: Job, arrival_us: float | None = None) -> tuple[int, int]:
+        """Submit a job over host tensors (pinned for asynchronous copies):
+        staged to the device when admitted, results copied back.  Returns
+        (request id, device bytes charged to the budget)."""
+        rid, nbytes = ctypes.c_int64(), ctypes.c_int64()
+        cj = job.to_c()
+        C.check(C.lib().bulkx_server_submit_host(self._h, ctypes.byref(cj),
+                                                 -1.0 if arrival_us is None else float(arrival_us),
+                                                 ctypes.byref(rid), ctypes.byref(nbytes)),
+                "bulkx_server_submit_host")
+        self._keep.append(job)
+        return rid.value, nbytes.value
+
     def drain(self) -> None:
         C.check(C.lib().bulkx_server_drain(self._h), "bulkx_server_drain")
 

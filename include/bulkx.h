@@ -327,6 +327,19 @@ int bulkx_server_submit(bulkx_server* server, const bulkx_job* job, int64_t byte
  * open-loop arrival process can be replayed exactly. */
 int bulkx_server_submit_at(bulkx_server* server, const bulkx_job* job, int64_t bytes,
                            double arrival_us, int64_t* request_id);
+/* Submit a job whose buffers are in host memory (pinned with hipHostMalloc
+ * for the copies to run asynchronously; pageable memory works but its
+ * copies hold the server's worker).  When the request is admitted the server
+ * allocates its device buffers, copies the inputs in ahead of its batch and
+ * the outputs back behind it (reduce_by_key: all n0 slots of keys and
+ * values, and the count), then frees the buffers, all on the slot's stream;
+ * the request is complete when its results are in host memory.  The host
+ * buffers must stay valid until then.  *device_bytes (if not NULL) receives
+ * the footprint charged to the budget.  arrival_us < 0 means now.  No
+ * counterpart in the reference (its drivers take device ranges only): for
+ * callers whose data starts and ends on the host. */
+int bulkx_server_submit_host(bulkx_server* server, const bulkx_job* job, double arrival_us,
+                             int64_t* request_id, int64_t* device_bytes);
 /* Microseconds elapsed on the server clock. */
 double bulkx_server_now_us(bulkx_server* server);
 /* Block until every submitted request has completed. */

@@ -170,3 +170,45 @@ def test_server_per_call_mode_and_budget_errors(bx):
         srv.submit(jobs[0].job, 1 << 31)
     srv.close()
 
+
+
+def test_host_staged_requests_equal_alone(bx):
+    """Requests over pinned host buffers: staged in and out by the server,
+    under a budget that makes admission wait, each result equal to the same
+    job run alone on the device (floats within the bound)."""
+    from bulk_amd import workload as W
+    from bulk_amd.batch import Job
+    from bulk_amd.serve import Server
+    jobs = _small_stream(num_jobs=64, seed=41)
+    alone = _run_alone(jobs)
+    W.reset_all(jobs)
+    torch.cuda.synchronize()
+    host = []
+    for jd in jobs:
+        ht = {k: v.cpu().pin_memory() for k, v in jd.job.tensors.items()}
+        for name in ("out0", "out1"):  # scribble over outputs so stale results cannot pass
+            if jd.spec.kind != C.JOB_SORT_BY_KEY and name in ht:
+                ht[name].fill_(3)
+        j = jd.job
+        host.append(Job(j.kind, j.dtype, j.value_dtype, j.n0, j.n1, ht))
+    budget = 8 << 20
+    srv = Server(budget, max_inflight=3, max_batch_jobs=16, max_batch_elements=1 << 20)
+    ids, charged = zip(*[srv.submit_host(h) for h in host])
+    srv.drain()
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == len(jobs) and st["peak_bytes"] <= budget and st["batches"] > 1
+    assert all(c > 0 and c % 256 == 0 for c in charged)  # buffers of 256-byte multiples
+    scan = next(i for i, jd in enumerate(jobs) if jd.spec.kind == C.JOB_INCLUSIVE_SCAN)
+    assert charged[scan] == 2 * ((jobs[scan].spec.n * 4 + 255) // 256 * 256)
+
+    def snap(jd, ht):
+        if jd.spec.kind == C.JOB_REDUCE_BY_KEY:
+            m = int(ht["out2"].item())
+            return (m, ht["out0"][:m].cuda(), ht["out1"][:m].cuda())
+        if jd.spec.kind == C.JOB_SORT_BY_KEY:
+            return (ht["out0"].cuda(), ht["out1"].cuda())
+        return tuple(ht[k].cuda() for k in ("out0", "out1") if k in ht)
+
+    bad = [i for i, jd in enumerate(jobs) if not _same(jd, alone[i], snap(jd, host[i].tensors))]
+    assert bad == [], f"host-staged jobs {bad} differ from the device runs"
