@@ -2,6 +2,7 @@
 // the C-ABI of bulkx.h, with the reference's names and argument order:
 //
 //   inclusive_scan(first, last, result, init, op)          scan.cu:87-148
+//   exclusive_scan(first, last, result, init, op)          scan.hpp:553-591
 //   my_reduce(first, last, init, op)                       reduce.cu:58-97
 //   my_merge(first1, last1, first2, last2, result, comp)   merge.cu:127-162
 //   merge_by_key(k1f, k1l, v1f, k2f, k2l, v2f, kout, vout, comp)
@@ -165,6 +166,22 @@ It2 inclusive_scan(It1 first, It1 last, It2 result, T init, Op op) {
   check(bulkx_inclusive_scan(detail::dtype_of<V>::value, detail::op_code<Op>(), detail::raw(first),
                              detail::raw(result), n, &iv, nullptr, 0, default_stream()),
         "inclusive_scan");
+  return result + n;
+}
+
+// exclusive_scan(first, last, result, init, op): result[0] = init,
+// result[i] = init op x[0] op ... op x[i-1] (the device-wide form of
+// bulk::exclusive_scan, scan.hpp:553-591).
+template <class It1, class It2, class T, class Op>
+It2 exclusive_scan(It1 first, It1 last, It2 result, T init, Op op) {
+  using V = detail::value_t<It1>;
+  static_assert(std::is_same<V, detail::value_t<It2>>::value, "input and output types match");
+  static_assert(detail::op_code<Op>() >= 0, "op must be plus, minimum or maximum");
+  const int64_t n = int64_t(last - first);
+  const V iv = V(init);
+  check(bulkx_exclusive_scan(detail::dtype_of<V>::value, detail::op_code<Op>(), detail::raw(first),
+                             detail::raw(result), n, &iv, nullptr, 0, default_stream()),
+        "exclusive_scan");
   return result + n;
 }
 

@@ -21,7 +21,8 @@
 //                                         falling back to device malloc when exhausted
 // Group algorithms (copy_n, reduce, accumulate, inclusive_scan,
 // exclusive_scan, merge_path, merge, merge_by_key, inplace_merge,
-// stable_sort_by_key, reduce_by_key, scatter_if, gather) run on a
+// stable_sort_by_key, reduce_by_key, scatter_if, gather,
+// adjacent_difference) run on a
 // concurrent_group with wavefront shuffles and LDS; the same names on a
 // bounded agent (bulk::bound<N>(self)) are the sequential bodies of one
 // work-item.  head_flags / tail_flags are index-computed flag ranges.
@@ -1328,6 +1329,68 @@ __device__ It3 gather(concurrent_group<Agent, S>& g, It1 map_first, It1 map_last
     result[i] = input_first[map_first[i]];
   g.wait();
   return result + n;
+}
+
+// adjacent_difference with init: result[i] = op(x[i], x[i-1]), x[-1] = init
+// (adjacent_difference.hpp:27-47, 50-107).  Each agent reads its grain and
+// the element before it, the group waits, then writes, so result may equal
+// first; the last element of each tile is carried to the next.  (The
+// reference's group loop hands every agent the tile's last element as the
+// predecessor of its next grain, adjacent_difference.hpp:80, which is right
+// for agent 0 only; this form gives every element its own predecessor.)
+template <class Agent, std::size_t S, class It1, class It2, class T, class Op>
+__device__ It2 adjacent_difference(concurrent_group<Agent, S>& g, It1 first, It1 last, It2 result,
+                                   T init, Op op) {
+  const long long n = last - first;
+  constexpr int G = Agent::static_grainsize;
+  const long long tile = (long long)(g.size()) * G;
+  const int tid = g.this_exec.index();
+  T carry = init;  // element before the tile
+  for (long long base = 0; base < n; base += tile) {
+    const long long lo = base + (long long)(tid)*G;
+    const long long rest = n - lo;
+    const int cnt = rest <= 0 ? 0 : (rest < G ? int(rest) : G);
+    T x[G];
+    T before = carry;
+    if (cnt > 0 && tid > 0) before = T(first[lo - 1]);
+#pragma unroll
+    for (int j = 0; j < G; ++j)
+      if (j < cnt) x[j] = T(first[lo + j]);
+    const long long tend = base + tile < n ? base + tile : n;
+    const T next_carry = T(first[tend - 1]);
+    g.wait();
+#pragma unroll
+    for (int j = 0; j < G; ++j)
+      if (j < cnt) result[lo + j] = op(x[j], j > 0 ? x[j > 0 ? j - 1 : 0] : before);
+    carry = next_carry;
+    g.wait();
+  }
+  return result + n;
+}
+
+// adjacent_difference without init: result[0] = x[0], then as above
+// (adjacent_difference.hpp:110-137).
+template <class Agent, std::size_t S, class It1, class It2, class Op>
+__device__ It2 adjacent_difference(concurrent_group<Agent, S>& g, It1 first, It1 last, It2 result,
+                                   Op op) {
+  if (!(first < last)) return result;
+  typedef typename std::iterator_traits<It1>::value_type T;
+  const T init = first[0];
+  g.wait();  // first may equal result
+  if (g.this_exec.index() == 0) result[0] = init;
+  return bulk::adjacent_difference(g, first + 1, last, result + 1, init, op);
+}
+
+// adjacent_difference of one agent, sequential (adjacent_difference.hpp:27-47).
+template <std::size_t G, class It1, class It2, class T, class Op>
+__device__ It2 adjacent_difference(agent<G>& exec, It1 first, It1 last, It2 result, T init,
+                                   Op op) {
+  for (; first != last; ++first, ++result) {
+    const T x = *first;
+    *result = op(x, init);
+    init = x;
+  }
+  return result;
 }
 
 }  // namespace bulk

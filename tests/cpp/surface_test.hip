@@ -209,6 +209,26 @@ struct group_more {
   }
 };
 
+// adjacent_difference: a group over several tiles, out of place with init and
+// in place without; one agent over its own 16 elements
+struct group_adjdiff {
+  template <std::size_t G, std::size_t S>
+  __device__ void operator()(bulk::concurrent_group<bulk::agent<G>, S>& g, const int* x, int n,
+                             int* d, int* inplace) {
+    auto sub = [](int a, int b) { return a - b; };
+    bulk::adjacent_difference(g, x, x + n, d, 100, sub);
+    bulk::adjacent_difference(g, inplace, inplace + n, inplace, sub);
+  }
+};
+
+struct agent_adjdiff {
+  __device__ void operator()(bulk::agent<>& self, const int* x, int* d) {
+    const int t = self.index();
+    bulk::adjacent_difference(self, x + 16 * t, x + 16 * t + 16, d + 16 * t, -1,
+                              [](int a, int b) { return a - b; });
+  }
+};
+
 int main() {
   // hello: par(32), bulk::root is the group
   {
@@ -485,6 +505,25 @@ int main() {
     CHECK(hfc[0] == heads && hfc[1] == tails && hfc[2] == heads_init,
           "head_flags / tail_flags / head_flags_with_init");
   }
+  // adjacent_difference (group, several tiles; in place; one agent)
+  {
+    const int n = 5003, agents = 40;
+    std::vector<int> x(n);
+    for (int i = 0; i < n; ++i) x[i] = int((i * 2654435761u) % 1999) - 999;
+    dvec<int> dx(x), dd(n), din(x), da(16 * agents);
+    bulk::async(bulk::con<256, 4>(0), group_adjdiff(), bulk::root, dx.p, n, dd.p, din.p).wait();
+    bulk::async(bulk::par(agents), agent_adjdiff(), bulk::root.this_exec, dx.p, da.p).wait();
+    auto hd = dd.get(), hin = din.get(), ha = da.get();
+    bool ok1 = true, ok2 = true, ok3 = true;
+    for (int i = 0; i < n; ++i) {
+      ok1 &= hd[i] == x[i] - (i ? x[i - 1] : 100);
+      ok2 &= hin[i] == (i ? x[i] - x[i - 1] : x[0]);
+    }
+    for (int i = 0; i < 16 * agents; ++i) ok3 &= ha[i] == x[i] - (i % 16 ? x[i - 1] : -1);
+    CHECK(ok1, "group adjacent_difference with init (5 tiles)");
+    CHECK(ok2, "group adjacent_difference in place, no init");
+    CHECK(ok3, "agent adjacent_difference");
+  }
   // driver facade over the C-ABI
   {
     const int n = 1000003;
@@ -496,6 +535,13 @@ int main() {
     int run = 13;
     for (int i = 0; i < n; ++i) ref[i] = (run += h[i]);
     CHECK(r.get() == ref, "bulkx::inclusive_scan");
+    bulkx::exclusive_scan(d.p, d.p + n, r.p, 5, std::plus<int>());
+    run = 5;
+    for (int i = 0; i < n; ++i) {
+      ref[i] = run;
+      run += h[i];
+    }
+    CHECK(r.get() == ref, "bulkx::exclusive_scan");
     const long long sum = std::accumulate(h.begin(), h.end(), 13LL);
     CHECK(bulkx::my_reduce(d.p, d.p + n, 13, std::plus<int>()) == int(sum), "bulkx::my_reduce");
 
