@@ -22,7 +22,7 @@
 // Group algorithms (copy_n, reduce, accumulate, inclusive_scan,
 // exclusive_scan, merge_path, merge, merge_by_key, inplace_merge,
 // stable_sort_by_key, reduce_by_key, scatter_if, gather,
-// adjacent_difference) run on a
+// adjacent_difference, for_each_n) run on a
 // concurrent_group with wavefront shuffles and LDS; the same names on a
 // bounded agent (bulk::bound<N>(self)) are the sequential bodies of one
 // work-item.  head_flags / tail_flags are index-computed flag ranges.
@@ -1391,6 +1391,22 @@ __device__ It2 adjacent_difference(agent<G>& exec, It1 first, It1 last, It2 resu
     init = x;
   }
   return result;
+}
+
+// for_each_n: f(x[i]) for i < n, striped over the group, then a wait
+// (for_each.hpp:28-45); on a bounded agent, sequential (for_each.hpp:48-70).
+template <class Agent, std::size_t S, class It, class Size, class F>
+__device__ It for_each_n(concurrent_group<Agent, S>& g, It first, Size n, F f) {
+  for (Size i = Size(g.this_exec.index()); i < n; i += Size(g.size())) f(first[i]);
+  g.wait();
+  return first + n;
+}
+
+template <std::size_t B, std::size_t G, class It, class Size, class F>
+__device__ It for_each_n(bounded<B, agent<G>>& exec, It first, Size n, F f) {
+  for (int i = 0; i < exec.bound(); ++i)
+    if (Size(i) < n) f(first[i]);
+  return first + n;
 }
 
 }  // namespace bulk

@@ -229,6 +229,23 @@ struct agent_adjdiff {
   }
 };
 
+// for_each_n on a group and on bounded agents: add 1 to every element
+struct add_one {
+  __device__ void operator()(int& x) const { x += 1; }
+};
+struct group_for_each {
+  template <std::size_t G, std::size_t S>
+  __device__ void operator()(bulk::concurrent_group<bulk::agent<G>, S>& g, int* x, int n) {
+    bulk::for_each_n(g, x, n, add_one());
+  }
+};
+struct agent_for_each {
+  __device__ void operator()(bulk::agent<>& self, int* x, int n) {
+    const int lo = self.index() * 4;
+    bulk::for_each_n(bulk::bound<4>(self), x + lo, n - lo < 4 ? n - lo : 4, add_one());
+  }
+};
+
 int main() {
   // hello: par(32), bulk::root is the group
   {
@@ -523,6 +540,19 @@ int main() {
     CHECK(ok1, "group adjacent_difference with init (5 tiles)");
     CHECK(ok2, "group adjacent_difference in place, no init");
     CHECK(ok3, "agent adjacent_difference");
+  }
+  // for_each_n: the group adds 1 to all, then the agents add 1 again
+  {
+    const int n = 3001;
+    std::vector<int> x(n);
+    for (int i = 0; i < n; ++i) x[i] = i * 3;
+    dvec<int> dx(x);
+    bulk::async(bulk::con<256, 4>(0), group_for_each(), bulk::root, dx.p, n).wait();
+    bulk::async(bulk::par((n + 3) / 4), agent_for_each(), bulk::root.this_exec, dx.p, n).wait();
+    auto h = dx.get();
+    bool ok = true;
+    for (int i = 0; i < n; ++i) ok &= h[i] == i * 3 + 2;
+    CHECK(ok, "group and bounded agent for_each_n");
   }
   // driver facade over the C-ABI
   {
