@@ -206,3 +206,48 @@ def test_flags(bx, orc):
     keys = orc.ref_hash_i32(100000, mod10=True)
     assert np.array_equal(_host(bx.tail_flags(_dev(keys))), orc.tail_flags(keys))
     assert np.array_equal(_host(bx.head_flags(_dev(keys))), orc.head_flags(keys))
+
+
+# ------------------------------------------------------- signed zeros
+def _zeros_keys(seed, n):
+    """float32 keys from {-1, -0.0, +0.0, 1}: -0.0 and +0.0 compare equal
+    under less and equal_to, so only the input order separates them."""
+    pool = np.array([-1.0, -0.0, 0.0, 1.0], dtype=np.float32)
+    return pool[np.random.default_rng(seed).integers(0, 4, n)]
+
+
+def _bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+def test_signed_zero_keys_sort_merge_reduce_by_key(bx, orc):
+    """-0.0 and +0.0 are equivalent keys: stable sort and merge keep their
+    input order (bits compared), and reduce_by_key puts them in one run whose
+    output key is the run's last key (reduce_by_key.hpp:87-92)."""
+    n = 100003
+    k = _zeros_keys(1, n)
+    v = np.arange(n, dtype=np.int32)
+    dk, dv = _dev(k), _dev(v)
+    bx.stable_merge_sort_by_key(dk, dv)
+    rk, rv = orc.stable_sort_by_key(k.copy(), v.copy())
+    assert np.array_equal(_bits(_host(dk)), _bits(rk)) and np.array_equal(_host(dv), rv)
+
+    a = np.sort(_zeros_keys(2, 70001), kind="stable")
+    b = np.sort(_zeros_keys(3, 50003), kind="stable")
+    va = np.arange(a.size, dtype=np.int32)
+    vb = (1 << 28) + np.arange(b.size, dtype=np.int32)
+    ko, vo = bx.merge_by_key(_dev(a), _dev(va), _dev(b), _dev(vb))
+    mk, mv = orc.merge_by_key(a, va, b, vb)
+    assert np.array_equal(_bits(_host(ko)), _bits(mk)) and np.array_equal(_host(vo), mv)
+
+    # runs of equal keys built from +-0.0 and +-1 in random lengths
+    runs = np.random.default_rng(4).integers(1, 9, 40000)
+    rkeys = np.repeat(np.where(np.arange(runs.size) % 2 == 0, 0.0, 1.0).astype(np.float32), runs)
+    signs = np.where(np.random.default_rng(5).integers(0, 2, rkeys.size) == 0, -1.0, 1.0)
+    rkeys = (rkeys * signs.astype(np.float32)).astype(np.float32)
+    # adjacent runs of +-1 with different signs are different keys; +-0.0 are one key
+    vals = np.random.default_rng(6).integers(-100, 100, rkeys.size).astype(np.int32)
+    ko, vo, m = bx.my_reduce_by_key(_dev(rkeys), _dev(vals))
+    ek, ev = orc.reduce_by_key(rkeys, vals)
+    assert m == ek.size
+    assert np.array_equal(_bits(_host(ko)[:m]), _bits(ek)) and np.array_equal(_host(vo)[:m], ev)

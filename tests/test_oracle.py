@@ -242,3 +242,28 @@ def test_generator_distributions(orc):
     assert np.all(np.diff(keys) >= 0) and keys[-1] == runs - 1
     z, zr = orc.gen_runs_zipf(0, 1 << 20, 1.2, 1 << 16)
     assert np.all(np.diff(z) >= 0) and z[-1] == zr - 1
+
+
+def test_oracle_signed_zero_keys(orc):
+    """The oracle treats -0.0 and +0.0 as one key: its stable sort equals
+    numpy's stable argsort (which compares with <), and its reduce_by_key
+    writes each run's last key, bit for bit, as a plain loop does."""
+    rng = np.random.default_rng(7)
+    k = np.array([-1.0, -0.0, 0.0, 1.0], dtype=np.float32)[rng.integers(0, 4, 5000)]
+    v = np.arange(k.size, dtype=np.int32)
+    sk, sv = orc.stable_sort_by_key(k.copy(), v.copy())
+    order = np.argsort(k, kind="stable")
+    assert np.array_equal(sk.view(np.uint32), k[order].view(np.uint32))
+    assert np.array_equal(sv, v[order])
+    vals = rng.integers(-9, 9, k.size).astype(np.int32)
+    ek, ev = orc.reduce_by_key(k, vals)
+    tk, tv = [], []
+    for i in range(k.size):
+        if i > 0 and k[i] == k[i - 1]:
+            tk[-1] = k[i]
+            tv[-1] += int(vals[i])
+        else:
+            tk.append(k[i])
+            tv.append(int(vals[i]))
+    assert np.array_equal(ek.view(np.uint32), np.array(tk, dtype=np.float32).view(np.uint32))
+    assert np.array_equal(ev, np.array(tv, dtype=np.int32))
