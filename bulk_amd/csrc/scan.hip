@@ -131,6 +131,12 @@ static int scan_entry(int dtype, int op, const void* in, void* out, int64_t n, c
         case 1: st = scan_launch<T, OP, EXCLUSIVE, 1>(i, o, n, ini, scratch.ptr(), stream); break;
         case 2: st = scan_launch<T, OP, EXCLUSIVE, 2>(i, o, n, ini, scratch.ptr(), stream); break;
         case 3: st = scan_launch<T, OP, EXCLUSIVE, 3>(i, o, n, ini, scratch.ptr(), stream); break;
+        case 4:
+          if constexpr (ScanConfig<T, 4>::kWide)
+            st = scan_launch<T, OP, EXCLUSIVE, 4>(i, o, n, ini, scratch.ptr(), stream);
+          else
+            done = false;
+          break;
         default: done = false; break;
       }
     }

@@ -116,7 +116,8 @@ int size_class(int64_t n) {
   if (n < (int64_t(1) << 16)) return 0;
   if (n < (int64_t(1) << 20)) return 1;
   if (n < (int64_t(1) << 24)) return 2;
-  return 3;
+  if (n < (int64_t(1) << 26)) return 3;
+  return 4;
 }
 
 int num_variants(int algo) {
@@ -161,6 +162,12 @@ int variant_shape(int algo, int dtype, int64_t n, int v, LaunchShape* out) {
         BULKX_SCAN_SHAPE(1)
         BULKX_SCAN_SHAPE(2)
         BULKX_SCAN_SHAPE(3)
+        case 4:  // 4-byte integers only; the others run variant 0's shape
+          s.group_size = 1024;
+          s.grain_size = (dtype == BULKX_INT32 || dtype == BULKX_UINT32)
+                             ? ScanConfig<int32_t, 4>::ITEMS
+                             : es == 4 ? ScanConfig<float, 0>::ITEMS : ScanConfig<double, 0>::ITEMS;
+          break;
 #undef BULKX_SCAN_SHAPE
       }
       const int64_t tile = int64_t(s.group_size) * s.grain_size;

@@ -14,7 +14,7 @@ struct LaunchShape {
 };
 
 // Size classes of the tuning table: n < 2^16, < 2^20, < 2^24, >= 2^24.
-constexpr int kSizeClasses = 4;
+constexpr int kSizeClasses = 5;
 int size_class(int64_t n);
 
 // Number of tile-shape variants `algo` is compiled with.

@@ -114,7 +114,7 @@ int bulkx_load_tuning(const char* path, int* entries);
  * sizes are constants in each driver): every algorithm is compiled with
  * bulkx_tuning_variants() tile shapes; the table maps (algo, dtype, size
  * class) to one of them.  Size classes: 0: n < 2^16, 1: < 2^20, 2: < 2^24,
- * 3: >= 2^24.  The dtype key is the key/element dtype, and the value dtype
+ * 3: < 2^26, 4: >= 2^26.  The dtype key is the key/element dtype, and the value dtype
  * for reduce_by_key.  variant = -1 removes the entry (built-in default).
  */
 int bulkx_tuning_variants(int algo, int* count);

@@ -70,7 +70,7 @@ def test_workspace_size_query_is_host_only():
         assert lib.bulkx_workspace_size(C.ALGO_INCLUSIVE_SCAN, dt, -1, n, 0, ctypes.byref(nb)) == 0
         tiles = []
         for v in range(nvar.value):
-            for sc in range(4):
+            for sc in range(5):
                 assert lib.bulkx_set_tuning(C.ALGO_INCLUSIVE_SCAN, dt, sc, v) == 0
             assert lib.bulkx_choose_sizes(C.ALGO_INCLUSIVE_SCAN, dt, n, ctypes.byref(ng),
                                           ctypes.byref(gs), ctypes.byref(gr)) == 0
@@ -107,8 +107,8 @@ def test_tuning_table_round_trip(tmp_path):
     for v in range(nvar.value):
         path = tmp_path / f"t{v}.json"
         path.write_text('{"device": "x", "entries": ['
-                        f'{{"algo": {C.ALGO_REDUCE_BY_KEY}, "dtype": {C.FLOAT32}, "size_class": 3, "variant": {v}}},'
-                        f'{{"algo": {C.ALGO_REDUCE_BY_KEY}, "dtype": {C.FLOAT32}, "size_class": 3, "variant": 99}},'
+                        f'{{"algo": {C.ALGO_REDUCE_BY_KEY}, "dtype": {C.FLOAT32}, "size_class": 4, "variant": {v}}},'
+                        f'{{"algo": {C.ALGO_REDUCE_BY_KEY}, "dtype": {C.FLOAT32}, "size_class": 4, "variant": 99}},'
                         '{"algo": 42, "dtype": 0, "size_class": 0, "variant": 0}]}')
         entries = ctypes.c_int()
         assert lib.bulkx_load_tuning(str(path).encode(), ctypes.byref(entries)) == 0
@@ -119,7 +119,7 @@ def test_tuning_table_round_trip(tmp_path):
         assert ng.value == (1 << 26) // (gs.value * gr.value)
     assert {v: shapes[v] for v in range(6)} == {0: (512, 16), 1: (512, 8), 2: (256, 16),
                                                  3: (1024, 16), 4: (1024, 8), 5: (256, 32)}
-    assert lib.bulkx_set_tuning(C.ALGO_SCAN if hasattr(C, "ALGO_SCAN") else 0, 0, 4, 0) == 1
+    assert lib.bulkx_set_tuning(C.ALGO_SCAN if hasattr(C, "ALGO_SCAN") else 0, 0, 5, 0) == 1  # size classes 0..4
     assert lib.bulkx_set_tuning(C.ALGO_REDUCE_BY_KEY, C.FLOAT32, 0, nvar.value) == 1
     assert lib.bulkx_load_tuning(str(tmp_path / "missing.json").encode(), None) == 1
     assert lib.bulkx_load_tuning(None, None) == 0

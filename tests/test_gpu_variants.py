@@ -27,7 +27,7 @@ def _variants(algo):
 @pytest.fixture
 def force(bx):
     def f(algo, dtype, v):
-        for sc in range(4):
+        for sc in range(5):
             C.check(C.lib().bulkx_set_tuning(algo, dtype, sc, v), "bulkx_set_tuning")
     yield f
     C.check(C.lib().bulkx_load_tuning(None, None), "bulkx_load_tuning")
@@ -36,14 +36,16 @@ def force(bx):
 SIZES = [1, 4095, 4096, 4097, 8193, 16385, 100003, 1 << 20]
 
 
-@pytest.mark.parametrize("v", [0, 1, 2, 3])
+@pytest.mark.parametrize("v", [0, 1, 2, 3, 4])
 def test_scan_variants(force, orc, v):
+    """Variant 4 (65536-element tiles for 4-byte integers; float64 runs the
+    default shape under it)."""
     from bulk_amd import api
-    assert _variants(C.ALGO_INCLUSIVE_SCAN) == 4
+    assert _variants(C.ALGO_INCLUSIVE_SCAN) == 5
     force(C.ALGO_INCLUSIVE_SCAN, C.INT32, v)
     force(C.ALGO_EXCLUSIVE_SCAN, C.INT32, v)
     force(C.ALGO_INCLUSIVE_SCAN, C.FLOAT64, v)
-    for n in SIZES + [(1 << 20) + 12345]:
+    for n in SIZES + [65535, 65536, 65537, 3 * 65536 + 1, (1 << 20) + 12345]:
         x = orc.gen_uniform_i32(n, -1000, 1000, n)
         d = _dev(x)
         assert np.array_equal(api.inclusive_scan(d, init=3).cpu().numpy(), orc.inclusive_scan(x, init=3)), n
