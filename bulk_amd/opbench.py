@@ -223,3 +223,13 @@ def run(iters: int = 10, log=print, large: int = 28) -> list[dict]:
     for r in rows:
         log(f"{r['op']:46s} {r['ms'] * 1e3:9.1f} us {r['gbs']:7.0f} GB/s {100 * r['frac_peak']:5.1f}%")
     return rows
+
+
+if __name__ == "__main__":
+    import argparse
+
+    ap = argparse.ArgumentParser(description="single-op table at full size (one MI355X)")
+    ap.add_argument("--iters", type=int, default=10)
+    ap.add_argument("--large", type=int, default=28)
+    a = ap.parse_args()
+    run(iters=a.iters, large=a.large)
