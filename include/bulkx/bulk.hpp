@@ -25,7 +25,8 @@
 // adjacent_difference, for_each_n) run on a
 // concurrent_group with wavefront shuffles and LDS; the same names on a
 // bounded agent (bulk::bound<N>(self)) are the sequential bodies of one
-// work-item.  head_flags / tail_flags are index-computed flag ranges.
+// work-item.  head_flags / tail_flags are index-computed flag ranges and
+// join_iterator views two ranges as one.
 // Contracts: bulk/algorithm/*.hpp of the reference.
 #pragma once
 
@@ -1407,6 +1408,31 @@ __device__ It for_each_n(bounded<B, agent<G>>& exec, It first, Size n, F f) {
   for (int i = 0; i < exec.bound(); ++i)
     if (Size(i) < n) f(first[i]);
   return first + n;
+}
+
+// join_iterator: two ranges viewed as one, element i being first1[i] for
+// i < n1 and first2[i - n1] after (join_iterator.hpp:47-100); read-only.
+namespace detail {
+template <class It1, class It2, class Difference>
+struct join_at {
+  It1 first1;
+  Difference n1;
+  It2 first2;
+  __host__ __device__ typename std::iterator_traits<It1>::value_type operator()(Difference i) const {
+    return i < n1 ? first1[i] : typename std::iterator_traits<It1>::value_type(first2[i - n1]);
+  }
+};
+}  // namespace detail
+
+template <class It1, class It2,
+          class Difference = typename std::iterator_traits<It1>::difference_type>
+using join_iterator = detail::index_iterator<detail::join_at<It1, It2, Difference>,
+                                             typename std::iterator_traits<It1>::value_type,
+                                             Difference>;
+
+template <class It1, class Size, class It2>
+__host__ __device__ join_iterator<It1, It2, Size> make_join_iterator(It1 first1, Size n1, It2 first2) {
+  return join_iterator<It1, It2, Size>(detail::join_at<It1, It2, Size>{first1, n1, first2}, Size(0));
 }
 
 }  // namespace bulk

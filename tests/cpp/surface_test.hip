@@ -246,6 +246,18 @@ struct agent_for_each {
   }
 };
 
+// join_iterator: a group copies and reduces two ranges viewed as one
+struct group_join {
+  template <std::size_t G, std::size_t S>
+  __device__ void operator()(bulk::concurrent_group<bulk::agent<G>, S>& g, const int* a, int na,
+                             const int* b, int nb, int* out, int* sum) {
+    auto j = bulk::make_join_iterator(a, na, b);
+    bulk::copy_n(g, j, na + nb, out);
+    const int s = bulk::reduce(g, j, j + (na + nb), 0, add_op());
+    if (g.this_exec.index() == 0) *sum = s;
+  }
+};
+
 int main() {
   // hello: par(32), bulk::root is the group
   {
@@ -553,6 +565,20 @@ int main() {
     bool ok = true;
     for (int i = 0; i < n; ++i) ok &= h[i] == i * 3 + 2;
     CHECK(ok, "group and bounded agent for_each_n");
+  }
+  // join_iterator over two ranges
+  {
+    std::vector<int> a(1234), b(777);
+    for (size_t i = 0; i < a.size(); ++i) a[i] = int(i) * 2 + 1;
+    for (size_t i = 0; i < b.size(); ++i) b[i] = -int(i);
+    dvec<int> da(a), db(b), dout(a.size() + b.size()), dsum(1);
+    bulk::async(bulk::con<256, 4>(0), group_join(), bulk::root, da.p, int(a.size()), db.p,
+                int(b.size()), dout.p, dsum.p)
+        .wait();
+    std::vector<int> ref(a);
+    ref.insert(ref.end(), b.begin(), b.end());
+    CHECK(dout.get() == ref && dsum.get()[0] == std::accumulate(ref.begin(), ref.end(), 0),
+          "join_iterator (copy_n and reduce over two ranges)");
   }
   // driver facade over the C-ABI
   {
