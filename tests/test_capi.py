@@ -48,6 +48,17 @@ def test_invalid_arguments_rejected_before_launch(call):
     assert status in (1, 2, 3)
 
 
+def test_server_submit_host_rejects_bad_arguments():
+    """Host-staged submission validates before touching a server or device."""
+    lib = C.lib()
+    rid, nb = ctypes.c_int64(), ctypes.c_int64()
+    job = C.Job(0, 0, -1, 0, 16, 0, None, None, None, None, None, None, None)
+    assert lib.bulkx_server_submit_host(None, ctypes.byref(job), -1.0, ctypes.byref(rid),
+                                        ctypes.byref(nb)) == 1
+    assert lib.bulkx_server_submit_host(ctypes.c_void_p(1), None, -1.0, ctypes.byref(rid),
+                                        ctypes.byref(nb)) == 1
+
+
 def test_zero_length_calls_are_noops():
     lib = C.lib()
     assert lib.bulkx_inclusive_scan(0, 0, None, None, 0, None, None, 0, None) == 0
