@@ -401,6 +401,27 @@ def main():
         del kept
         _log(f"cpu baseline on {threads} threads {cpu['all_cores']['value']:.1f} Melem/s")
 
+        # The plain int32 inclusive scan of 2^24 elements (uniform [-1000, 1000],
+        # the single-op table's input) through the sequential body on one
+        # core, beside the GPU kernel's result, which must equal it bit for bit.
+        from bulk_amd import api as A
+        g1 = torch.Generator(device=dev)
+        g1.manual_seed(0)
+        x1 = torch.randint(-1000, 1001, (1 << 24,), generator=g1, device=dev, dtype=torch.int32)
+        gpu1 = A.inclusive_scan(x1).cpu().numpy()
+        h1 = x1.cpu().numpy()
+        ts = []
+        for _ in range(6):
+            a = time.perf_counter()
+            ref1 = O.inclusive_scan(h1)
+            ts.append(time.perf_counter() - a)
+        t1 = float(np.median(ts[1:]))
+        cpu["scan_int32_2^24"] = {"ms": t1 * 1e3, "melem_per_s": (1 << 24) / t1 / 1e6, "cores": 1,
+                                  "gpu_equals_cpu": bool(np.array_equal(gpu1, ref1))}
+        del x1, gpu1, h1, ref1
+        _log(f"cpu int32 scan 2^24: {t1 * 1e3:.2f} ms, GPU result equal: "
+             f"{cpu['scan_int32_2^24']['gpu_equals_cpu']}")
+
     # ------------------------------------------------- single ops at full size
     ops = None
     if not args.no_ops:
