@@ -27,8 +27,9 @@ STATUS_NAMES = {
     0: "BULKX_SUCCESS", 1: "BULKX_ERR_INVALID_ARGUMENT", 2: "BULKX_ERR_UNSUPPORTED_DTYPE",
     3: "BULKX_ERR_UNSUPPORTED_OP", 4: "BULKX_ERR_WORKSPACE_TOO_SMALL", 5: "BULKX_ERR_NO_DEVICE",
     6: "BULKX_ERR_HIP", 7: "BULKX_ERR_OUT_OF_MEMORY", 8: "BULKX_ERR_BUDGET",
-    9: "BULKX_ERR_NOT_READY", 10: "BULKX_ERR_INTERNAL",
+    9: "BULKX_ERR_NOT_READY", 10: "BULKX_ERR_INTERNAL", 11: "BULKX_ERR_LOOKBACK",
 }
+ERR_LOOKBACK = 11
 
 
 class BulkLibraryMissing(RuntimeError):
@@ -66,6 +67,8 @@ SIGNATURES = {
     "bulkx_last_error": (ctypes.c_char_p, []),
     "bulkx_version": (ctypes.c_char_p, []),
     "bulkx_dtype_size": (_S, [_I]),
+    "bulkx_stream_status": (_I, [_P]),
+    "bulkx_set_lookback_spin_limit": (_I, [ctypes.c_uint32]),
     "bulkx_device_info": (_I, [ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I),
                                ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_longlong)]),
     "bulkx_choose_sizes": (_I, [_I, _I, _L, ctypes.POINTER(_L), ctypes.POINTER(_I),
@@ -89,6 +92,8 @@ SIGNATURES = {
     "bulkx_batch_set_jobs": (_I, [_P, ctypes.POINTER(Job), _I]),
     "bulkx_batch_launch": (_I, [_P, _I]),
     "bulkx_batch_launch_per_call": (_I, [_P]),
+    "bulkx_batch_status": (_I, [_P]),
+    "bulkx_batch_reserve": (_I, [_P, _L]),
     "bulkx_batch_last_timing": (_I, [_P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
     "bulkx_batch_stage_timing": (_I, [_P, ctypes.POINTER(ctypes.c_float), _I]),
     "bulkx_batch_sort_geometry": (_I, [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),

@@ -204,7 +204,9 @@ def my_reduce_by_key(keys, values, keys_result=None, values_result=None, pred="e
         values_result = torch.empty(max(n, 1), dtype=values.dtype, device=values.device)
     cnt = torch.zeros(1, dtype=torch.int64, device=keys.device)
     reduce_by_key_async(keys, values, keys_result, values_result, cnt, pred, op, workspace)
-    return keys_result, values_result, int(cnt.item())
+    m = int(cnt.item())
+    check_stream()  # the call blocks anyway: a look-back give-up raises here
+    return keys_result, values_result, m
 
 
 def reduce_intervals(first: torch.Tensor, num_partitions: int, tile_size: int,
@@ -234,6 +236,24 @@ def head_flags(keys: torch.Tensor, pred: str = "equal_to", stream=None) -> torch
     C.check(C.lib().bulkx_head_flags(dtype_code(keys), _PREDS[pred], _ptr(keys), keys.numel(),
                                      _ptr(out), _stream(stream)), "bulkx_head_flags")
     return out
+
+
+def stream_status(stream=None) -> int:
+    """Waits for `stream` and returns BULKX_ERR_LOOKBACK (11) if a look-back
+    wait on this device gave up since the last query (its launch's results
+    are wrong), else 0 (bulkx_stream_status)."""
+    return int(C.lib().bulkx_stream_status(_stream(stream)))
+
+
+def check_stream(stream=None) -> None:
+    """stream_status, raised as BulkError when non-zero."""
+    C.check(stream_status(stream), "bulkx_stream_status")
+
+
+def set_lookback_spin_limit(polls: int) -> None:
+    """Polls a look-back wait makes before giving up (0: give up at once;
+    a knob for tests of the error path)."""
+    C.check(C.lib().bulkx_set_lookback_spin_limit(int(polls)), "bulkx_set_lookback_spin_limit")
 
 
 def device_info() -> dict:

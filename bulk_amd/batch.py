@@ -85,6 +85,15 @@ class Batch:
     def launch_per_call(self) -> None:
         C.check(C.lib().bulkx_batch_launch_per_call(self._h), "bulkx_batch_launch_per_call")
 
+    def status(self) -> int:
+        """Waits for the last launch; BULKX_ERR_LOOKBACK (11) if one of its
+        look-back waits gave up, else 0."""
+        return int(C.lib().bulkx_batch_status(self._h))
+
+    def reserve(self, elements: int) -> None:
+        """Allocate now the scratch job lists of up to `elements` elements need."""
+        C.check(C.lib().bulkx_batch_reserve(self._h, int(elements)), "bulkx_batch_reserve")
+
     def last_timing(self) -> tuple[float, float]:
         """(phase-0 ms, sort-merge-rounds ms) of the last eager launch, from
         HIP events recorded on the batch's stream."""

@@ -105,7 +105,7 @@ struct BatchDesc {
   int64_t total_sort_tiles;
   int64_t num_status_words;
   int64_t num_jobs;
-  uint32_t err;
+  LookbackGuard guard;  // the look-back waits of every family; err cleared per launch
 };
 
 template <int FAM>
@@ -153,7 +153,7 @@ __global__ void batch_clear_kernel(BatchDesc* d, uint64_t* status, uint32_t* job
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nw; i += gs) status[i] = 0;
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < d->num_jobs; i += gs)
     job_ctr[i] = 0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) d->err = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) d->guard.err = 0;
 }
 
 template <class K, class SM>
@@ -184,20 +184,20 @@ __device__ __forceinline__ void run_tile(FamilySmem<FAM>& sm, BatchDesc* d, cons
     if (job.op == OP_SCAN_I32)
       scan_tile<int32_t, OpPlus, kBatchBlock, kBatchScanItems, false>(
           sm.s, static_cast<const int32_t*>(job.in0), static_cast<int32_t*>(job.out0), job.n0, 0,
-          false, lt, job.status, &d->err, single, nullptr);
+          false, lt, job.status, &d->guard, single, nullptr);
     else
       scan_tile<int32_t, OpPlus, kBatchBlock, kBatchScanItems, false, false>(
           sm.s, static_cast<const int32_t*>(job.in0), nullptr, job.n0, 0, false, lt, job.status,
-          &d->err, single, static_cast<int32_t*>(job.out0));
+          &d->guard, single, static_cast<int32_t*>(job.out0));
   } else if constexpr (FAM == FAM_SCAN_F32) {
     if (job.op == OP_SCAN_F32)
       scan_tile<float, OpPlus, kBatchBlock, kBatchScanItems, false>(
           sm.s, static_cast<const float*>(job.in0), static_cast<float*>(job.out0), job.n0, 0.f,
-          false, lt, job.status, &d->err, single, nullptr);
+          false, lt, job.status, &d->guard, single, nullptr);
     else
       scan_tile<float, OpPlus, kBatchBlock, kBatchScanItems, false, false>(
           sm.s, static_cast<const float*>(job.in0), nullptr, job.n0, 0.f, false, lt, job.status,
-          &d->err, single, static_cast<float*>(job.out0));
+          &d->guard, single, static_cast<float*>(job.out0));
   } else if constexpr (FAM == FAM_RBK_I32 || FAM == FAM_RBK_F32 || FAM == FAM_RBK_I64) {
     // the third pass: the tile's exclusive prefix was scanned by
     // batch_rbk_scan_kernel into the job's status words
@@ -205,7 +205,7 @@ __device__ __forceinline__ void run_tile(FamilySmem<FAM>& sm, BatchDesc* d, cons
     rbk_tile<int32_t, V, OpPlus, kBatchBlock, kBatchRbkItems>(
         sm.s, static_cast<const int32_t*>(job.in0), static_cast<const V*>(job.in1), job.n0,
         static_cast<int32_t*>(job.out0), static_cast<V*>(job.out1),
-        static_cast<int64_t*>(job.out2), lt, nullptr, &d->err, single,
+        static_cast<int64_t*>(job.out2), lt, nullptr, &d->guard, single,
         single ? nullptr : reinterpret_cast<const RbkAgg<FAM>*>(job.status));
   } else if constexpr (FAM == FAM_MERGE) {
     if (job.op == OP_MERGE_I32)
@@ -460,6 +460,31 @@ static int host_family(int op) {
   }
 }
 
+}  // namespace bulkx
+
+// Whether the batch can run job j (its kind, dtypes and pointers); the same
+// check bulkx_batch_set_jobs applies to every job of a list.
+int bulkx::batch_check_job(const bulkx_job& j) {
+  const int op = batch_op(j);
+  if (op == OP_NONE) return BULKX_ERR_UNSUPPORTED_DTYPE;
+  if (j.n0 < 0 || j.n1 < 0) return BULKX_ERR_INVALID_ARGUMENT;
+  const int64_t n = j.n0 + (op == OP_MERGE_I32 || op == OP_MERGE_F32 ? j.n1 : 0);
+  if (n == 0 && j.kind != BULKX_JOB_REDUCE && j.kind != BULKX_JOB_REDUCE_BY_KEY)
+    return BULKX_SUCCESS;
+  const bool rbk = j.kind == BULKX_JOB_REDUCE_BY_KEY;
+  if (!j.out0 || (rbk && (!j.out1 || !j.out2)) ||
+      (n > 0 && !j.in0 && j.kind != BULKX_JOB_SORT_BY_KEY))
+    return BULKX_ERR_INVALID_ARGUMENT;
+  if (rbk && n > 0 && !j.in1) return BULKX_ERR_INVALID_ARGUMENT;
+  if (j.kind == BULKX_JOB_MERGE_BY_KEY &&
+      ((j.n0 > 0 && !j.in1) || (j.n1 > 0 && (!j.in2 || !j.in3)) || !j.out1))
+    return BULKX_ERR_INVALID_ARGUMENT;
+  if (j.kind == BULKX_JOB_SORT_BY_KEY && !j.out1) return BULKX_ERR_INVALID_ARGUMENT;
+  return BULKX_SUCCESS;
+}
+
+namespace bulkx {
+
 // Workspace one plain entry point needs for job j.
 static size_t per_call_workspace(const bulkx_job& j) {
   switch (j.kind) {
@@ -519,6 +544,8 @@ struct bulkx_batch {
   uint32_t* h_tile_job = nullptr;
   uint32_t* h_sort_tile_job = nullptr;
   uint32_t* h_rbk_jobs = nullptr;
+  uint32_t* h_err = nullptr;  // the last launch's error word, copied back behind it
+  bool last_per_call = false;  // the last launch ran the plain entry points
   int64_t rbk_jobs[kRbkFamilies] = {};  // multi-tile jobs of each reduce_by_key family
   // current job list
   std::vector<bulkx_job> jobs;
@@ -711,11 +738,14 @@ extern "C" int bulkx_batch_create(int max_jobs, int64_t max_elements, void* stre
   hmalloc(reinterpret_cast<void**>(&b->h_tile_job), sizeof(uint32_t) * b->max_tiles);
   hmalloc(reinterpret_cast<void**>(&b->h_sort_tile_job), sizeof(uint32_t) * b->max_tiles);
   hmalloc(reinterpret_cast<void**>(&b->h_rbk_jobs), sizeof(uint32_t) * max_jobs);
+  hmalloc(reinterpret_cast<void**>(&b->h_err), sizeof(uint32_t));
   if (st != BULKX_SUCCESS) {
     bulkx_batch_destroy(b);
     return st;
   }
   memset(b->h_desc, 0, sizeof(BatchDesc));
+  b->h_desc->guard.spin_limit = host_spin_limit();
+  *b->h_err = 0;
   hipError_t r = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking);
   for (int i = 0; i <= kBatchStages && r == hipSuccess; ++i) r = hipEventCreate(&b->ev[i]);
   for (int i = 0; i <= 2 * kBatchMaxPasses && r == hipSuccess; ++i) r = hipEventCreate(&b->rev[i]);
@@ -762,10 +792,50 @@ extern "C" int bulkx_batch_destroy(bulkx_batch* b) {
     if (p) (void)hipFree(p);
   for (void* p : {static_cast<void*>(b->h_desc), static_cast<void*>(b->h_jobs),
                   static_cast<void*>(b->h_tile_job), static_cast<void*>(b->h_sort_tile_job),
-                  static_cast<void*>(b->h_rbk_jobs)})
+                  static_cast<void*>(b->h_rbk_jobs), static_cast<void*>(b->h_err)})
     if (p) (void)hipHostFree(p);
   delete b;
   return BULKX_SUCCESS;
+}
+
+// Grows the sort ping-pong space and the per-call workspace to at least the
+// given sizes (never inside a launch: the caller has synchronised the stream).
+static int batch_grow(bulkx_batch* b, size_t tmp_bytes, size_t ws_need) {
+  if (tmp_bytes > b->tmp_bytes_cap) {
+    if (b->d_tmp) BULKX_HIP(hipFree(b->d_tmp));
+    b->d_tmp = nullptr;
+    b->tmp_bytes_cap = 0;
+    const size_t want = tmp_bytes + tmp_bytes / 4;
+    BULKX_HIP(hipMalloc(reinterpret_cast<void**>(&b->d_tmp), want));
+    b->tmp_bytes_cap = want;
+  }
+  if (ws_need > b->ws_bytes) {
+    if (b->d_ws) BULKX_HIP(hipFree(b->d_ws));
+    b->d_ws = nullptr;
+    b->ws_bytes = 0;
+    BULKX_HIP(hipMalloc(reinterpret_cast<void**>(&b->d_ws), ws_need));
+    b->ws_bytes = ws_need;
+  }
+  return BULKX_SUCCESS;
+}
+
+extern "C" int bulkx_batch_reserve(bulkx_batch* b, int64_t elements) {
+  if (!b || elements < 0 || elements > b->max_elements) return BULKX_ERR_INVALID_ARGUMENT;
+  // sort jobs of `elements` elements in all, each with two 256-byte aligned
+  // arrays of keys and of values
+  const size_t tmp = 2 * align_up(size_t(elements) * 4, 256) + size_t(b->max_jobs) * 1024;
+  size_t ws = 256;
+  const int64_t n = elements;
+  for (int dt : {int(BULKX_INT32), int(BULKX_FLOAT32)}) {
+    ws = std::max(ws, scan_workspace_bytes(dt, n));
+    ws = std::max(ws, reduce_workspace_bytes(dt, n));
+    ws = std::max(ws, sort_workspace_bytes(dt, BULKX_UINT32, n));
+    ws = std::max(ws, merge_workspace_bytes(dt, BULKX_UINT32, n, 0));
+  }
+  for (int vt : {int(BULKX_INT32), int(BULKX_FLOAT32), int(BULKX_INT64)})
+    ws = std::max(ws, reduce_by_key_workspace_bytes(BULKX_INT32, vt, n));
+  BULKX_HIP(hipStreamSynchronize(b->stream));
+  return batch_grow(b, tmp, ws);
 }
 
 extern "C" int bulkx_batch_set_jobs(bulkx_batch* b, const bulkx_job* jobs, int num_jobs) {
@@ -776,17 +846,12 @@ extern "C" int bulkx_batch_set_jobs(bulkx_batch* b, const bulkx_job* jobs, int n
   size_t status_words = 0, tmp_bytes = 0, ws_need = 256;
   for (int i = 0; i < num_jobs; ++i) {
     const bulkx_job& j = jobs[i];
+    BULKX_TRY(batch_check_job(j));
     const int op = batch_op(j);
-    if (op == OP_NONE) return BULKX_ERR_UNSUPPORTED_DTYPE;
-    if (j.n0 < 0 || j.n1 < 0) return BULKX_ERR_INVALID_ARGUMENT;
     const int64_t n = j.n0 + (op == OP_MERGE_I32 || op == OP_MERGE_F32 ? j.n1 : 0);
     elements += n;
     ws_need = std::max(ws_need, per_call_workspace(j));
     if (n == 0 && j.kind != BULKX_JOB_REDUCE && j.kind != BULKX_JOB_REDUCE_BY_KEY) continue;
-    const bool needs_out2 = j.kind == BULKX_JOB_REDUCE_BY_KEY;
-    if (!j.out0 || (needs_out2 && (!j.out1 || !j.out2)) ||
-        (n > 0 && !j.in0 && j.kind != BULKX_JOB_SORT_BY_KEY))
-      return BULKX_ERR_INVALID_ARGUMENT;
     int64_t t;
     switch (op) {
       case OP_SCAN_I32: case OP_SCAN_F32: case OP_REDUCE_I32: case OP_REDUCE_F32:
@@ -815,21 +880,7 @@ extern "C" int bulkx_batch_set_jobs(bulkx_batch* b, const bulkx_job* jobs, int n
 
   // The staging buffers may still feed the previous upload.
   BULKX_HIP(hipStreamSynchronize(b->stream));
-  if (tmp_bytes > b->tmp_bytes_cap) {  // grow the sort ping-pong space (never inside a launch)
-    if (b->d_tmp) BULKX_HIP(hipFree(b->d_tmp));
-    b->d_tmp = nullptr;
-    b->tmp_bytes_cap = 0;
-    const size_t want = tmp_bytes + tmp_bytes / 4;
-    BULKX_HIP(hipMalloc(reinterpret_cast<void**>(&b->d_tmp), want));
-    b->tmp_bytes_cap = want;
-  }
-  if (ws_need > b->ws_bytes) {  // grow the workspace of the per-call path
-    if (b->d_ws) BULKX_HIP(hipFree(b->d_ws));
-    b->d_ws = nullptr;
-    b->ws_bytes = 0;
-    BULKX_HIP(hipMalloc(reinterpret_cast<void**>(&b->d_ws), ws_need));
-    b->ws_bytes = ws_need;
-  }
+  BULKX_TRY(batch_grow(b, tmp_bytes, ws_need));
   b->jobs.assign(jobs, jobs + num_jobs);
   b->num_jobs = num_jobs;
   // Phase-0 tiles are grouped by family, jobs in list order within a family.
@@ -855,6 +906,7 @@ extern "C" int bulkx_batch_set_jobs(bulkx_batch* b, const bulkx_job* jobs, int n
   }
   int64_t cursor[FAM_COUNT];
   memset(b->h_desc, 0, sizeof(BatchDesc));
+  b->h_desc->guard.spin_limit = host_spin_limit();
   b->h_desc->fam_begin[0] = 0;
   for (int f = 0; f < FAM_COUNT; ++f) {
     cursor[f] = b->h_desc->fam_begin[f];
@@ -1024,11 +1076,20 @@ static int batch_capture(bulkx_batch* b, BatchGraph& G) {
   return BULKX_SUCCESS;
 }
 
+// The launch's error word into pinned host memory, behind the launch.
+static int batch_copy_err(bulkx_batch* b) {
+  BULKX_HIP(hipMemcpyAsync(b->h_err, &b->d_desc->guard.err, sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, b->stream));
+  return BULKX_SUCCESS;
+}
+
 extern "C" int bulkx_batch_launch(bulkx_batch* b, int use_graph) {
   if (!b) return BULKX_ERR_INVALID_ARGUMENT;
+  b->last_per_call = false;
   if (!use_graph) {
     b->timed = true;
-    return batch_enqueue_kernels(b, b->stream, true);
+    BULKX_TRY(batch_enqueue_kernels(b, b->stream, true));
+    return batch_copy_err(b);
   }
   BatchGraph& G = b->graphs[b->rounds];
   if (G.exec) {  // resize the job-list-sized grids in place
@@ -1058,11 +1119,20 @@ extern "C" int bulkx_batch_launch(bulkx_batch* b, int use_graph) {
   }
   if (!G.exec) BULKX_TRY(batch_capture(b, G));
   BULKX_HIP(hipGraphLaunch(G.exec, b->stream));
-  return BULKX_SUCCESS;
+  return batch_copy_err(b);
+}
+
+extern "C" int bulkx_batch_status(bulkx_batch* b) {
+  if (!b) return BULKX_ERR_INVALID_ARGUMENT;
+  // the plain entry points report through the device's guard
+  if (b->last_per_call) return bulkx_stream_status(b->stream);
+  BULKX_HIP(hipStreamSynchronize(b->stream));
+  return *b->h_err ? BULKX_ERR_LOOKBACK : BULKX_SUCCESS;
 }
 
 extern "C" int bulkx_batch_launch_per_call(bulkx_batch* b) {
   if (!b) return BULKX_ERR_INVALID_ARGUMENT;
+  b->last_per_call = true;
   void* s = b->stream;
   for (const bulkx_job& j : b->jobs) {
     int st = BULKX_SUCCESS;

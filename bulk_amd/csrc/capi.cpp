@@ -99,13 +99,14 @@ extern "C" const char* bulkx_status_string(int status) {
     case BULKX_ERR_BUDGET: return "BULKX_ERR_BUDGET";
     case BULKX_ERR_NOT_READY: return "BULKX_ERR_NOT_READY";
     case BULKX_ERR_INTERNAL: return "BULKX_ERR_INTERNAL";
+    case BULKX_ERR_LOOKBACK: return "BULKX_ERR_LOOKBACK";
     default: return "BULKX_ERR_UNKNOWN";
   }
 }
 
 extern "C" const char* bulkx_last_error(void) { return g_last_error.c_str(); }
 
-extern "C" const char* bulkx_version(void) { return "0.1.0"; }
+extern "C" const char* bulkx_version(void) { return "0.2.0"; }
 
 extern "C" size_t bulkx_dtype_size(int dtype) { return dtype_size(dtype); }
 

@@ -296,9 +296,22 @@ __device__ __forceinline__ void st_agent(uint64_t* p, uint64_t v) {
 constexpr uint32_t kStatusInvalid = 0, kStatusAggregate = 1, kStatusPrefix = 2;
 
 // Bounded spin: a wave that waits this many polls for a predecessor gives up,
-// flags the error word and treats the predecessor as an identity prefix, so
-// no kernel can hang forever on a protocol bug.
+// flags the guard's error word and treats the predecessor as an identity
+// prefix, so no kernel can hang forever on a protocol bug.  The host turns a
+// set error word into BULKX_ERR_LOOKBACK (bulkx_stream_status,
+// bulkx_batch_status), so a give-up is never reported as success.
 constexpr uint32_t kSpinLimit = 1u << 22;
+
+// Error word and spin limit shared by the look-back waits of a launch.
+// spin_limit = 0 makes every look-back give up at once without reading its
+// predecessors (a host-set knob for tests of the error path).
+struct LookbackGuard {
+  uint32_t err;
+  uint32_t spin_limit;
+  uint32_t taken;  // the error word as last read and cleared by the host
+  uint32_t pad;
+};
+
 // Sleep between polls of a window whose statuses are not all published
 // (units of 64 clocks).
 #ifndef BULKX_LOOKBACK_SLEEP
@@ -342,10 +355,16 @@ struct TileStatus {
 // advances 64*Q tiles per round trip.  Returns the exclusive prefix of
 // `tile` (tile >= 1).
 template <class Op, int Q = 1, class T>
-__device__ T lookback_exclusive_prefix(const TileStatus<T>& st, int64_t tile, uint32_t* err) {
+__device__ T lookback_exclusive_prefix(const TileStatus<T>& st, int64_t tile,
+                                       LookbackGuard* guard) {
   constexpr int WIN = kWave * Q;
   const int lane = lane_id();
   const T id = Op::template identity<T>();
+  const uint32_t limit = guard->spin_limit;
+  if (limit == 0) {
+    if (lane == 0) atomicOr(&guard->err, 1u);
+    return id;
+  }
   T acc = id;  // combination of every tile in (pos, tile)
   int64_t pos = tile - 1;  // newest tile of the current window
   while (true) {
@@ -368,11 +387,11 @@ __device__ T lookback_exclusive_prefix(const TileStatus<T>& st, int64_t tile, ui
 #pragma unroll
       for (int j = 0; j < Q; ++j)
         if (kind[j] == kStatusInvalid) kind[j] = st.read(first + j, &val[j]);
-      if (++spins > kSpinLimit) {
+      if (++spins > limit) {
 #pragma unroll
         for (int j = 0; j < Q; ++j) {
           if (kind[j] == kStatusInvalid) {
-            atomicOr(err, 1u);
+            atomicOr(&guard->err, 1u);
             kind[j] = kStatusPrefix;
             val[j] = id;
           }

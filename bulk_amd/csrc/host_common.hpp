@@ -105,6 +105,18 @@ class Scratch {
     default: return BULKX_ERR_UNSUPPORTED_OP;                    \
   }
 
+// The look-back guard of the current device (error word + spin limit), a
+// device address valid for kernels of every translation unit.
+struct LookbackGuard;
+int lookback_guard(LookbackGuard** out);
+// Spin limit the host last set (bulkx_set_lookback_spin_limit), for guards
+// that live in other device memory (the batch executor's).
+uint32_t host_spin_limit();
+
+// Whether the batch executor can run job j (kind, dtypes, pointers):
+// BULKX_SUCCESS or the status bulkx_batch_set_jobs would return for it.
+int batch_check_job(const bulkx_job& j);
+
 // Per-algorithm scratch sizes (defined next to each algorithm).
 size_t scan_workspace_bytes(int dtype, int64_t n);
 size_t reduce_workspace_bytes(int dtype, int64_t n);

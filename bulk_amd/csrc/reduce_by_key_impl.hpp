@@ -147,7 +147,7 @@ template <class K, class V, class Op, int BLOCK, int ITEMS>
 __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, const K* keys,
                                          const V* vals, int64_t n, K* kout, V* vout,
                                          int64_t* num_runs, int64_t tile, uint64_t* words,
-                                         uint32_t* err, bool single_tile,
+                                         LookbackGuard* guard, bool single_tile,
                                          const Seg<typename Acc<V, Op>::type>* prefixes = nullptr,
                                          const Seg<typename Acc<V, Op>::type>* group_prefixes = nullptr,
                                          const uint8_t* skip = nullptr, int64_t skip_tiles = 0) {
@@ -307,7 +307,7 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
   if (!single_tile && prefixes == nullptr && threadIdx.x == 0)
     st.publish(tile, tile == 0 ? kStatusPrefix : kStatusAggregate, tile_agg);
   if (chained && threadIdx.x < kWave) {
-    S excl = lookback_exclusive_prefix<SOp, kRbkLookbackQ>(st, tile, err);
+    S excl = lookback_exclusive_prefix<SOp, kRbkLookbackQ>(st, tile, guard);
     if (threadIdx.x == 0) {
       st.publish(tile, kStatusPrefix, SOp::apply(excl, tile_agg));
       sm.prefix = excl;
@@ -384,18 +384,18 @@ template <class K, class V, class Op, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void reduce_by_key_kernel(
     const K* __restrict__ keys, const V* __restrict__ vals, int64_t n, K* __restrict__ kout,
     V* __restrict__ vout, int64_t* __restrict__ num_runs, uint32_t* header, uint64_t* words,
-    bool single_tile) {
+    LookbackGuard* guard, bool single_tile) {
   __shared__ RbkSmem<K, V, Op, BLOCK, ITEMS> sm;
   __shared__ uint32_t s_tile;
   if (single_tile) {
-    rbk_tile<K, V, Op, BLOCK, ITEMS>(sm, keys, vals, n, kout, vout, num_runs, 0, words,
-                                     header + 1, true);
+    rbk_tile<K, V, Op, BLOCK, ITEMS>(sm, keys, vals, n, kout, vout, num_runs, 0, words, guard,
+                                     true);
     return;
   }
   if (threadIdx.x == 0) s_tile = atomicAdd(header, 1u);
   __syncthreads();
   rbk_tile<K, V, Op, BLOCK, ITEMS>(sm, keys, vals, n, kout, vout, num_runs,
-                                   int64_t(wave_uniform(s_tile)), words, header + 1, false);
+                                   int64_t(wave_uniform(s_tile)), words, guard, false);
 }
 
 // Three-pass form (variants kRbkShapes and up): the tiles never wait on
@@ -686,7 +686,7 @@ int rbk_launch(const K* keys, const V* vals, int64_t n, K* kout, V* vout, int64_
   const int64_t tiles = rbk_num_tiles<K, V, VAR>(n);
   if (tiles == 1) {
     reduce_by_key_kernel<K, V, Op, C::BLOCK, C::ITEMS><<<1, C::BLOCK, 0, stream>>>(
-        keys, vals, n, kout, vout, num_runs, nullptr, nullptr, true);
+        keys, vals, n, kout, vout, num_runs, nullptr, nullptr, nullptr, true);
     BULKX_LAUNCHED();
     return BULKX_SUCCESS;
   }
@@ -717,6 +717,8 @@ int rbk_launch(const K* keys, const V* vals, int64_t n, K* kout, V* vout, int64_
     BULKX_LAUNCHED();
     return BULKX_SUCCESS;
   }
+  LookbackGuard* guard = nullptr;
+  BULKX_TRY(lookback_guard(&guard));
   uint32_t* header = reinterpret_cast<uint32_t*>(ws);
   uint64_t* words = reinterpret_cast<uint64_t*>(ws + kRbkHeaderBytes);
   const int64_t num_words = tiles * TileStatus<S>::W;
@@ -724,7 +726,7 @@ int rbk_launch(const K* keys, const V* vals, int64_t n, K* kout, V* vout, int64_
   rbk_prepare_kernel<<<int(pb < 1024 ? pb : 1024), 256, 0, stream>>>(header, words, num_words);
   BULKX_LAUNCHED();
   reduce_by_key_kernel<K, V, Op, C::BLOCK, C::ITEMS><<<dim3(uint32_t(tiles)), C::BLOCK, 0, stream>>>(
-      keys, vals, n, kout, vout, num_runs, header, words, false);
+      keys, vals, n, kout, vout, num_runs, header, words, guard, false);
   BULKX_LAUNCHED();
   return BULKX_SUCCESS;
 }

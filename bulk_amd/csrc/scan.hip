@@ -33,7 +33,8 @@ __global__ void scan_prepare_kernel(uint32_t* header, uint64_t* words, int64_t n
 template <class T, class Op, int BLOCK, int ITEMS, bool EXCLUSIVE>
 __global__ __launch_bounds__(BLOCK) void scan_kernel(const T* in, T* out, int64_t n, T init,
                                                      bool has_init, uint32_t* header,
-                                                     uint64_t* status_words, bool single_tile) {
+                                                     uint64_t* status_words, LookbackGuard* guard,
+                                                     bool single_tile) {
   __shared__ ScanSmem<T, Op, BLOCK> sm;
   __shared__ uint32_t s_tile;
   int64_t tile = 0;
@@ -43,7 +44,7 @@ __global__ __launch_bounds__(BLOCK) void scan_kernel(const T* in, T* out, int64_
     tile = wave_uniform(s_tile);
   }
   scan_tile<T, Op, BLOCK, ITEMS, EXCLUSIVE>(sm, in, out, n, init, has_init, tile, status_words,
-                                            header + 1, single_tile, nullptr);
+                                            guard, single_tile, nullptr);
 }
 
 template <class T, int V>
@@ -87,10 +88,12 @@ static int scan_launch(const T* in, T* out, int64_t n, const T* init, char* ws,
   const T init_v = init ? *init : Op::template identity<T>();
   if (tiles <= 1) {
     scan_kernel<T, Op, C::BLOCK, C::ITEMS, EXCLUSIVE><<<1, C::BLOCK, 0, stream>>>(
-        in, out, n, init_v, init != nullptr, nullptr, nullptr, true);
+        in, out, n, init_v, init != nullptr, nullptr, nullptr, nullptr, true);
     BULKX_LAUNCHED();
     return BULKX_SUCCESS;
   }
+  LookbackGuard* guard = nullptr;
+  BULKX_TRY(lookback_guard(&guard));
   uint32_t* header = reinterpret_cast<uint32_t*>(ws);
   uint64_t* words = reinterpret_cast<uint64_t*>(ws + kHeaderBytes);
   const int64_t num_words = tiles * TileStatus<A>::W;
@@ -98,7 +101,7 @@ static int scan_launch(const T* in, T* out, int64_t n, const T* init, char* ws,
   scan_prepare_kernel<<<prep_blocks, 256, 0, stream>>>(header, words, num_words);
   BULKX_LAUNCHED();
   scan_kernel<T, Op, C::BLOCK, C::ITEMS, EXCLUSIVE><<<dim3(uint32_t(tiles)), C::BLOCK, 0, stream>>>(
-      in, out, n, init_v, init != nullptr, header, words, false);
+      in, out, n, init_v, init != nullptr, header, words, guard, false);
   BULKX_LAUNCHED();
   return BULKX_SUCCESS;
 }

@@ -19,7 +19,7 @@ struct ScanSmem {
 template <class T, class Op, int BLOCK, int ITEMS, bool EXCLUSIVE, bool WRITE = true>
 __device__ __forceinline__ void scan_tile(ScanSmem<T, Op, BLOCK>& sm, const T* in, T* out,
                                           int64_t n, T init, bool has_init, int64_t tile,
-                                          uint64_t* status_words, uint32_t* err,
+                                          uint64_t* status_words, LookbackGuard* guard,
                                           bool single_tile, T* total) {
   using AccP = Acc<T, Op>;
   using A = typename AccP::type;
@@ -80,7 +80,7 @@ __device__ __forceinline__ void scan_tile(ScanSmem<T, Op, BLOCK>& sm, const T* i
   } else {
     if (threadIdx.x == 0) st.publish(tile, kStatusAggregate, tile_agg);
     if (threadIdx.x < kWave) {
-      A excl = lookback_exclusive_prefix<AOp>(st, tile, err);
+      A excl = lookback_exclusive_prefix<AOp>(st, tile, guard);
       if (threadIdx.x == 0) {
         st.publish(tile, kStatusPrefix, AOp::apply(excl, tile_agg));
         sm.prefix = excl;

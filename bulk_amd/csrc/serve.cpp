@@ -201,14 +201,69 @@ static int64_t job_elements(const bulkx_job& j) {
   return j.n0 + (j.kind == BULKX_JOB_MERGE_BY_KEY ? j.n1 : 0);
 }
 
-// One pass of the worker: retire finished slots, then fill free slots.
+// Fails every request still queued (after an error): each counts as
+// completed now, so drain() returns the error instead of waiting forever.
 // Called with the mutex held.
-static void serve_step(bulkx_server* s) {
+static void fail_pending(bulkx_server* s) {
+  const double t = s->now_us();
+  for (int64_t id : s->pending) s->reqs[size_t(id)].t_done = t;
+  s->completed += int64_t(s->pending.size());
+  s->pending.clear();
+  s->cv_done.notify_all();
+}
+
+static void set_error(bulkx_server* s, int st) {
+  if (s->error == BULKX_SUCCESS) s->error = st;
+  fail_pending(s);
+}
+
+// Enqueues one slot's batch on its stream: staging of host requests, the
+// job list, the launch, the copies back and the completion event.  Runs
+// without the server mutex (only the worker touches a slot).
+static int launch_slot(bulkx_server* s, Slot& sl, const std::vector<Request>& taken) {
+  std::vector<bulkx_job> jobs;
+  jobs.reserve(taken.size());
+  int st = BULKX_SUCCESS;
+  for (const Request& r : taken) {
+    if (r.host) {
+      bulkx_job dj;
+      if (st == BULKX_SUCCESS) st = stage_in(r.job, sl.stream, &dj, sl.staged, sl.back);
+      jobs.push_back(st == BULKX_SUCCESS ? dj : r.job);
+    } else {
+      jobs.push_back(r.job);
+    }
+  }
+  if (st == BULKX_SUCCESS) st = bulkx_batch_set_jobs(sl.batch, jobs.data(), int(jobs.size()));
+  if (st == BULKX_SUCCESS) {
+    if (s->flags & BULKX_SERVE_PER_CALL) st = bulkx_batch_launch_per_call(sl.batch);
+    else st = bulkx_batch_launch(sl.batch, (s->flags & BULKX_SERVE_NO_GRAPH) ? 0 : 1);
+  }
+  // host-staged results back, then their device buffers released, in stream order
+  for (const Copy& c : sl.back) {
+    if (st != BULKX_SUCCESS) break;
+    const hipError_t e = hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyDeviceToHost, sl.stream);
+    if (e != hipSuccess) {
+      bulkx::set_last_error("host-staged request", e);
+      st = BULKX_ERR_HIP;
+    }
+  }
+  for (void* p : sl.staged) (void)hipFreeAsync(p, sl.stream);
+  sl.staged.clear();
+  sl.back.clear();
+  if (st == BULKX_SUCCESS && hipEventRecord(sl.done, sl.stream) != hipSuccess) st = BULKX_ERR_HIP;
+  return st;
+}
+
+// One pass of the worker: retire finished slots, then fill free slots.
+// Called with the mutex held; releases it while a slot's batch is enqueued.
+static void serve_step(bulkx_server* s, std::unique_lock<std::mutex>& lock) {
   for (Slot& sl : s->slots) {
     if (!sl.busy) continue;
     hipError_t q = hipEventQuery(sl.done);
     if (q == hipErrorNotReady) continue;
-    if (q != hipSuccess) s->error = BULKX_ERR_HIP;
+    int st = q == hipSuccess ? BULKX_SUCCESS : BULKX_ERR_HIP;
+    // a look-back that gave up inside the batch makes its results wrong
+    if (st == BULKX_SUCCESS) st = bulkx_batch_status(sl.batch);
     const double t = s->now_us();
     for (int64_t id : sl.reqs) s->reqs[size_t(id)].t_done = t;
     s->completed += int64_t(sl.reqs.size());
@@ -216,67 +271,47 @@ static void serve_step(bulkx_server* s) {
     sl.reqs.clear();
     sl.bytes = 0;
     sl.busy = false;
+    if (st != BULKX_SUCCESS) set_error(s, st);
     s->cv_done.notify_all();
   }
-  const double now = s->now_us();
   for (Slot& sl : s->slots) {
     if (sl.busy || s->pending.empty() || s->error != BULKX_SUCCESS) continue;
-    std::vector<bulkx_job> jobs;
+    const double now = s->now_us();
+    std::vector<Request> taken;
     int64_t bytes = 0, elems = 0;
-    int st = BULKX_SUCCESS;
     const int cap = (s->flags & BULKX_SERVE_PER_CALL) ? 1 : s->max_batch_jobs;
-    while (!s->pending.empty() && int(jobs.size()) < cap) {
+    while (!s->pending.empty() && int(taken.size()) < cap) {
       const Request& r = s->reqs[size_t(s->pending.front())];
       if (r.t_arrival > now) break;  // not arrived yet
       const int64_t e = job_elements(r.job);
       if (s->admitted_bytes + bytes + r.bytes > s->budget) break;  // wait for memory
-      if (!jobs.empty() && elems + e > s->max_batch_elements) break;
-      if (r.host) {
-        bulkx_job dj;
-        if (st == BULKX_SUCCESS) st = stage_in(r.job, sl.stream, &dj, sl.staged, sl.back);
-        jobs.push_back(st == BULKX_SUCCESS ? dj : r.job);
-      } else {
-        jobs.push_back(r.job);
-      }
+      if (!taken.empty() && elems + e > s->max_batch_elements) break;
+      taken.push_back(r);
       sl.reqs.push_back(s->pending.front());
       bytes += r.bytes;
       elems += e;
       s->pending.pop_front();
     }
-    if (jobs.empty()) break;  // the head request waits for budget
-    if (st == BULKX_SUCCESS) st = bulkx_batch_set_jobs(sl.batch, jobs.data(), int(jobs.size()));
-    if (st == BULKX_SUCCESS) {
-      if (s->flags & BULKX_SERVE_PER_CALL) st = bulkx_batch_launch_per_call(sl.batch);
-      else st = bulkx_batch_launch(sl.batch, (s->flags & BULKX_SERVE_NO_GRAPH) ? 0 : 1);
-    }
-    // host-staged results back, then their device buffers released, in stream order
-    for (const Copy& c : sl.back) {
-      if (st != BULKX_SUCCESS) break;
-      const hipError_t e = hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyDeviceToHost, sl.stream);
-      if (e != hipSuccess) {
-        bulkx::set_last_error("host-staged request", e);
-        st = BULKX_ERR_HIP;
-      }
-    }
-    for (void* p : sl.staged) (void)hipFreeAsync(p, sl.stream);
-    sl.staged.clear();
-    sl.back.clear();
-    if (st == BULKX_SUCCESS && hipEventRecord(sl.done, sl.stream) != hipSuccess) st = BULKX_ERR_HIP;
+    if (taken.empty()) break;  // the head request waits for budget
+    // charge the budget before the lock is released
+    s->admitted_bytes += bytes;
+    if (s->admitted_bytes > s->peak_bytes) s->peak_bytes = s->admitted_bytes;
+    lock.unlock();
+    const int st = launch_slot(s, sl, taken);
+    lock.lock();
     const double t = s->now_us();
     for (int64_t id : sl.reqs) s->reqs[size_t(id)].t_launch = t;
     if (st != BULKX_SUCCESS) {
-      s->error = st;
-      // fail the slot's requests so drain() returns
+      // fail the slot's requests and everything queued, so drain() returns
+      s->admitted_bytes -= bytes;
       for (int64_t id : sl.reqs) s->reqs[size_t(id)].t_done = t;
       s->completed += int64_t(sl.reqs.size());
       sl.reqs.clear();
-      s->cv_done.notify_all();
+      set_error(s, st);
       continue;
     }
     sl.busy = true;
     sl.bytes = bytes;
-    s->admitted_bytes += bytes;
-    if (s->admitted_bytes > s->peak_bytes) s->peak_bytes = s->admitted_bytes;
     s->batches += 1;
   }
 }
@@ -285,7 +320,7 @@ static void serve_loop(bulkx_server* s) {
   (void)hipSetDevice(s->device);
   std::unique_lock<std::mutex> lock(s->mu);
   while (true) {
-    serve_step(s);
+    serve_step(s, lock);
     bool any_busy = false;
     for (const Slot& sl : s->slots) any_busy |= sl.busy;
     if (s->stop && s->pending.empty() && !any_busy) break;
@@ -296,8 +331,14 @@ static void serve_loop(bulkx_server* s) {
       lock.unlock();
       std::this_thread::sleep_for(std::chrono::microseconds(10));
       lock.lock();
-    } else {
+    } else if (s->pending.empty() || s->error != BULKX_SUCCESS) {
+      if (s->error != BULKX_SUCCESS) fail_pending(s);
       s->cv_work.wait(lock, [&] { return s->stop || !s->pending.empty(); });
+    } else {
+      // requests have arrived but none fits yet: nothing is on the device,
+      // so the head request needs more than the free budget can ever give
+      // (submit() rejects such requests; this keeps the loop from spinning)
+      set_error(s, BULKX_ERR_BUDGET);
     }
   }
 }
@@ -325,6 +366,8 @@ extern "C" int bulkx_server_create(int64_t hbm_budget_bytes, int max_inflight, i
     int st = e == hipSuccess ? BULKX_SUCCESS : BULKX_ERR_HIP;
     if (st == BULKX_SUCCESS)
       st = bulkx_batch_create(max_batch_jobs, max_batch_elements, sl.stream, &sl.batch);
+    // scratch for the largest batch now: serving never allocates or frees it
+    if (st == BULKX_SUCCESS) st = bulkx_batch_reserve(sl.batch, max_batch_elements);
     // graphs of every merge-round count up front: no request waits on a capture
     if (st == BULKX_SUCCESS && !(flags & (BULKX_SERVE_PER_CALL | BULKX_SERVE_NO_GRAPH)))
       st = bulkx_batch_prepare_graphs(sl.batch);
@@ -363,6 +406,8 @@ static int submit_impl(bulkx_server* s, const bulkx_job* job, int64_t bytes, dou
   if (!s || !job || bytes < 0) return BULKX_ERR_INVALID_ARGUMENT;
   if (bytes > s->budget) return BULKX_ERR_BUDGET;
   if (job_elements(*job) > s->max_batch_elements) return BULKX_ERR_BUDGET;
+  // a job the batch cannot run is refused here, never queued
+  BULKX_TRY(batch_check_job(*job));
   std::lock_guard<std::mutex> lock(s->mu);
   if (s->error != BULKX_SUCCESS) return s->error;
   const double now = s->now_us();

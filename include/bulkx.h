@@ -70,7 +70,8 @@ enum bulkx_status {
   BULKX_ERR_OUT_OF_MEMORY = 7,
   BULKX_ERR_BUDGET = 8,
   BULKX_ERR_NOT_READY = 9,
-  BULKX_ERR_INTERNAL = 10
+  BULKX_ERR_INTERNAL = 10,
+  BULKX_ERR_LOOKBACK = 11 /* a look-back wait gave up: the results of that launch are wrong */
 };
 
 /* Name of a status code (static string; never NULL). */
@@ -84,6 +85,29 @@ const char* bulkx_version(void);
 
 /* Size in bytes of one element of `dtype`, 0 if unknown. */
 size_t bulkx_dtype_size(int dtype);
+
+/*
+ * Deferred device-side failure of the plain entry points.  A tile of a
+ * single-pass scan or reduce_by_key waits for its predecessors' published
+ * prefixes (decoupled look-back); a wait bounded by the spin limit that runs
+ * out sets the current device's sticky look-back error word, and that
+ * launch's results are wrong.  bulkx_stream_status waits for `stream`, then
+ * reads and clears the word: BULKX_ERR_LOOKBACK if any launch on this device
+ * gave up since the last query, else BULKX_SUCCESS.  The counterpart of the
+ * reference's sticky launch errors, reported by cudaGetLastError after a
+ * synchronize (bulk/detail/throw_on_error.hpp:33-49, terminate.hpp:34).
+ */
+int bulkx_stream_status(void* stream);
+
+/*
+ * Polls a look-back wait makes before it gives up (default 2^22, about 0.2 s).
+ * 0 makes every look-back give up at once without reading its predecessors,
+ * so a test can drive the error path from the host while every kernel still
+ * exits normally.  Applies to launches enqueued afterwards on the current
+ * device and to batches whose job list is set afterwards.  No counterpart
+ * in the reference (its waits are unbounded).
+ */
+int bulkx_set_lookback_spin_limit(uint32_t polls);
 
 /*
  * Device properties the launcher sizes against (replaces
@@ -271,6 +295,13 @@ int bulkx_batch_set_jobs(bulkx_batch* batch, const bulkx_job* jobs, int num_jobs
 int bulkx_batch_launch(bulkx_batch* batch, int use_graph);
 /* Run each job as its own plain call on the batch's stream (the per-call baseline). */
 int bulkx_batch_launch_per_call(bulkx_batch* batch);
+/* Waits for the batch's last launch and returns BULKX_ERR_LOOKBACK if one of
+ * its look-back waits gave up (see bulkx_stream_status), else BULKX_SUCCESS. */
+int bulkx_batch_status(bulkx_batch* batch);
+/* Allocates now the scratch that job lists of up to `elements` elements can
+ * need (sort ping-pong space, the per-call path's workspace), so that later
+ * bulkx_batch_set_jobs calls within that size allocate and free nothing. */
+int bulkx_batch_reserve(bulkx_batch* batch, int64_t elements);
 /* Record ahead of time the graph of every number of merge rounds the batch
  * can need (one graph per count is recorded on first use otherwise), so no
  * later graph launch pays a capture.  No counterpart in the reference; for

@@ -60,6 +60,14 @@ inline hipStream_t& default_stream() {
   return s;
 }
 
+// Waits for the stream and throws if a look-back wait on this device gave
+// up since the last check (the results of that launch are wrong): the
+// counterpart of the reference's synchronize + throw_on_error
+// (bulk/detail/synchronize.hpp:47-54, throw_on_error.hpp:33-49).
+inline void synchronize(hipStream_t s = default_stream()) {
+  check(bulkx_stream_status(s), "bulkx::synchronize");
+}
+
 // ------------------------------------------------------------- functors
 template <class T = void>
 struct plus {
@@ -269,6 +277,8 @@ std::pair<KIt2, VIt2> my_reduce_by_key(KIt1 keys_first, KIt1 keys_last, VIt1 val
       hipMemcpyAsync(&m, d, sizeof(m), hipMemcpyDeviceToHost, default_stream()) != hipSuccess)
     st = BULKX_ERR_HIP;
   if (hipStreamSynchronize(default_stream()) != hipSuccess && st == BULKX_SUCCESS) st = BULKX_ERR_HIP;
+  // the call has blocked anyway: a look-back that gave up is reported here
+  if (st == BULKX_SUCCESS) st = bulkx_stream_status(default_stream());
   (void)hipFreeAsync(d, default_stream());
   check(st, "my_reduce_by_key");
   return {keys_result + m, values_result + m};

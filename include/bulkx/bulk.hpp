@@ -130,7 +130,11 @@ class concurrent_group : public parallel_group<ExecutionAgent, size_> {
                                                 agent_type exec = agent_type(),
                                                 size_type i = invalid_index)
       : super_t(size_type(SS), exec, i),
-        m_heap_size(size_type(heap_size == use_default ? 0 : heap_size)) {}
+        m_heap_size(heap_size == use_default ? default_heap : size_type(heap_size)) {}
+
+  // heap_size() == default_heap: the launcher sizes the LDS heap from the
+  // kernel's occupancy (bulk/detail/cuda_launcher/cuda_launcher.hpp:105-143)
+  static const size_type default_heap = -1;
 
   __device__ void wait() const { __syncthreads(); }
   __host__ __device__ size_type heap_size() const { return m_heap_size; }
@@ -256,14 +260,17 @@ inline async_launch<parallel_group<>> par(future<void>& before, std::size_t n) {
 
 inline concurrent_group<> con(std::size_t size, std::size_t heap_size = use_default) {
   return concurrent_group<>(int(size == use_default ? 256 : size),
-                            int(heap_size == use_default ? 0 : heap_size));
+                            heap_size == use_default ? concurrent_group<>::default_heap
+                                                     : int(heap_size));
 }
 
 template <class ExecutionAgent>
 concurrent_group<ExecutionAgent> con(ExecutionAgent exec, std::size_t size,
                                      std::size_t heap_size = use_default) {
-  return concurrent_group<ExecutionAgent>(int(size == use_default ? 256 : size),
-                                          int(heap_size == use_default ? 0 : heap_size), exec);
+  return concurrent_group<ExecutionAgent>(
+      int(size == use_default ? 256 : size),
+      heap_size == use_default ? concurrent_group<ExecutionAgent>::default_heap : int(heap_size),
+      exec);
 }
 
 template <std::size_t groupsize, std::size_t grainsize>
@@ -345,40 +352,169 @@ __device__ decltype(auto) substitute(G& root, A& a) {
 static constexpr detail::cursor<0> root{};
 
 // ------------------------------------------------------------------ malloc
-// Collective allocation in the group's LDS arena: every agent of the group
+// The group's on-chip heap: the workgroup's dynamic LDS (the launcher sizes
+// it, heap_size bytes or, by default, the LDS share that keeps the kernel's
+// occupancy) managed as a first-fit list of blocks, the allocator of
+// bulk/malloc.hpp:133-199 written afresh for LDS.  Arena layout: a 16-byte
+// header {break offset, lock}, then blocks of a 16-byte header {data bytes,
+// free flag, offset of the previous block} followed by their data.  Blocks
+// can be freed in any order; a freed block merges with free neighbours, and
+// a free block at the end moves the break back.  When the arena has no room,
+// allocation falls back to the device heap (::malloc); if that fails too the
+// result is null, as with the reference's shmalloc (malloc.hpp:519-531).
+namespace detail {
+
+struct lds_block {
+  unsigned size;  // data bytes, a multiple of 16
+  unsigned free;
+  unsigned prev;  // offset of the previous block's header; 0 for the first block
+  unsigned pad;
+};
+static_assert(sizeof(lds_block) == 16, "block headers keep data 16-byte aligned");
+
+constexpr unsigned kArenaHeader = 16;
+
+__device__ inline lds_block* lds_block_at(char* base, unsigned off) {
+  return reinterpret_cast<lds_block*>(base + off);
+}
+__device__ inline unsigned& lds_break(char* base) { return *reinterpret_cast<unsigned*>(base); }
+__device__ inline unsigned& lds_lock(char* base) { return reinterpret_cast<unsigned*>(base)[1]; }
+
+// First fit over the block list, else a new block at the break; null when
+// the arena has no room.  Not thread safe: one agent at a time.
+__device__ inline void* lds_allocate(const lds_arena& a, std::size_t bytes) {
+  if (!a.base || a.bytes <= kArenaHeader) return nullptr;
+  char* base = a.base;
+  const std::size_t need = (bytes + 15) & ~std::size_t(15);
+  if (need > a.bytes) return nullptr;
+  const unsigned brk = lds_break(base);
+  unsigned last = 0;
+  for (unsigned off = kArenaHeader; off < brk;) {
+    lds_block* b = lds_block_at(base, off);
+    if (b->free && b->size >= need) {
+      if (b->size - need >= 2 * sizeof(lds_block)) {  // split off the rest as a free block
+        const unsigned rest = off + unsigned(sizeof(lds_block) + need);
+        lds_block* r = lds_block_at(base, rest);
+        r->size = b->size - unsigned(need) - unsigned(sizeof(lds_block));
+        r->free = 1;
+        r->prev = off;
+        const unsigned after = rest + unsigned(sizeof(lds_block)) + r->size;
+        if (after < brk) lds_block_at(base, after)->prev = rest;
+        b->size = unsigned(need);
+      }
+      b->free = 0;
+      return base + off + sizeof(lds_block);
+    }
+    last = off;
+    off += unsigned(sizeof(lds_block)) + b->size;
+  }
+  if (std::size_t(brk) + sizeof(lds_block) + need > a.bytes) return nullptr;
+  lds_block* b = lds_block_at(base, brk);
+  b->size = unsigned(need);
+  b->free = 0;
+  b->prev = last;
+  lds_break(base) = brk + unsigned(sizeof(lds_block) + need);
+  return base + brk + sizeof(lds_block);
+}
+
+__device__ inline bool lds_owns(const lds_arena& a, const void* p) {
+  const char* c = static_cast<const char*>(p);
+  return a.base && c >= a.base + kArenaHeader + sizeof(lds_block) && c < a.base + a.bytes;
+}
+
+__device__ inline void lds_deallocate(const lds_arena& a, void* p) {
+  char* base = a.base;
+  unsigned off = unsigned(static_cast<char*>(p) - base) - unsigned(sizeof(lds_block));
+  lds_block* b = lds_block_at(base, off);
+  b->free = 1;
+  unsigned brk = lds_break(base);
+  // merge with a free successor
+  unsigned next = off + unsigned(sizeof(lds_block)) + b->size;
+  if (next < brk && lds_block_at(base, next)->free) {
+    b->size += unsigned(sizeof(lds_block)) + lds_block_at(base, next)->size;
+    next = off + unsigned(sizeof(lds_block)) + b->size;
+    if (next < brk) lds_block_at(base, next)->prev = off;
+  }
+  // merge into a free predecessor
+  if (off > kArenaHeader && lds_block_at(base, b->prev)->free) {
+    const unsigned p0 = b->prev;
+    lds_block* pb = lds_block_at(base, p0);
+    pb->size += unsigned(sizeof(lds_block)) + b->size;
+    off = p0;
+    b = pb;
+    if (next < brk) lds_block_at(base, next)->prev = off;
+  }
+  // a free block at the end gives its space back to the break
+  if (next >= brk) lds_break(base) = off;
+}
+
+}  // namespace detail
+
+// Collective allocation (bulk/malloc.hpp:589-604): every agent of the group
 // calls it with the same size and receives the same pointer.
-// The arena is a stack: its first 16 bytes hold the offset of the free
-// space; free(p) of an arena block releases p and every block allocated
-// after it (the reference's kernels free in reverse order of allocation).
 template <class Agent, std::size_t S>
 __device__ void* malloc(concurrent_group<Agent, S>& g, std::size_t bytes) {
   __shared__ void* s_ptr;
   g.wait();
   if (g.this_exec.index() == 0) {
-    unsigned* top = reinterpret_cast<unsigned*>(g.arena.base);
-    const std::size_t need = (bytes + 15) & ~std::size_t(15);
-    if (top && std::size_t(*top) + need <= g.arena.bytes) {
-      s_ptr = g.arena.base + *top;
-      *top += unsigned(need);
-    } else {
-      s_ptr = ::malloc(bytes);  // device heap when the LDS arena is exhausted
-    }
+    void* p = detail::lds_allocate(g.arena, bytes);
+    s_ptr = p ? p : ::malloc(bytes);  // device heap when the arena has no room; may be null
   }
   g.wait();
   return s_ptr;
 }
 
+// Collective free (bulk/malloc.hpp:607-615); any order, null ignored.
 template <class Agent, std::size_t S>
 __device__ void free(concurrent_group<Agent, S>& g, void* ptr) {
   g.wait();
   if (g.this_exec.index() == 0 && ptr) {
-    char* p = static_cast<char*>(ptr);
-    if (g.arena.base && p >= g.arena.base + 16 && p < g.arena.base + g.arena.bytes)
-      *reinterpret_cast<unsigned*>(g.arena.base) = unsigned(p - g.arena.base);
-    else
-      ::free(ptr);
+    if (detail::lds_owns(g.arena, ptr)) detail::lds_deallocate(g.arena, ptr);
+    else ::free(ptr);
   }
   g.wait();
+}
+
+// Allocation by one agent (bulk/malloc.hpp:519-569: shmalloc / shfree): the
+// arena is guarded by an LDS lock, so agents of a group may call it
+// independently.  The lock loop keeps the critical section inside the loop
+// body, so lanes of one wavefront that contend take turns instead of
+// waiting on each other.
+template <class Agent, std::size_t S>
+__device__ void* shmalloc(concurrent_group<Agent, S>& g, std::size_t bytes) {
+  void* p = nullptr;
+  if (g.arena.base) {
+    unsigned* lock = &detail::lds_lock(g.arena.base);
+    bool done = false;
+    while (!done) {
+      if (atomicCAS(lock, 0u, 1u) == 0u) {
+        p = detail::lds_allocate(g.arena, bytes);
+        __threadfence_block();
+        atomicExch(lock, 0u);
+        done = true;
+      }
+    }
+  }
+  return p ? p : ::malloc(bytes);
+}
+
+template <class Agent, std::size_t S>
+__device__ void shfree(concurrent_group<Agent, S>& g, void* ptr) {
+  if (!ptr) return;
+  if (!detail::lds_owns(g.arena, ptr)) {
+    ::free(ptr);
+    return;
+  }
+  unsigned* lock = &detail::lds_lock(g.arena.base);
+  bool done = false;
+  while (!done) {
+    if (atomicCAS(lock, 0u, 1u) == 0u) {
+      detail::lds_deallocate(g.arena, ptr);
+      __threadfence_block();
+      atomicExch(lock, 0u);
+      done = true;
+    }
+  }
 }
 
 // -------------------------------------------------------------- the launcher
@@ -394,13 +530,12 @@ struct launch_traits<parallel_group<A, S>> {
   static dim3 block(const parallel_group<A, S>&) { return dim3(256); }
   static std::size_t lds(const parallel_group<A, S>&) { return 0; }
 };
-// con(n): one workgroup
+// con(n): one workgroup.  heap(): the requested heap, or default_heap.
 template <class A, std::size_t S>
 struct launch_traits<concurrent_group<A, S>> {
   static dim3 grid(const concurrent_group<A, S>&) { return dim3(1); }
   static dim3 block(const concurrent_group<A, S>& g) { return dim3(unsigned(g.size())); }
-  static std::size_t lds(const concurrent_group<A, S>& g) { return arena_bytes(g.heap_size()); }
-  static std::size_t arena_bytes(int heap) { return heap > 0 ? std::size_t(heap) + 16 : 0; }
+  static int heap(const concurrent_group<A, S>& g) { return g.heap_size(); }
 };
 // grid: one workgroup per concurrent group
 template <class A, std::size_t S, std::size_t PS>
@@ -408,17 +543,59 @@ struct launch_traits<parallel_group<concurrent_group<A, S>, PS>> {
   typedef parallel_group<concurrent_group<A, S>, PS> G;
   static dim3 grid(const G& g) { return dim3(unsigned(g.size())); }
   static dim3 block(const G& g) { return dim3(unsigned(g.this_exec.size())); }
-  static std::size_t lds(const G& g) {
-    return launch_traits<concurrent_group<A, S>>::arena_bytes(g.this_exec.heap_size());
-  }
+  static int heap(const G& g) { return g.this_exec.heap_size(); }
 };
+
+// Dynamic LDS of a launch whose group asked for `requested` heap bytes
+// (choose_heap_size, bulk/detail/cuda_launcher/cuda_launcher.hpp:105-143):
+// 0 -> no heap; default_heap -> the LDS share of one workgroup at the
+// kernel's occupancy with no dynamic LDS (the heap then costs no
+// occupancy); an explicit request -> the larger of the request (plus the
+// arena's bookkeeping) and that share.  Capped by what one workgroup may
+// allocate.
+template <class K>
+inline std::size_t heap_bytes(K kernel, int requested, int block_threads) {
+  if (requested == 0) return 0;
+  int dev = 0;
+  throw_on_error(hipGetDevice(&dev), "hipGetDevice in bulk::async");
+  int lds_cu = 0, lds_block_max = 0;
+  throw_on_error(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                                       dev),
+                 "hipDeviceGetAttribute in bulk::async");
+  throw_on_error(hipDeviceGetAttribute(&lds_block_max, hipDeviceAttributeSharedMemPerBlockOptin,
+                                       dev),
+                 "hipDeviceGetAttribute in bulk::async");
+  if (lds_block_max <= 0) lds_block_max = lds_cu;
+  hipFuncAttributes attr;
+  throw_on_error(hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(kernel)),
+                 "hipFuncGetAttributes in bulk::async");
+  int blocks = 0;
+  throw_on_error(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                     &blocks, reinterpret_cast<const void*>(kernel), block_threads, 0),
+                 "hipOccupancyMaxActiveBlocksPerMultiprocessor in bulk::async");
+  if (blocks < 1) blocks = 1;
+  const long long cap = (long long)lds_block_max - (long long)attr.sharedSizeBytes;
+  long long share = (long long)lds_cu / blocks - (long long)attr.sharedSizeBytes;
+  share = share < cap ? share : cap;
+  share = share > 0 ? share & ~15LL : 0;
+  long long want = share;
+  if (requested > 0) {
+    const long long req = (long long)((requested + 15) & ~15) + 64;  // + block headers
+    if (req > want) want = req;
+  }
+  if (want > cap) want = cap & ~15LL;
+  return want > (long long)detail::kArenaHeader ? std::size_t(want) : 0;
+}
 
 template <class A, std::size_t S>
 __device__ void bind_arena(concurrent_group<A, S>& cg, std::size_t bytes) {
   extern __shared__ __align__(16) char bulk_dynamic_lds[];
   cg.arena.base = bytes ? bulk_dynamic_lds : nullptr;
   cg.arena.bytes = unsigned(bytes);
-  if (bytes && threadIdx.x == 0) *reinterpret_cast<unsigned*>(bulk_dynamic_lds) = 16;
+  if (bytes && threadIdx.x == 0) {
+    detail::lds_break(bulk_dynamic_lds) = detail::kArenaHeader;
+    detail::lds_lock(bulk_dynamic_lds) = 0;
+  }
   __syncthreads();
 }
 
@@ -474,7 +651,8 @@ template <class A, std::size_t S, class F, class... Args>
 void launch(const concurrent_group<A, S>& g, hipStream_t s, F f, Args... args) {
   if (g.size() <= 0 || g.size() > 1024)
     throw system_error(hipErrorInvalidValue, "bulk::async: a concurrent_group holds 1..1024 agents");
-  const std::size_t lds = launch_traits<concurrent_group<A, S>>::lds(g);
+  const std::size_t lds = heap_bytes(con_kernel<A, S, F, Args...>,
+                                     launch_traits<concurrent_group<A, S>>::heap(g), g.size());
   allow_lds(con_kernel<A, S, F, Args...>, lds);
   con_kernel<A, S, F, Args...><<<dim3(1), dim3(unsigned(g.size())), lds, s>>>(g, lds, f, args...);
 }
@@ -485,7 +663,8 @@ void launch(const parallel_group<concurrent_group<A, S>, PS>& g, hipStream_t s, 
   if (g.this_exec.size() <= 0 || g.this_exec.size() > 1024)
     throw system_error(hipErrorInvalidValue, "bulk::async: a concurrent_group holds 1..1024 agents");
   typedef parallel_group<concurrent_group<A, S>, PS> G;
-  const std::size_t lds = launch_traits<G>::lds(g);
+  const std::size_t lds = heap_bytes(grid_kernel<A, S, PS, F, Args...>, launch_traits<G>::heap(g),
+                                     g.this_exec.size());
   allow_lds(grid_kernel<A, S, PS, F, Args...>, lds);
   grid_kernel<A, S, PS, F, Args...><<<launch_traits<G>::grid(g), launch_traits<G>::block(g), lds, s>>>(
       g, lds, f, args...);

@@ -8,7 +8,7 @@ import subprocess
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-EXE = os.path.join(ROOT, "tests", "cpp", "_bin", "surface_test")
+BIN = os.path.join(ROOT, "tests", "cpp", "_bin")
 
 
 @pytest.mark.parametrize("header", ["bulkx/bulk.hpp", "bulkx/algorithm.hpp"])
@@ -25,9 +25,11 @@ def test_header_compiles_alone(tmp_path, header):
 
 
 @pytest.mark.gpu
-def test_surface_program_passes():
-    assert os.path.exists(EXE), "tests/cpp/_bin/surface_test is not built (run the build first)"
-    r = subprocess.run([EXE], capture_output=True, text=True, timeout=120)
+@pytest.mark.parametrize("program", ["surface_test", "heap_test"])
+def test_surface_program_passes(program):
+    exe = os.path.join(BIN, program)
+    assert os.path.exists(exe), f"tests/cpp/_bin/{program} is not built (run the build first)"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     assert "ALL PASS" in r.stdout

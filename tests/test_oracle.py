@@ -110,6 +110,59 @@ def test_sum_exact_matches_fsum(orc):
     assert ab == pytest.approx(float(np.sum(np.abs(x))), rel=1e-12)
 
 
+@pytest.mark.parametrize("seed,n,kind", [(12, 1, "u"), (13, 1000, "u"), (14, 100003, "u"),
+                                         (15, 65536, "n"), (16, 300001, "n")])
+def test_sum_exact_f32_matches_fsum(orc, seed, n, kind):
+    """The float32 path (orc_sum_ld_f32, the checker of every f32 reduce):
+    every float32 converts to float64 exactly, so math.fsum of the converted
+    values is the exact sum rounded once; the long-double accumulator is
+    within n * 2^-64 * sum|x| of it."""
+    import math
+    x = orc.gen_uniform_f32(seed, n) if kind == "u" else orc.gen_normal(seed, n, np.float32)
+    assert x.dtype == np.float32
+    hi, lo, ab = orc.sum_exact(x)
+    exact = math.fsum(x.astype(np.float64))
+    assert abs((hi + lo) - exact) <= n * 2.0 ** -63 * ab + 2.0 ** -52 * abs(exact)
+    assert ab == math.fsum(np.abs(x).astype(np.float64))
+    # hi is the sum rounded to double, lo the remainder
+    assert hi == hi + lo or abs(lo) <= 2.0 ** -52 * abs(hi)
+
+
+def test_sum_exact_f32_carries_more_than_double(orc):
+    """2^60 + 1 - 2^60 is 1 exactly but 0 in float64 (and float32)
+    accumulation: a checker that accumulated in double would miss the
+    small terms a float32 reduce must keep."""
+    x = np.array([2.0 ** 60, 1.0, -(2.0 ** 60)] * 5, dtype=np.float32)
+    hi, lo, ab = orc.sum_exact(x)
+    assert hi + lo == 5.0
+    assert ab == 10 * 2.0 ** 60 + 5.0
+    # float64 accumulation of the same values gives 0
+    acc = 0.0
+    for v in x.astype(np.float64):
+        acc += v
+    assert acc == 0.0
+
+
+@pytest.mark.parametrize("n", [0, 1, 19999, 20000, 20001, 1152 * 37 + 5, 1 << 20])
+@pytest.mark.parametrize("threads", [1, 2, 3, 8])
+def test_mt_inclusive_scan_matches_single_thread(orc, n, threads):
+    """orc_mt_inclusive_scan_i32 (the reference's 3-phase decomposition,
+    scan.cu:109-145, one partition per thread) equals the one-thread body
+    and a wrapping cumsum, bit for bit."""
+    x = orc.gen_uniform_i32(7, -1000, 1000, n) if n else np.zeros(0, np.int32)
+    got = orc.mt_inclusive_scan_i32(x, 13, threads)
+    assert np.array_equal(got, orc.inclusive_scan(x, init=13))
+    ref = (np.cumsum(x.astype(np.int64)) + 13).astype(np.int64)
+    assert np.array_equal(got.astype(np.int64), ((ref + 2 ** 31) % 2 ** 32) - 2 ** 31)
+
+
+def test_mt_inclusive_scan_wraps_like_int32(orc):
+    x = np.full(100000, 2 ** 30, dtype=np.int32)
+    got = orc.mt_inclusive_scan_i32(x, 0, 4)
+    ref = ((np.arange(1, x.size + 1, dtype=np.int64) * 2 ** 30 + 2 ** 31) % 2 ** 32) - 2 ** 31
+    assert np.array_equal(got.astype(np.int64), ref)
+
+
 # ----------------------------------------------------------------- merge
 @pytest.mark.parametrize("n1,n2", [(0, 0), (0, 5), (5, 0), (1, 1), (1000, 37), (4096, 4097)])
 def test_merge_matches_stable_argsort(orc, n1, n2):
