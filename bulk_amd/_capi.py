@@ -79,6 +79,7 @@ SIGNATURES = {
     "bulkx_workspace_size": (_I, [_I, _I, _I, _L, _L, ctypes.POINTER(_S)]),
     "bulkx_inclusive_scan": (_I, [_I, _I, _P, _P, _L, _P, _P, _S, _P]),
     "bulkx_exclusive_scan": (_I, [_I, _I, _P, _P, _L, _P, _P, _S, _P]),
+    "bulkx_inclusive_scan_noalloc": (_I, [_I, _I, _P, _P, _L, _P, _P]),
     "bulkx_reduce": (_I, [_I, _I, _P, _L, _P, _P, _P, _S, _P]),
     "bulkx_merge": (_I, [_I, _I, _P, _L, _P, _L, _P, _P, _S, _P]),
     "bulkx_merge_by_key": (_I, [_I, _I, _I, _P, _P, _L, _P, _P, _L, _P, _P, _P, _S, _P]),

@@ -12,6 +12,8 @@
 // sequentially (the bounded-agent body, scan.hpp:35-55), the per-thread
 // totals are scanned across the wavefront with DPP and across the
 // workgroup through LDS, and wave 0 looks back over the predecessors.
+#include <mutex>
+
 #include "device_common.hpp"
 #include "host_common.hpp"
 #include "scan_config.hpp"
@@ -117,8 +119,8 @@ static int scan_entry(int dtype, int op, const void* in, void* out, int64_t n, c
   if (in == nullptr || out == nullptr) return BULKX_ERR_INVALID_ARGUMENT;
   if (n > (int64_t(1) << 40)) return BULKX_ERR_INVALID_ARGUMENT;
   hipStream_t stream = static_cast<hipStream_t>(stream_);
-  Scratch scratch;
-  BULKX_TRY(scratch.acquire(workspace, workspace_bytes, scan_workspace_bytes(dtype, n), stream));
+  Scratch ws;
+  BULKX_TRY(ws.acquire(workspace, workspace_bytes, scan_workspace_bytes(dtype, n), stream));
   int st = BULKX_SUCCESS;
   const int v = tuned_variant(EXCLUSIVE ? BULKX_ALGO_EXCLUSIVE_SCAN : BULKX_ALGO_INCLUSIVE_SCAN,
                               dtype, n);
@@ -131,12 +133,12 @@ static int scan_entry(int dtype, int op, const void* in, void* out, int64_t n, c
       const T* ini = static_cast<const T*>(init);
       done = true;
       switch (v) {
-        case 1: st = scan_launch<T, OP, EXCLUSIVE, 1>(i, o, n, ini, scratch.ptr(), stream); break;
-        case 2: st = scan_launch<T, OP, EXCLUSIVE, 2>(i, o, n, ini, scratch.ptr(), stream); break;
-        case 3: st = scan_launch<T, OP, EXCLUSIVE, 3>(i, o, n, ini, scratch.ptr(), stream); break;
+        case 1: st = scan_launch<T, OP, EXCLUSIVE, 1>(i, o, n, ini, ws.ptr(), stream); break;
+        case 2: st = scan_launch<T, OP, EXCLUSIVE, 2>(i, o, n, ini, ws.ptr(), stream); break;
+        case 3: st = scan_launch<T, OP, EXCLUSIVE, 3>(i, o, n, ini, ws.ptr(), stream); break;
         case 4:
           if constexpr (ScanConfig<T, 4>::kWide)
-            st = scan_launch<T, OP, EXCLUSIVE, 4>(i, o, n, ini, scratch.ptr(), stream);
+            st = scan_launch<T, OP, EXCLUSIVE, 4>(i, o, n, ini, ws.ptr(), stream);
           else
             done = false;
           break;
@@ -145,10 +147,41 @@ static int scan_entry(int dtype, int op, const void* in, void* out, int64_t n, c
     }
     if (!done)
       st = scan_launch<T, OP, EXCLUSIVE, 0>(static_cast<const T*>(in), static_cast<T*>(out), n,
-                                            static_cast<const T*>(init), scratch.ptr(), stream);
+                                            static_cast<const T*>(init), ws.ptr(), stream);
   }));
-  int rs = scratch.release();
+  int rs = ws.release();
   return st != BULKX_SUCCESS ? st : rs;
+}
+
+// Scratch of the allocation-free scan: the look-back header and status
+// words live in this array, one copy per device (each device that loads the
+// library's code object holds its own until the process ends).  4 MiB of
+// status words plus the 256-byte header: enough for 2^32 4-byte integers,
+// 2^31 floats, 2^30 8-byte integers or 2^29 doubles.
+constexpr size_t kNoAllocScratchBytes = (size_t(4) << 20) + kHeaderBytes;
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxScratchDevices] = {};
+}  // namespace
+
+// The device address of `scratch` on the current device, looked up once per
+// device with that device current.  In host code the array's name is only a
+// host-side stand-in; kernels get this address.
+static int noalloc_scratch(char** out) {
+  int dev = 0;
+  BULKX_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxScratchDevices) return BULKX_ERR_INTERNAL;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    BULKX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<char*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return BULKX_SUCCESS;
 }
 
 }  // namespace bulkx
@@ -157,6 +190,20 @@ extern "C" int bulkx_inclusive_scan(int dtype, int op, const void* in, void* out
                                     const void* init, void* workspace, size_t workspace_bytes,
                                     void* stream) {
   return bulkx::scan_entry<false>(dtype, op, in, out, n, init, workspace, workspace_bytes, stream);
+}
+
+extern "C" int bulkx_inclusive_scan_noalloc(int dtype, int op, const void* in, void* out,
+                                            int64_t n, const void* init, void* stream) {
+  if (bulkx::dtype_size(dtype) == 0) return BULKX_ERR_UNSUPPORTED_DTYPE;
+  if (n < 0) return BULKX_ERR_INVALID_ARGUMENT;
+  if (bulkx::scan_workspace_bytes(dtype, n) > bulkx::kNoAllocScratchBytes)
+    return BULKX_ERR_WORKSPACE_TOO_SMALL;
+  char* ws = nullptr;
+  BULKX_TRY(bulkx::noalloc_scratch(&ws));
+  // the scan's preparation kernel clears the header and the status words
+  // of every call, so the previous call's contents never leak in
+  return bulkx::scan_entry<false>(dtype, op, in, out, n, init, ws, bulkx::kNoAllocScratchBytes,
+                                  stream);
 }
 
 extern "C" int bulkx_exclusive_scan(int dtype, int op, const void* in, void* out, int64_t n,

@@ -159,6 +159,22 @@ int bulkx_inclusive_scan(int dtype, int op, const void* in, void* out, int64_t n
                          void* stream);
 
 /*
+ * inclusive_scan_noalloc -- bulkx_inclusive_scan with no workspace argument
+ * and no allocation of any kind: the look-back scratch is a static 4 MiB
+ * (+256 B) device array that the library holds on each device until the
+ * process ends.  No counterpart in the reference; for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Limits: n <= 2^32 for int32/uint32, 2^31 for
+ * float32, 2^30 for int64/uint64, 2^29 for float64; a larger n returns
+ * BULKX_ERR_WORKSPACE_TOO_SMALL before anything is enqueued.  Every call on
+ * one GPU shares that GPU's array: two calls whose executions overlap on
+ * one GPU (different streams, no ordering between them) race and give wrong
+ * results; calls ordered on one stream are safe.
+ */
+int bulkx_inclusive_scan_noalloc(int dtype, int op, const void* in, void* out, int64_t n,
+                                 const void* init, void* stream);
+
+/*
  * exclusive_scan -- device-wide form of bulk::exclusive_scan(group, ...),
  * scan.hpp:553-591: result[0] = init, result[i] = init op x[0] op ... op x[i-1].
  * init must not be NULL.  in == out is allowed.
