@@ -215,8 +215,8 @@ def main():
     _log(f"batched: {np.mean(bt_ms):.2f} ms/step")
 
     # per-stage device time: eager launches timed with HIP events on the
-    # batch's stream (one event pair around each phase-0 family kernel and
-    # around all sort merge rounds)
+    # batch's stream (one event pair around each phase-0 family kernel, the
+    # radix passes, and each pass kernel)
     reps = max(3, args.steps)
     acc = {k: 0.0 for k in Batch.STAGES}
     for _ in range(reps):
@@ -236,11 +236,16 @@ def main():
                 j.algorithmic_bytes(runs.get(i))
         elif k == C.JOB_MERGE_BY_KEY:
             stage_bytes["merge_by_key"] += j.algorithmic_bytes()
-        else:
+        elif Batch.sort_passes(n) == 0:
             stage_bytes["sort_block"] += j.algorithmic_bytes()
-            stage_bytes["sort_rounds"] += Batch.sort_rounds(n) * 16 * n
-    stage_bytes["sort_pass_kernels"] = stage_bytes["sort_rounds"]
-    max_rounds = max([Batch.sort_rounds(j.spec.n) for j in jobs if j.spec.kind == C.JOB_SORT_BY_KEY] + [0])
+        else:
+            # compulsory: every key and value read and written once (2*(4+4)*n);
+            # each of the 4 downsweep launches moves exactly that, each
+            # upsweep reads the keys (4*n)
+            stage_bytes["sort_radix"] += j.algorithmic_bytes()
+            stage_bytes["sort_downsweep_kernels"] += 4 * 16 * n
+            stage_bytes["sort_upsweep_kernels"] += 4 * 4 * n
+    radix_launches = 4
     stages = {k: {"ms": acc[k], "bytes": stage_bytes[k],
                   "gbs": stage_bytes[k] / (acc[k] * 1e-3) / 1e9 if acc[k] > 0 else None}
               for k in Batch.STAGES}
@@ -248,12 +253,14 @@ def main():
         if v["bytes"]:
             _log(f"stage {k:18s} {v['ms']:7.3f} ms {v['gbs'] or 0:7.0f} GB/s")
     # roofline: the kernel with the largest device time per step -- a
-    # phase-0 family kernel (one launch) or the merge-pass kernel (one launch
-    # per sort round); per-launch bytes and duration are the step's averages
+    # phase-0 family kernel (one launch) or the radix downsweep or upsweep
+    # kernel (one launch per radix pass); per-launch bytes and duration are
+    # the step's averages
     kernel_of = {k: f"batch_phase0_kernel<{i}>" for i, k in enumerate(Batch.STAGES[:7])}
-    kernel_of["sort_pass_kernels"] = "batch_pass_kernel"
+    kernel_of["sort_downsweep_kernels"] = "batch_radix_downsweep_kernel"
+    kernel_of["sort_upsweep_kernels"] = "batch_radix_upsweep_kernel"
     dom = max(kernel_of, key=lambda k: acc[k])
-    dom_launches = max(1, max_rounds) if dom == "sort_pass_kernels" else 1
+    dom_launches = radix_launches if dom in ("sort_downsweep_kernels", "sort_upsweep_kernels") else 1
     dom_gbs = stages[dom]["gbs"]
 
     # ---------------------------------------------------------------- serving

@@ -1,9 +1,10 @@
 """Python front end of the batch executor (bulkx_batch_* in include/bulkx.h).
 
 A job list is a sequence of ``Job`` records over device tensors; ``Batch``
-runs the whole list as one persistent launch (plus one per sort merge
-round), recorded into a hipGraph on first use, or as one plain call per job
-(the per-call baseline).
+runs the whole list as segmented work (one launch per op family over the
+tiles of all its jobs, plus the radix passes of the larger sort jobs),
+recorded into a hipGraph on first use, or as one plain call per job (the
+per-call baseline).
 """
 from __future__ import annotations
 
@@ -103,20 +104,23 @@ class Batch:
         return a.value, b.value
 
     STAGES = ("scan_i32", "scan_f32", "reduce_by_key_i32", "reduce_by_key_f32",
-              "reduce_by_key_i64", "merge_by_key", "sort_block", "sort_rounds",
-              "sort_pass_kernels", "sort_partition_kernels")
+              "reduce_by_key_i64", "merge_by_key", "sort_block", "sort_radix",
+              "sort_downsweep_kernels", "sort_upsweep_kernels", "sort_count_scan_kernels")
 
     @staticmethod
-    def sort_rounds(n: int) -> int:
-        """Merge rounds the batch runs for a sort job of n elements."""
+    def sort_geometry() -> tuple[int, int]:
+        """(largest sort job block-sorted in one tile, radix tile)."""
         run, tile = ctypes.c_int64(), ctypes.c_int64()
         C.check(C.lib().bulkx_batch_sort_geometry(ctypes.byref(run), ctypes.byref(tile)),
                 "bulkx_batch_sort_geometry")
-        runs = (n + run.value - 1) // run.value
-        p = 0
-        while (1 << p) < runs:
-            p += 1
-        return p
+        return run.value, tile.value
+
+    @staticmethod
+    def sort_passes(n: int) -> int:
+        """Radix passes the batch runs for a sort job of n elements (0: the
+        job is one block-sorted tile)."""
+        run, _ = Batch.sort_geometry()
+        return 0 if n <= run else 4
 
     def stage_timing(self) -> dict[str, float]:
         """Per-stage ms of the last eager launch (HIP events on the batch's stream)."""

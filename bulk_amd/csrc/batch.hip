@@ -1,21 +1,28 @@
 // Batch executor: many uneven scan / reduce / reduce_by_key / merge_by_key /
-// sort_by_key jobs run as segmented work in one persistent launch (plus one
-// launch per sort merge round), recorded once into a hipGraph.
+// sort_by_key jobs run as segmented work, a fixed sequence of launches
+// recorded once into a hipGraph.
 //
-// Work decomposition.  Every job is cut into fixed-size tiles; the tiles of
-// all jobs form one global queue, ordered by job.  A persistent grid of
-// resident workgroups claims tiles from an atomic counter, maps the tile to
-// its job through a device-resident tile->job table and runs the same tile
-// body the device-wide kernel runs (scan_tile, rbk_tile, merge_two_tile,
-// block_sort_tile).  Scan, reduce and reduce_by_key jobs keep a decoupled
-// look-back chain of their own (per-job status words); because tiles are
-// claimed in queue order, a tile's predecessors always belong to running or
-// finished workgroups.  Sort jobs need global order between merge rounds,
-// so each round is one more persistent launch over the sort tiles.
+// Work decomposition.  Every job is cut into fixed-size tiles, and the tiles
+// of the jobs of one op family (int32 scan/reduce, float32 scan/reduce, the
+// three reduce_by_key value types, merge, sort) form that family's tile
+// table.  Each family is one launch with one workgroup per tile (the grids
+// follow the job list); a workgroup maps its tile to the job through the
+// device-resident tile->job table and runs the tile body the device-wide
+// kernels run (scan_tile, rbk_tile, merge_two_tile, block_sort_tile).  Scan
+// and reduce jobs keep a decoupled look-back chain of their own (per-job
+// status words, tiles claimed from a per-job counter, so a tile's
+// predecessors always run or are done); reduce_by_key jobs run the
+// three-pass form per job (tile aggregates, one scan workgroup per job,
+// then the tiles); merge jobs read their tile boundaries from a partition
+// launch.  Sort jobs of one tile are block-sorted; larger ones run the four
+// radix passes (upsweep, per-job counts scan, downsweep) over all their
+// tiles together.  The graph forks into four concurrent branches (sort,
+// merge, reduce_by_key, scan) after one clear kernel.
 //
 // Everything a launch needs (tile counts, tables, counters) lives in device
 // memory, so the captured graph stays valid when the job list changes: a new
-// list is uploaded with stream-ordered copies and the graph is replayed.
+// list is uploaded with stream-ordered copies, the size-dependent grids are
+// updated in place, and the graph is replayed.
 #include <algorithm>
 #include <vector>
 
@@ -34,19 +41,19 @@ constexpr int kBatchMergeItems = 8;   // 4096 outputs per merge / sort tile
 constexpr int64_t kBatchScanTile = int64_t(kBatchBlock) * kBatchScanItems;
 constexpr int64_t kBatchRbkTile = int64_t(kBatchBlock) * kBatchRbkItems;
 constexpr int64_t kBatchMergeTile = int64_t(kBatchBlock) * kBatchMergeItems;
-// The sort family's block sort writes runs of 8192 elements (512 x 16 in
-// LDS, two workgroups per CU): one merge round fewer than runs of one merge
-// tile.  Measured on the mixed stream: block sort + rounds 7.9 ms with these
-// runs, 8.5 ms with 4096-element runs, 8.1 ms with 16384 (1024 x 16, one
-// workgroup per CU).
+// Sort jobs of at most one block-sort tile (512 x 16 = 8192 keys, in LDS)
+// are sorted by the block sort alone; larger ones by the radix passes
+// (radix_impl.hpp: four digit passes, each an upsweep of per-tile digit
+// counts, a scan of the job's counts and a downsweep that scatters keys and
+// values), which read and write a job 4 times however large it is.
 constexpr int kBatchSortBlock = 512;
 constexpr int kBatchSortItems = 16;
 constexpr int64_t kBatchSortRun = int64_t(kBatchSortBlock) * kBatchSortItems;
-static_assert(kBatchSortRun % kBatchMergeTile == 0, "runs are whole merge tiles");
+using BatchRadix = RadixConfig<uint32_t>;
+constexpr int64_t kBatchRadixTile = BatchRadix::TILE;
 constexpr int64_t kBatchMinTile = kBatchScanTile < kBatchRbkTile
                                       ? (kBatchScanTile < kBatchMergeTile ? kBatchScanTile : kBatchMergeTile)
                                       : (kBatchRbkTile < kBatchMergeTile ? kBatchRbkTile : kBatchMergeTile);
-constexpr int kBatchMaxPasses = 24;
 constexpr int kStatusWordsPerTile = 6;  // widest look-back record used here (24 bytes)
 
 enum BatchOp : int32_t {
@@ -60,7 +67,7 @@ enum BatchOp : int32_t {
 
 struct BatchJobDev {
   int32_t op;
-  int32_t passes;  // sort: merge rounds
+  int32_t passes;  // sort: radix passes (0: one block-sorted tile)
   int64_t n0, n1;
   const void* in0;
   const void* in1;
@@ -73,36 +80,40 @@ struct BatchJobDev {
   int64_t* splits;   // merge: the merge path of every tile boundary (tiles0 + 1)
   void* tmp_k;       // sort ping-pong buffers
   void* tmp_v;
+  uint32_t* counts;  // sort: digit counts of the radix tiles (256 per tile, digit-major)
   int64_t tiles0;       // phase-0 tiles
   int64_t tile_begin0;  // first global phase-0 tile
-  int64_t sort_tile_begin;
+  int64_t sort_tile_begin;  // first global radix tile
+  int64_t sort_tiles;       // radix tiles
 };
 
-// Phase-0 work is split by op family; each family is one persistent kernel
+// Phase-0 work is split by op family; each family is one kernel
 // compiled for its own register and LDS footprint (a single kernel over all
 // ops would run at the occupancy of the heaviest body).
 enum BatchFamily : int32_t {
   FAM_SCAN_I32 = 0, FAM_SCAN_F32, FAM_RBK_I32, FAM_RBK_F32, FAM_RBK_I64, FAM_MERGE, FAM_SORT,
   FAM_COUNT
 };
-constexpr int kBatchStages = FAM_COUNT + 1;  // the family kernels, then the sort merge rounds
-// Launch-grid kinds: one per phase-0 family, then (partition, pass) for
-// every merge round, then the aggregate scans of the three reduce_by_key
-// families.
+constexpr int kBatchStages = FAM_COUNT + 1;  // the family kernels, then the sort radix passes
+// Launch-grid kinds: one per phase-0 family, the radix passes' upsweep, scan
+// and downsweep, the aggregate scans of the three reduce_by_key families,
+// and the merge partition.
 constexpr int kRbkFamilies = 3;
-constexpr int kGridKinds = FAM_COUNT + 2 * kBatchMaxPasses + kRbkFamilies + 1;
+constexpr int kGridRadixUp = FAM_COUNT;
+constexpr int kGridRadixScan = FAM_COUNT + 1;
+constexpr int kGridRadixDown = FAM_COUNT + 2;
+constexpr int grid_kind_rbk_scan(int fam) { return FAM_COUNT + 3 + fam - FAM_RBK_I32; }
+constexpr int kGridMergePartition = FAM_COUNT + 3 + kRbkFamilies;
+constexpr int kGridKinds = kGridMergePartition + 1;
 constexpr int kBatchBranches = 4;
-constexpr int grid_kind_partition(int round) { return FAM_COUNT + 2 * round; }
-constexpr int grid_kind_pass(int round) { return FAM_COUNT + 2 * round + 1; }
-constexpr int grid_kind_rbk_scan(int fam) { return FAM_COUNT + 2 * kBatchMaxPasses + fam - FAM_RBK_I32; }
-constexpr int kGridMergePartition = FAM_COUNT + 2 * kBatchMaxPasses + kRbkFamilies;
 
 struct BatchDesc {
   int64_t fam_begin[FAM_COUNT + 1];  // family f owns tiles [fam_begin[f], fam_begin[f+1])
   // multi-tile reduce_by_key jobs of family FAM_RBK_I32 + r:
   // rbk_jobs[rbk_begin[r], rbk_begin[r + 1])
   int64_t rbk_begin[kRbkFamilies + 1];
-  int64_t total_sort_tiles;
+  int64_t total_sort_tiles;  // radix tiles of all sort jobs
+  int64_t num_radix_jobs;    // sort jobs that run the radix passes
   int64_t num_status_words;
   int64_t num_jobs;
   LookbackGuard guard;  // the look-back waits of every family; err cleared per launch
@@ -146,6 +157,14 @@ struct FamilySmem<FAM_SORT> {
     SortSmem<float, uint32_t, true, kBatchSortBlock, kBatchSortItems> f;
   };
 };
+// The radix downsweep's LDS, one layout for every 4-byte key type.
+using BatchRadixSmem = RadixSmem<uint32_t, uint32_t, true, BatchRadix::BLOCK, BatchRadix::ITEMS>;
+template <class K>
+__device__ __forceinline__ RadixSmem<K, uint32_t, true, BatchRadix::BLOCK, BatchRadix::ITEMS>&
+radix_smem(BatchRadixSmem& sm) {
+  static_assert(sizeof(K) == 4, "4-byte keys share the layout");
+  return reinterpret_cast<RadixSmem<K, uint32_t, true, BatchRadix::BLOCK, BatchRadix::ITEMS>&>(sm);
+}
 
 __global__ void batch_clear_kernel(BatchDesc* d, uint64_t* status, uint32_t* job_ctr) {
   const int64_t nw = d->num_status_words;
@@ -158,13 +177,11 @@ __global__ void batch_clear_kernel(BatchDesc* d, uint64_t* status, uint32_t* job
 
 template <class K, class SM>
 __device__ __forceinline__ void sort_block_job(SM& sm, const BatchJobDev& job, int64_t lt) {
+  // a job of one tile, sorted in place (larger jobs run the radix passes)
   K* keys = static_cast<K*>(job.out0);
   uint32_t* vals = static_cast<uint32_t*>(job.out1);
-  // an odd number of rounds starts in the temporary so the last lands in place
-  K* dk = (job.passes & 1) ? static_cast<K*>(job.tmp_k) : keys;
-  uint32_t* dv = (job.passes & 1) ? static_cast<uint32_t*>(job.tmp_v) : vals;
   block_sort_tile<CmpLess, K, uint32_t, true, kBatchSortBlock, kBatchSortItems>(
-      sm, keys, vals, job.n0, dk, dv, lt);
+      sm, keys, vals, job.n0, keys, vals, lt);
 }
 
 // Value type of a reduce_by_key family, and its tile aggregates (kept in the
@@ -323,30 +340,17 @@ __global__ __launch_bounds__(256) void batch_rbk_scan_kernel(BatchDesc* d,
                                               job.tiles0);
 }
 
+// Source and destination of a sort job's radix pass `pass`: even passes
+// read the caller's arrays and write the temporaries, odd passes the
+// reverse, so the fourth pass ends in place.
 template <class K>
-__device__ __forceinline__ MergeSmem<K, uint32_t, true, kBatchBlock, kBatchMergeItems>& merge_smem(
-    FamilySmem<FAM_MERGE>& sm);
-template <>
-__device__ __forceinline__ MergeSmem<int32_t, uint32_t, true, kBatchBlock, kBatchMergeItems>&
-merge_smem<int32_t>(FamilySmem<FAM_MERGE>& sm) { return sm.i; }
-template <>
-__device__ __forceinline__ MergeSmem<float, uint32_t, true, kBatchBlock, kBatchMergeItems>&
-merge_smem<float>(FamilySmem<FAM_MERGE>& sm) { return sm.f; }
-template <>
-__device__ __forceinline__ MergeSmem<uint32_t, uint32_t, true, kBatchBlock, kBatchMergeItems>&
-merge_smem<uint32_t>(FamilySmem<FAM_MERGE>& sm) { return sm.u; }
-
-// Source and destination of a sort job's merge round `pass`: the block sort
-// wrote to "first" (the temporary for an odd number of rounds, so the last
-// round lands in place) and rounds alternate from there.
-template <class K>
-struct PassBuffers {
+struct RadixBuffers {
   const K* sk;
   const uint32_t* sv;
   K* dk;
   uint32_t* dv;
-  __device__ PassBuffers(const BatchJobDev& job, int pass) {
-    const bool src_is_tmp = ((job.passes & 1) != 0) ^ ((pass & 1) != 0);
+  __device__ RadixBuffers(const BatchJobDev& job, int pass) {
+    const bool src_is_tmp = (pass & 1) != 0;
     K* ok = static_cast<K*>(job.out0);
     uint32_t* ov = static_cast<uint32_t*>(job.out1);
     K* tk = static_cast<K*>(job.tmp_k);
@@ -358,71 +362,74 @@ struct PassBuffers {
   }
 };
 
-// Merge paths of the first diagonal of every sort tile of round `pass`,
-// one lane per tile (tiles of jobs that need fewer rounds are skipped).
-__global__ __launch_bounds__(256) void batch_partition_kernel(
+// Digit counts of every radix tile of pass `pass` (one workgroup per tile).
+__global__ __launch_bounds__(BatchRadix::BLOCK) void batch_radix_upsweep_kernel(
     BatchDesc* d, const BatchJobDev* __restrict__ jobs, const uint32_t* __restrict__ sort_tile_job,
-    int64_t* __restrict__ splits, int pass) {
-  const int64_t t = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (t >= d->total_sort_tiles) return;
-  const BatchJobDev& job = jobs[sort_tile_job[t]];
-  if (pass >= job.passes) return;
-  const int64_t lt = t - job.sort_tile_begin;
-  const int64_t width = kBatchSortRun << pass;
-  int64_t r;
-  switch (job.op) {
-    case OP_SORT_I32:
-      r = pass_split<CmpLess, int32_t, kBatchMergeTile>(PassBuffers<int32_t>(job, pass).sk,
-                                                        job.n0, width, lt);
-      break;
-    case OP_SORT_U32:
-      r = pass_split<CmpLess, uint32_t, kBatchMergeTile>(PassBuffers<uint32_t>(job, pass).sk,
-                                                         job.n0, width, lt);
-      break;
-    default:
-      r = pass_split<CmpLess, float, kBatchMergeTile>(PassBuffers<float>(job, pass).sk, job.n0,
-                                                      width, lt);
-      break;
-  }
-  splits[t] = r;
-}
-
-template <class K>
-__device__ __forceinline__ void sort_pass_job(FamilySmem<FAM_MERGE>& sm, const BatchJobDev& job,
-                                              int pass, int64_t lt, const int64_t* splits) {
-  const PassBuffers<K> pb(job, pass);
-  merge_pass_tile<CmpLess, K, uint32_t, true, kBatchBlock, kBatchMergeItems>(
-      merge_smem<K>(sm), pb.sk, pb.sv, job.n0, kBatchSortRun << pass, pb.dk, pb.dv, lt,
-      splits + job.sort_tile_begin);
-}
-
-// Merge round `pass` of every sort job: one workgroup per sort tile (the
-// host sizes the grid to the job list's sort tiles).
-__global__ __launch_bounds__(kBatchBlock) void batch_pass_kernel(
-    BatchDesc* d, const BatchJobDev* __restrict__ jobs, const uint32_t* __restrict__ sort_tile_job,
-    const int64_t* __restrict__ splits, int pass) {
-  __shared__ FamilySmem<FAM_MERGE> sm;
+    int pass) {
+  __shared__ uint32_t s_hist[BatchRadix::BLOCK / kWave][kRadixDigits];
   const int64_t t = blockIdx.x;
   if (t >= d->total_sort_tiles) return;
   const BatchJobDev& job = jobs[sort_tile_job[t]];
-  if (pass >= job.passes) return;
+  const int64_t lt = t - job.sort_tile_begin;
+  const int shift = pass * kRadixBits;
+  switch (job.op) {
+    case OP_SORT_I32:
+      radix_count_tiles<int32_t, false, BatchRadix::BLOCK, BatchRadix::ITEMS, 1>(
+          s_hist, RadixBuffers<int32_t>(job, pass).sk, job.n0, shift, job.counts, job.sort_tiles, lt);
+      break;
+    case OP_SORT_U32:
+      radix_count_tiles<uint32_t, false, BatchRadix::BLOCK, BatchRadix::ITEMS, 1>(
+          s_hist, RadixBuffers<uint32_t>(job, pass).sk, job.n0, shift, job.counts, job.sort_tiles, lt);
+      break;
+    default:
+      radix_count_tiles<float, false, BatchRadix::BLOCK, BatchRadix::ITEMS, 1>(
+          s_hist, RadixBuffers<float>(job, pass).sk, job.n0, shift, job.counts, job.sort_tiles, lt);
+      break;
+  }
+}
+
+// Exclusive scan of each radix job's counts (one workgroup per job): the
+// global position of every (digit, tile) run within the job.
+__global__ __launch_bounds__(1024) void batch_radix_scan_kernel(BatchDesc* d,
+                                                                const BatchJobDev* __restrict__ jobs,
+                                                                const uint32_t* __restrict__ radix_jobs) {
+  __shared__ uint32_t s_wave[1024 / kWave];
+  const int64_t b = blockIdx.x;
+  if (b >= d->num_radix_jobs) return;
+  const BatchJobDev& job = jobs[radix_jobs[b]];
+  radix_scan_counts_block(job.counts, job.sort_tiles * kRadixDigits, s_wave);
+}
+
+template <class K>
+__device__ __forceinline__ void radix_down_job(BatchRadixSmem& sm, const BatchJobDev& job,
+                                               int pass, int64_t lt) {
+  const RadixBuffers<K> rb(job, pass);
+  radix_scatter_tile<K, uint32_t, true, false, BatchRadix::BLOCK, BatchRadix::ITEMS>(
+      radix_smem<K>(sm), rb.sk, rb.sv, job.n0, pass * kRadixBits, job.counts, job.sort_tiles,
+      rb.dk, rb.dv, lt);
+}
+
+// Scatter of pass `pass` for every radix tile; each XCD takes a contiguous
+// range of the tiles (xcd_tile), so a job's neighbouring tiles share L2.
+__global__ __launch_bounds__(BatchRadix::BLOCK) void batch_radix_downsweep_kernel(
+    BatchDesc* d, const BatchJobDev* __restrict__ jobs, const uint32_t* __restrict__ sort_tile_job,
+    int pass) {
+  __shared__ BatchRadixSmem sm;
+  const int64_t total = d->total_sort_tiles;
+  if (int64_t(blockIdx.x) >= total) return;
+  const int64_t t = xcd_tile(blockIdx.x, total);
+  const BatchJobDev& job = jobs[sort_tile_job[t]];
   const int64_t lt = t - job.sort_tile_begin;
   switch (job.op) {
-    case OP_SORT_I32: sort_pass_job<int32_t>(sm, job, pass, lt, splits); break;
-    case OP_SORT_U32: sort_pass_job<uint32_t>(sm, job, pass, lt, splits); break;
-    default: sort_pass_job<float>(sm, job, pass, lt, splits); break;
+    case OP_SORT_I32: radix_down_job<int32_t>(sm, job, pass, lt); break;
+    case OP_SORT_U32: radix_down_job<uint32_t>(sm, job, pass, lt); break;
+    default: radix_down_job<float>(sm, job, pass, lt); break;
   }
 }
 
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
-static int passes_for(int64_t tiles) {
-  int p = 0;
-  while ((int64_t(1) << p) < tiles) ++p;
-  return p;
-}
-
 static int batch_op(const bulkx_job& j) {
   switch (j.kind) {
     case BULKX_JOB_INCLUSIVE_SCAN:
@@ -521,7 +528,6 @@ struct bulkx_batch {
   int max_jobs = 0;
   int64_t max_elements = 0;
   int64_t max_tiles = 0;
-  int max_passes = 0;  // rounds the largest allowed job could need
   int64_t fam_tiles[FAM_COUNT] = {};  // phase-0 grid of each family = its tile count
   // device
   BatchDesc* d_desc = nullptr;
@@ -529,8 +535,8 @@ struct bulkx_batch {
   uint32_t* d_tile_job = nullptr;
   uint32_t* d_sort_tile_job = nullptr;
   uint32_t* d_rbk_jobs = nullptr;  // multi-tile reduce_by_key jobs, by family
+  uint32_t* d_radix_jobs = nullptr;  // sort jobs that run the radix passes
   int64_t* d_merge_splits = nullptr;  // tile boundaries of the merge jobs
-  int64_t* d_splits = nullptr;  // merge-path splits of the current merge round
   uint32_t* d_job_ctr = nullptr;  // per-job tile counters of the look-back families
   uint64_t* d_status = nullptr;
   char* d_tmp = nullptr;
@@ -544,47 +550,42 @@ struct bulkx_batch {
   uint32_t* h_tile_job = nullptr;
   uint32_t* h_sort_tile_job = nullptr;
   uint32_t* h_rbk_jobs = nullptr;
+  uint32_t* h_radix_jobs = nullptr;
   uint32_t* h_err = nullptr;  // the last launch's error word, copied back behind it
   bool last_per_call = false;  // the last launch ran the plain entry points
   int64_t rbk_jobs[kRbkFamilies] = {};  // multi-tile jobs of each reduce_by_key family
   // current job list
   std::vector<bulkx_job> jobs;
   int num_jobs = 0;
-  int64_t total_tiles0 = 0, total_sort_tiles = 0;
-  // sort tiles still merging in round r (sort tiles are ordered by rounds,
-  // most first, so they are a prefix of the sort tile table)
-  int64_t active_sort_tiles[kBatchMaxPasses] = {};
+  int64_t total_tiles0 = 0, total_sort_tiles = 0, num_radix_jobs = 0;
   // stage timing of eager launches: ev[f]..ev[f+1] brackets family f,
-  // ev[FAM_COUNT]..ev[FAM_COUNT+1] the sort merge rounds; in round r,
-  // rev[2r]..rev[2r+1] brackets the partition kernel, rev[2r+1]..rev[2r+2]
-  // the pass kernel
+  // ev[FAM_COUNT]..ev[FAM_COUNT+1] the sort radix passes; in pass p,
+  // rev[3p]..rev[3p+1] brackets the upsweep, rev[3p+1]..rev[3p+2] the
+  // counts scan and rev[3p+2]..rev[3p+3] the downsweep
   hipEvent_t ev[kBatchStages + 1] = {};
-  hipEvent_t rev[2 * kBatchMaxPasses + 1] = {};
-  int timed_passes = 0;
+  hipEvent_t rev[3 * kRadixPasses + 1] = {};
   bool timed = false;
-  // merge rounds the current job list needs (its largest sort job's)
-  int rounds = 0;
-  // one graph per number of merge rounds, recorded on first use, so a launch
-  // never runs a round with nothing to merge
-  BatchGraph graphs[kBatchMaxPasses + 1];
+  // the recorded launch sequence (one graph for every job list)
+  BatchGraph graph;
 };
 
 // Grid of each job-list-sized launch: the phase-0 families (one workgroup per
-// tile), and per merge round the partition (one wavefront per sort tile still
-// merging) and the round itself (one workgroup per such tile).  Never 0: an
-// idle launch runs one workgroup that finds no work.
+// tile), the radix passes (one workgroup per radix tile; the counts scan one
+// per radix job), the reduce_by_key aggregate scans (one per multi-tile job)
+// and the merge partition (one lane per merge tile).  Never 0: an idle
+// launch runs one workgroup that finds no work.
 static unsigned batch_grid(const bulkx_batch* b, int kind) {
   int64_t g;
   if (kind < FAM_COUNT) {
     g = b->fam_tiles[kind];
   } else if (kind == kGridMergePartition) {
     g = (b->fam_tiles[FAM_MERGE] + 255) / 256;
-  } else if (kind >= grid_kind_rbk_scan(FAM_RBK_I32)) {
-    g = b->rbk_jobs[kind - grid_kind_rbk_scan(FAM_RBK_I32)];
+  } else if (kind == kGridRadixUp || kind == kGridRadixDown) {
+    g = b->total_sort_tiles;
+  } else if (kind == kGridRadixScan) {
+    g = b->num_radix_jobs;
   } else {
-    const int r = (kind - FAM_COUNT) / 2;
-    const int64_t active = b->active_sort_tiles[r];
-    g = ((kind - FAM_COUNT) & 1) ? active : (active + 255) / 256;
+    g = b->rbk_jobs[kind - grid_kind_rbk_scan(FAM_RBK_I32)];
   }
   return unsigned(g > 0 ? g : 1);
 }
@@ -628,6 +629,27 @@ static int batch_enqueue_family(bulkx_batch* b, int f, hipStream_t q) {
   return BULKX_SUCCESS;
 }
 
+// The radix passes of every sort job that has more than one tile.  With
+// `rev` the pass kernels are bracketed by events (eager launches only).
+static int batch_enqueue_radix(bulkx_batch* b, hipStream_t s, const hipEvent_t* rev) {
+  for (int p = 0; p < kRadixPasses; ++p) {
+    if (rev) BULKX_HIP(hipEventRecord(rev[3 * p], s));
+    batch_radix_upsweep_kernel<<<batch_grid(b, kGridRadixUp), BatchRadix::BLOCK, 0, s>>>(
+        b->d_desc, b->d_jobs, b->d_sort_tile_job, p);
+    BULKX_LAUNCHED();
+    if (rev) BULKX_HIP(hipEventRecord(rev[3 * p + 1], s));
+    batch_radix_scan_kernel<<<batch_grid(b, kGridRadixScan), 1024, 0, s>>>(b->d_desc, b->d_jobs,
+                                                                           b->d_radix_jobs);
+    BULKX_LAUNCHED();
+    if (rev) BULKX_HIP(hipEventRecord(rev[3 * p + 2], s));
+    batch_radix_downsweep_kernel<<<batch_grid(b, kGridRadixDown), BatchRadix::BLOCK, 0, s>>>(
+        b->d_desc, b->d_jobs, b->d_sort_tile_job, p);
+    BULKX_LAUNCHED();
+  }
+  if (rev) BULKX_HIP(hipEventRecord(rev[3 * kRadixPasses], s));
+  return BULKX_SUCCESS;
+}
+
 static int batch_enqueue_kernels(bulkx_batch* b, hipStream_t s, bool timed) {
   batch_clear_kernel<<<256, 256, 0, s>>>(b->d_desc, b->d_status, b->d_job_ctr);
   BULKX_LAUNCHED();
@@ -636,24 +658,14 @@ static int batch_enqueue_kernels(bulkx_batch* b, hipStream_t s, bool timed) {
     BULKX_TRY(batch_enqueue_family(b, f, s));
   }
   if (timed) BULKX_HIP(hipEventRecord(b->ev[FAM_COUNT], s));
-  for (int p = 0; p < b->rounds; ++p) {
-    if (timed) BULKX_HIP(hipEventRecord(b->rev[2 * p], s));
-    batch_partition_kernel<<<batch_grid(b, grid_kind_partition(p)), 256, 0, s>>>(
-        b->d_desc, b->d_jobs, b->d_sort_tile_job, b->d_splits, p);
-    BULKX_LAUNCHED();
-    if (timed) BULKX_HIP(hipEventRecord(b->rev[2 * p + 1], s));
-    batch_pass_kernel<<<batch_grid(b, grid_kind_pass(p)), kBatchBlock, 0, s>>>(
-        b->d_desc, b->d_jobs, b->d_sort_tile_job, b->d_splits, p);
-    BULKX_LAUNCHED();
-    if (timed) BULKX_HIP(hipEventRecord(b->rev[2 * p + 2], s));
-  }
+  BULKX_TRY(batch_enqueue_radix(b, s, timed ? b->rev : nullptr));
   if (timed) BULKX_HIP(hipEventRecord(b->ev[FAM_COUNT + 1], s));
-  b->timed_passes = b->rounds;
   return BULKX_SUCCESS;
 }
 
 // The graph form: after the clear kernel the work forks into independent
-// branches -- the sort chain (block sort, then the merge rounds), merge,
+// branches -- the sort chain (block sort of one-tile jobs, then the radix
+// passes of the others), merge,
 // reduce_by_key and scan -- which the device runs concurrently, so the
 // compute-bound sort block and the look-back stalls of reduce_by_key overlap
 // the memory-bound families.  Branches share only read-only tables and the
@@ -664,14 +676,7 @@ static int batch_enqueue_graph(bulkx_batch* b, hipStream_t s) {
   BULKX_HIP(hipEventRecord(b->fork_ev, s));
   for (int br = 0; br < kBatchBranches; ++br) BULKX_HIP(hipStreamWaitEvent(b->side[br], b->fork_ev, 0));
   BULKX_TRY(batch_enqueue_family(b, FAM_SORT, b->side[0]));
-  for (int p = 0; p < b->rounds; ++p) {
-    batch_partition_kernel<<<batch_grid(b, grid_kind_partition(p)), 256, 0, b->side[0]>>>(
-        b->d_desc, b->d_jobs, b->d_sort_tile_job, b->d_splits, p);
-    BULKX_LAUNCHED();
-    batch_pass_kernel<<<batch_grid(b, grid_kind_pass(p)), kBatchBlock, 0, b->side[0]>>>(
-        b->d_desc, b->d_jobs, b->d_sort_tile_job, b->d_splits, p);
-    BULKX_LAUNCHED();
-  }
+  BULKX_TRY(batch_enqueue_radix(b, b->side[0], nullptr));
   BULKX_TRY(batch_enqueue_family(b, FAM_MERGE, b->side[1]));
   for (int f : {int(FAM_RBK_I32), int(FAM_RBK_F32), int(FAM_RBK_I64)})
     BULKX_TRY(batch_enqueue_family(b, f, b->side[2]));
@@ -697,8 +702,8 @@ extern "C" int bulkx_batch_create(int max_jobs, int64_t max_elements, void* stre
   b->max_elements = max_elements;
   // every job can add one partial tile
   b->max_tiles = max_elements / kBatchMinTile + 2 * int64_t(max_jobs) + 1;
-  b->max_passes = passes_for((max_elements + kBatchSortRun - 1) / kBatchSortRun);
-  if (b->max_passes > kBatchMaxPasses) {
+  // radix positions are 32-bit per job; a batch never holds a larger job
+  if (max_elements > kRadixMaxN) {
     delete b;
     return BULKX_ERR_INVALID_ARGUMENT;
   }
@@ -727,9 +732,9 @@ extern "C" int bulkx_batch_create(int max_jobs, int64_t max_elements, void* stre
   dmalloc(reinterpret_cast<void**>(&b->d_tile_job), sizeof(uint32_t) * b->max_tiles);
   dmalloc(reinterpret_cast<void**>(&b->d_sort_tile_job), sizeof(uint32_t) * b->max_tiles);
   dmalloc(reinterpret_cast<void**>(&b->d_rbk_jobs), sizeof(uint32_t) * max_jobs);
+  dmalloc(reinterpret_cast<void**>(&b->d_radix_jobs), sizeof(uint32_t) * max_jobs);
   dmalloc(reinterpret_cast<void**>(&b->d_merge_splits),
           sizeof(int64_t) * (b->max_tiles + int64_t(max_jobs)));
-  dmalloc(reinterpret_cast<void**>(&b->d_splits), sizeof(int64_t) * b->max_tiles);
   dmalloc(reinterpret_cast<void**>(&b->d_job_ctr), sizeof(uint32_t) * max_jobs);
   dmalloc(reinterpret_cast<void**>(&b->d_status), sizeof(uint64_t) * b->status_words_cap);
   dmalloc(reinterpret_cast<void**>(&b->d_ws), b->ws_bytes);
@@ -738,6 +743,7 @@ extern "C" int bulkx_batch_create(int max_jobs, int64_t max_elements, void* stre
   hmalloc(reinterpret_cast<void**>(&b->h_tile_job), sizeof(uint32_t) * b->max_tiles);
   hmalloc(reinterpret_cast<void**>(&b->h_sort_tile_job), sizeof(uint32_t) * b->max_tiles);
   hmalloc(reinterpret_cast<void**>(&b->h_rbk_jobs), sizeof(uint32_t) * max_jobs);
+  hmalloc(reinterpret_cast<void**>(&b->h_radix_jobs), sizeof(uint32_t) * max_jobs);
   hmalloc(reinterpret_cast<void**>(&b->h_err), sizeof(uint32_t));
   if (st != BULKX_SUCCESS) {
     bulkx_batch_destroy(b);
@@ -748,7 +754,7 @@ extern "C" int bulkx_batch_create(int max_jobs, int64_t max_elements, void* stre
   *b->h_err = 0;
   hipError_t r = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking);
   for (int i = 0; i <= kBatchStages && r == hipSuccess; ++i) r = hipEventCreate(&b->ev[i]);
-  for (int i = 0; i <= 2 * kBatchMaxPasses && r == hipSuccess; ++i) r = hipEventCreate(&b->rev[i]);
+  for (int i = 0; i <= 3 * kRadixPasses && r == hipSuccess; ++i) r = hipEventCreate(&b->rev[i]);
   for (int i = 0; i < kBatchBranches && r == hipSuccess; ++i) {
     r = hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking);
     if (r == hipSuccess) r = hipEventCreateWithFlags(&b->join_ev[i], hipEventDisableTiming);
@@ -785,17 +791,25 @@ extern "C" int bulkx_batch_destroy(bulkx_batch* b) {
     if (e) (void)hipEventDestroy(e);
   for (void* p : {static_cast<void*>(b->d_desc), static_cast<void*>(b->d_jobs),
                   static_cast<void*>(b->d_tile_job), static_cast<void*>(b->d_sort_tile_job),
-                  static_cast<void*>(b->d_splits), static_cast<void*>(b->d_job_ctr),
+                  static_cast<void*>(b->d_radix_jobs), static_cast<void*>(b->d_job_ctr),
                   static_cast<void*>(b->d_status), static_cast<void*>(b->d_tmp),
                   static_cast<void*>(b->d_ws), static_cast<void*>(b->d_rbk_jobs),
                   static_cast<void*>(b->d_merge_splits)})
     if (p) (void)hipFree(p);
   for (void* p : {static_cast<void*>(b->h_desc), static_cast<void*>(b->h_jobs),
                   static_cast<void*>(b->h_tile_job), static_cast<void*>(b->h_sort_tile_job),
-                  static_cast<void*>(b->h_rbk_jobs), static_cast<void*>(b->h_err)})
+                  static_cast<void*>(b->h_rbk_jobs), static_cast<void*>(b->h_radix_jobs),
+                  static_cast<void*>(b->h_err)})
     if (p) (void)hipHostFree(p);
   delete b;
   return BULKX_SUCCESS;
+}
+
+// Scratch of one radix sort job of n elements: ping-pong keys and values,
+// and the digit counts of its tiles.
+static size_t radix_tmp_bytes(int64_t n) {
+  const int64_t rt = (n + kBatchRadixTile - 1) / kBatchRadixTile;
+  return 2 * align_up(size_t(n) * 4, 256) + align_up(size_t(rt) * kRadixDigits * sizeof(uint32_t), 256);
 }
 
 // Grows the sort ping-pong space and the per-call workspace to at least the
@@ -821,9 +835,11 @@ static int batch_grow(bulkx_batch* b, size_t tmp_bytes, size_t ws_need) {
 
 extern "C" int bulkx_batch_reserve(bulkx_batch* b, int64_t elements) {
   if (!b || elements < 0 || elements > b->max_elements) return BULKX_ERR_INVALID_ARGUMENT;
-  // sort jobs of `elements` elements in all, each with two 256-byte aligned
-  // arrays of keys and of values
-  const size_t tmp = 2 * align_up(size_t(elements) * 4, 256) + size_t(b->max_jobs) * 1024;
+  // sort jobs of `elements` elements in all: ping-pong keys and values and
+  // the digit counts, each array 256-byte aligned
+  const size_t tmp = 2 * align_up(size_t(elements) * 4, 256) +
+                     size_t(elements / kBatchRadixTile) * kRadixDigits * sizeof(uint32_t) +
+                     size_t(b->max_jobs) * (kRadixDigits * sizeof(uint32_t) + 3 * 256);
   size_t ws = 256;
   const int64_t n = elements;
   for (int dt : {int(BULKX_INT32), int(BULKX_FLOAT32)}) {
@@ -865,13 +881,17 @@ extern "C" int bulkx_batch_set_jobs(bulkx_batch* b, const bulkx_job* jobs, int n
       default:
         t = (n + kBatchMergeTile - 1) / kBatchMergeTile;
         if (op >= OP_SORT_I32) {
-          sort_tiles += t;  // merge-round tiles
-          t = (n + kBatchSortRun - 1) / kBatchSortRun;  // block-sort runs
-          if (t > 1) tmp_bytes += align_up(size_t(n) * 4, 256) * 2;
+          t = 1;  // one block-sorted tile, or the radix passes (no phase-0 tile)
+          if (n > kBatchSortRun) {
+            const int64_t rt = (n + kBatchRadixTile - 1) / kBatchRadixTile;
+            sort_tiles += rt;
+            tmp_bytes += radix_tmp_bytes(n);
+            t = 0;
+          }
         }
         break;
     }
-    if (t < 1) t = 1;  // empty reduce / reduce_by_key: one tile writes the result
+    if (t < 1 && op < OP_SORT_I32) t = 1;  // empty reduce / reduce_by_key: one tile writes the result
     tiles0 += t;
   }
   if (elements > b->max_elements || tiles0 > b->max_tiles || sort_tiles > b->max_tiles ||
@@ -888,7 +908,7 @@ extern "C" int bulkx_batch_set_jobs(bulkx_batch* b, const bulkx_job* jobs, int n
     int64_t t;
     if (op <= OP_REDUCE_F32) t = (n + kBatchScanTile - 1) / kBatchScanTile;
     else if (op <= OP_RBK_I32_I64) t = (n + kBatchRbkTile - 1) / kBatchRbkTile;
-    else if (op >= OP_SORT_I32) t = (n + kBatchSortRun - 1) / kBatchSortRun;  // block-sort runs
+    else if (op >= OP_SORT_I32) return n <= kBatchSortRun ? int64_t(1) : int64_t(0);
     else t = (n + kBatchMergeTile - 1) / kBatchMergeTile;
     return t < 1 ? int64_t(1) : t;  // an empty reduce / reduce_by_key still writes its result
   };
@@ -948,25 +968,20 @@ extern "C" int bulkx_batch_set_jobs(bulkx_batch* b, const bulkx_job* jobs, int n
       dj.splits = b->d_merge_splits + ms;
       ms += t + 1;
     }
-    if (op >= OP_SORT_I32) {
-      dj.passes = passes_for(t);
-      if (t > 1) {
-        dj.tmp_k = b->d_tmp + tb;
-        tb += align_up(size_t(n) * 4, 256);
-        dj.tmp_v = b->d_tmp + tb;
-        tb += align_up(size_t(n) * 4, 256);
-      }
+    if (op >= OP_SORT_I32 && n > kBatchSortRun) {
+      dj.passes = kRadixPasses;
+      dj.sort_tiles = (n + kBatchRadixTile - 1) / kBatchRadixTile;
+      dj.tmp_k = b->d_tmp + tb;
+      tb += align_up(size_t(n) * 4, 256);
+      dj.tmp_v = b->d_tmp + tb;
+      tb += align_up(size_t(n) * 4, 256);
+      dj.counts = reinterpret_cast<uint32_t*>(b->d_tmp + tb);
+      tb += align_up(size_t(dj.sort_tiles) * kRadixDigits * sizeof(uint32_t), 256);
       sort_jobs.push_back(i);
     }
     for (int64_t k = 0; k < t; ++k) b->h_tile_job[cursor[f] + k] = uint32_t(i);
     cursor[f] += t;
   }
-  // Sort tiles ordered by merge rounds, most first: the tiles still merging
-  // in round r are then a prefix of the table, and each round's launch
-  // covers exactly them.
-  std::stable_sort(sort_jobs.begin(), sort_jobs.end(),
-                   [&](int x, int y) { return b->h_jobs[x].passes > b->h_jobs[y].passes; });
-  for (int r = 0; r < kBatchMaxPasses; ++r) b->active_sort_tiles[r] = 0;
   // multi-tile reduce_by_key jobs by family, for the aggregate scans
   {
     int64_t c = 0;
@@ -985,16 +1000,20 @@ extern "C" int bulkx_batch_set_jobs(bulkx_batch* b, const bulkx_job* jobs, int n
       BULKX_HIP(hipMemcpyAsync(b->d_rbk_jobs, b->h_rbk_jobs, sizeof(uint32_t) * c,
                                hipMemcpyHostToDevice, b->stream));
   }
-  b->rounds = 0;
+  // the radix tiles of the sort jobs that run the radix passes, job by job
+  int64_t nr = 0;
   for (int i : sort_jobs) {
     BatchJobDev& dj = b->h_jobs[i];
-    b->rounds = dj.passes > b->rounds ? dj.passes : b->rounds;
     dj.sort_tile_begin = ts;
-    const int64_t pass_tiles = (dj.n0 + kBatchMergeTile - 1) / kBatchMergeTile;
-    for (int64_t k = 0; k < pass_tiles; ++k) b->h_sort_tile_job[ts + k] = uint32_t(i);
-    ts += pass_tiles;
-    for (int r = 0; r < dj.passes && r < kBatchMaxPasses; ++r) b->active_sort_tiles[r] = ts;
+    for (int64_t k = 0; k < dj.sort_tiles; ++k) b->h_sort_tile_job[ts + k] = uint32_t(i);
+    ts += dj.sort_tiles;
+    b->h_radix_jobs[nr++] = uint32_t(i);
   }
+  b->num_radix_jobs = nr;
+  b->h_desc->num_radix_jobs = nr;
+  if (nr > 0)
+    BULKX_HIP(hipMemcpyAsync(b->d_radix_jobs, b->h_radix_jobs, sizeof(uint32_t) * nr,
+                             hipMemcpyHostToDevice, b->stream));
   const int64_t t0 = b->h_desc->fam_begin[FAM_COUNT];
   for (int f = 0; f < FAM_COUNT; ++f) b->fam_tiles[f] = fam_tiles[f];
   b->h_desc->total_sort_tiles = ts;
@@ -1024,10 +1043,10 @@ static void batch_drop_one(BatchGraph& G) {
 }
 
 static void batch_drop_graph(bulkx_batch* b) {
-  for (BatchGraph& G : b->graphs) batch_drop_one(G);
+  batch_drop_one(b->graph);
 }
 
-// Records clear + phase-0 + the job list's merge rounds once on the private
+// Records clear + phase-0 + the radix passes once on the private
 // capture stream and finds the kernel nodes whose grids follow the job list.
 static int batch_capture(bulkx_batch* b, BatchGraph& G) {
   BULKX_HIP(hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal));
@@ -1061,14 +1080,9 @@ static int batch_capture(bulkx_batch* b, BatchGraph& G) {
       if (kp.func == reinterpret_cast<void*>(kRbkUp[r])) kind = FAM_RBK_I32 + r;
       if (kp.func == reinterpret_cast<void*>(kRbkScan[r])) kind = grid_kind_rbk_scan(FAM_RBK_I32 + r);
     }
-    // the merge-round kernels' last argument is the round
-    if (kp.func == reinterpret_cast<void*>(batch_partition_kernel) ||
-        kp.func == reinterpret_cast<void*>(batch_pass_kernel)) {
-      const int round = *static_cast<const int*>(kp.kernelParams[4]);
-      if (round < 0 || round >= kBatchMaxPasses) return BULKX_ERR_INTERNAL;
-      kind = kp.func == reinterpret_cast<void*>(batch_pass_kernel) ? grid_kind_pass(round)
-                                                                   : grid_kind_partition(round);
-    }
+    if (kp.func == reinterpret_cast<void*>(batch_radix_upsweep_kernel)) kind = kGridRadixUp;
+    if (kp.func == reinterpret_cast<void*>(batch_radix_scan_kernel)) kind = kGridRadixScan;
+    if (kp.func == reinterpret_cast<void*>(batch_radix_downsweep_kernel)) kind = kGridRadixDown;
     if (kind >= 0) G.sized_nodes.emplace_back(node, kind);
   }
   BULKX_HIP(hipGraphInstantiate(&G.exec, g, nullptr, nullptr, 0));
@@ -1091,7 +1105,7 @@ extern "C" int bulkx_batch_launch(bulkx_batch* b, int use_graph) {
     BULKX_TRY(batch_enqueue_kernels(b, b->stream, true));
     return batch_copy_err(b);
   }
-  BatchGraph& G = b->graphs[b->rounds];
+  BatchGraph& G = b->graph;
   if (G.exec) {  // resize the job-list-sized grids in place
     bool stale[kGridKinds];
     bool any = false;
@@ -1178,21 +1192,14 @@ extern "C" int bulkx_batch_last_timing(bulkx_batch* b, float* phase0_ms, float* 
 
 extern "C" int bulkx_batch_prepare_graphs(bulkx_batch* b) {
   if (!b) return BULKX_ERR_INVALID_ARGUMENT;
-  const int keep = b->rounds;
-  int st = BULKX_SUCCESS;
-  for (int r = 0; r <= b->max_passes && st == BULKX_SUCCESS; ++r) {
-    if (b->graphs[r].exec) continue;
-    b->rounds = r;
-    st = batch_capture(b, b->graphs[r]);
-  }
-  b->rounds = keep;
-  return st;
+  if (b->graph.exec) return BULKX_SUCCESS;
+  return batch_capture(b, b->graph);
 }
 
-extern "C" int bulkx_batch_sort_geometry(int64_t* run_elements, int64_t* merge_tile) {
-  if (!run_elements || !merge_tile) return BULKX_ERR_INVALID_ARGUMENT;
+extern "C" int bulkx_batch_sort_geometry(int64_t* run_elements, int64_t* radix_tile) {
+  if (!run_elements || !radix_tile) return BULKX_ERR_INVALID_ARGUMENT;
   *run_elements = kBatchSortRun;
-  *merge_tile = kBatchMergeTile;
+  *radix_tile = kBatchRadixTile;
   return BULKX_SUCCESS;
 }
 
@@ -1200,21 +1207,24 @@ extern "C" int bulkx_batch_stage_timing(bulkx_batch* b, float* stage_ms, int num
   if (!b || !stage_ms || num_stages < 0) return BULKX_ERR_INVALID_ARGUMENT;
   if (!b->timed) return BULKX_ERR_NOT_READY;
   BULKX_HIP(hipEventSynchronize(b->ev[kBatchStages]));
-  // stages [0, kBatchStages): families and the merge rounds; then the
-  // pass kernels and the partition kernels of the rounds on their own
-  float pass_ms = 0.f, part_ms = 0.f;
-  for (int p = 0; p < b->timed_passes && num_stages > kBatchStages; ++p) {
-    float a = 0.f, c = 0.f;
-    BULKX_HIP(hipEventElapsedTime(&a, b->rev[2 * p], b->rev[2 * p + 1]));
-    BULKX_HIP(hipEventElapsedTime(&c, b->rev[2 * p + 1], b->rev[2 * p + 2]));
-    part_ms += a;
-    pass_ms += c;
+  // stages [0, kBatchStages): the families and the radix passes; then the
+  // passes' downsweep, upsweep and counts-scan kernels on their own
+  float up_ms = 0.f, scan_ms = 0.f, down_ms = 0.f;
+  for (int p = 0; p < kRadixPasses && num_stages > kBatchStages; ++p) {
+    float a = 0.f, c = 0.f, e = 0.f;
+    BULKX_HIP(hipEventElapsedTime(&a, b->rev[3 * p], b->rev[3 * p + 1]));
+    BULKX_HIP(hipEventElapsedTime(&c, b->rev[3 * p + 1], b->rev[3 * p + 2]));
+    BULKX_HIP(hipEventElapsedTime(&e, b->rev[3 * p + 2], b->rev[3 * p + 3]));
+    up_ms += a;
+    scan_ms += c;
+    down_ms += e;
   }
   for (int k = 0; k < num_stages; ++k) {
     float ms = 0.f;
     if (k < kBatchStages) BULKX_HIP(hipEventElapsedTime(&ms, b->ev[k], b->ev[k + 1]));
-    else if (k == kBatchStages) ms = pass_ms;
-    else if (k == kBatchStages + 1) ms = part_ms;
+    else if (k == kBatchStages) ms = down_ms;
+    else if (k == kBatchStages + 1) ms = up_ms;
+    else if (k == kBatchStages + 2) ms = scan_ms;
     stage_ms[k] = ms;
   }
   return BULKX_SUCCESS;

@@ -124,4 +124,9 @@ size_t merge_workspace_bytes(int kdtype, int vdtype, int64_t n1, int64_t n2);
 size_t sort_workspace_bytes(int kdtype, int vdtype, int64_t n);
 size_t reduce_by_key_workspace_bytes(int kdtype, int vdtype, int64_t n);
 
+// Exclusive plus-scan of uint32 values (scan.hip), workspace
+// scan_workspace_bytes(BULKX_UINT32, n).
+int exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, char* ws,
+                       hipStream_t stream);
+
 }  // namespace bulkx

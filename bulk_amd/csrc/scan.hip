@@ -153,6 +153,16 @@ static int scan_entry(int dtype, int op, const void* in, void* out, int64_t n, c
   return st != BULKX_SUCCESS ? st : rs;
 }
 
+// Exclusive plus-scan of uint32 values (in == out allowed) on `stream`, with
+// a workspace of scan_workspace_bytes(BULKX_UINT32, n) bytes: the radix
+// sort's digit counts become global positions with it.
+int exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, char* ws,
+                       hipStream_t stream) {
+  if (n <= 0) return BULKX_SUCCESS;
+  const uint32_t zero = 0;
+  return scan_launch<uint32_t, OpPlus, true, 0>(in, out, n, &zero, ws, stream);
+}
+
 // Scratch of the allocation-free scan: the look-back header and status
 // words live in this array, one copy per device (each device that loads the
 // library's code object holds its own until the process ends).  4 MiB of

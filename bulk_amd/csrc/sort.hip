@@ -1,4 +1,6 @@
 // stable_sort_by_key entry point.
+#include <algorithm>
+
 #include "sort_impl.hpp"
 #include "tuning.hpp"
 
@@ -10,7 +12,11 @@ size_t sort_workspace_bytes(int kdtype, int vdtype, int64_t n) {
   const int64_t tile = ks == 4 ? SortConfig<float>::TILE : SortConfig<double>::TILE;
   if (n <= tile) return 0;
   size_t vs = vdtype >= 0 ? dtype_size(vdtype) : 0;
-  return SortWorkspace(ks, vs, n, (n + tile - 1) / tile).total;
+  size_t bytes = SortWorkspace(ks, vs, n, (n + tile - 1) / tile).total;
+  // any variant may run: the radix form's digit counts need more than the merge splits
+  if (ks == 4 && n > kRadixTile && n <= kRadixMaxN)
+    bytes = std::max(bytes, RadixWorkspace(ks, vs, n).total);
+  return bytes;
 }
 
 }  // namespace bulkx

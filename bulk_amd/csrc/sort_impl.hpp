@@ -21,6 +21,7 @@
 #pragma once
 
 #include "merge_impl.hpp"
+#include "radix_impl.hpp"
 
 namespace bulkx {
 
@@ -42,7 +43,10 @@ struct SortShape {
   static constexpr int RUN0 = BBLOCK * BITEMS;
   static_assert(RUN0 % Pass::TILE == 0, "runs are whole pass tiles");
 };
-constexpr int kSortVariants = 4;
+// Variant 4: the radix form (radix_impl.hpp) for 4-byte keys; 8-byte keys
+// run variant 0.
+constexpr int kSortVariants = 5;
+constexpr int kSortRadixVariant = 4;
 
 // A key and its index in the tile: 8 bytes for 4-byte keys (one ds_read_b64
 // per merge step), 16 for 8-byte keys.
@@ -374,9 +378,34 @@ int sort_dispatch_variant(int vdtype, int comp, void* keys, void* vals, int64_t 
   return st;
 }
 
+// The radix form over 4-byte keys; one tile is sorted by the block sort
+// alone (variant 2 writes 8192-element runs), and n past 32-bit positions
+// takes the merge form.
+template <class K>
+int sort_dispatch_radix(int vdtype, int comp, void* keys, void* vals, int64_t n, char* ws,
+                        hipStream_t stream) {
+  if (n <= kRadixTile) return sort_dispatch_variant<K, 2>(vdtype, comp, keys, vals, n, ws, stream);
+  if (n > kRadixMaxN) return sort_dispatch_variant<K, 3>(vdtype, comp, keys, vals, n, ws, stream);
+  int st = BULKX_SUCCESS;
+  BULKX_DISPATCH_CMP(comp, CMP, {
+    K* k = static_cast<K*>(keys);
+    if (vdtype < 0)
+      st = radix_sort_launch<CMP, K, uint32_t, false>(k, nullptr, n, ws, stream);
+    else if (dtype_size(vdtype) == 4)
+      st = radix_sort_launch<CMP, K, uint32_t, true>(k, static_cast<uint32_t*>(vals), n, ws, stream);
+    else
+      st = radix_sort_launch<CMP, K, uint64_t, true>(k, static_cast<uint64_t*>(vals), n, ws, stream);
+  });
+  return st;
+}
+
 template <class K>
 int sort_dispatch_key(int vdtype, int comp, void* keys, void* vals, int64_t n, char* ws,
                       hipStream_t stream, int variant) {
+  if constexpr (sizeof(K) == 4) {
+    if (variant == kSortRadixVariant)
+      return sort_dispatch_radix<K>(vdtype, comp, keys, vals, n, ws, stream);
+  }
   switch (variant) {
     case 1: return sort_dispatch_variant<K, 1>(vdtype, comp, keys, vals, n, ws, stream);
     case 2: return sort_dispatch_variant<K, 2>(vdtype, comp, keys, vals, n, ws, stream);

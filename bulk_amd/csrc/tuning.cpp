@@ -217,6 +217,9 @@ int variant_shape(int algo, int dtype, int64_t n, int v, LaunchShape* out) {
       } else if (v == 1) {
         s.group_size = es == 4 ? MergeConfig<float, 1>::BLOCK : MergeConfig<double, 1>::BLOCK;
         s.grain_size = es == 4 ? MergeConfig<float, 1>::ITEMS : MergeConfig<double, 1>::ITEMS;
+      } else if (v == kSortRadixVariant && algo == BULKX_ALGO_SORT_BY_KEY && es == 4) {
+        s.group_size = RadixConfig<uint32_t>::BLOCK;  // the radix passes' tile
+        s.grain_size = RadixConfig<uint32_t>::ITEMS;
       } else {  // sort variants 2 and 3: the block sort's shape (4-byte keys)
         const bool k4 = es == 4;
         s.group_size = k4 ? (v == 2 ? SortShape<float, float, 2>::BBLOCK

@@ -260,9 +260,9 @@ int bulkx_head_flags(int key_dtype, int pred, const void* keys, int64_t n, uint8
                      void* stream);
 
 /* ------------------------------------------------------------------------
- * Batching: many uneven jobs run as segmented work in one persistent launch
- * (plus one launch per sort merge round), recorded once into a hipGraph and
- * replayed.  No reference counterpart: the reference issues one launch chain
+ * Batching: many uneven jobs run as segmented work -- one launch per op
+ * family over the tiles of all its jobs, plus the sort radix passes --
+ * recorded once into a hipGraph and replayed.  No reference counterpart: the reference issues one launch chain
  * per call.
  * ------------------------------------------------------------------------ */
 enum bulkx_job_kind {
@@ -318,29 +318,28 @@ int bulkx_batch_status(bulkx_batch* batch);
  * need (sort ping-pong space, the per-call path's workspace), so that later
  * bulkx_batch_set_jobs calls within that size allocate and free nothing. */
 int bulkx_batch_reserve(bulkx_batch* batch, int64_t elements);
-/* Record ahead of time the graph of every number of merge rounds the batch
- * can need (one graph per count is recorded on first use otherwise), so no
- * later graph launch pays a capture.  No counterpart in the reference; for
+/* Record the batch's graph ahead of time (it is recorded on first use
+ * otherwise), so no later graph launch pays a capture.  No counterpart in the reference; for
  * latency-sensitive callers (the server calls it for each of its slots). */
 int bulkx_batch_prepare_graphs(bulkx_batch* batch);
 /* Device time in ms of the phases of the last eager launch (use_graph = 0):
- * the persistent first-phase kernels (one per op family) and all sort merge
- * rounds together.  Waits for that launch to finish.  BULKX_ERR_NOT_READY if
+ * the first-phase kernels (one per op family) and the sort radix passes
+ * together.  Waits for that launch to finish.  BULKX_ERR_NOT_READY if
  * there is none. */
 int bulkx_batch_last_timing(bulkx_batch* batch, float* phase0_ms, float* passes_ms);
 /* Per-stage device time in ms of the last eager launch, in launch order:
  * [0] int32 scan/reduce, [1] float32 scan/reduce, [2..4] reduce_by_key with
- * int32 / float32 / int64 values, [5] merge_by_key, [6] sort block phase,
- * [7] sort merge rounds; then the rounds split by kernel: [8] the merge
- * pass kernels, [9] the partition kernels.  Writes num_stages entries
- * (zeros beyond 10). */
+ * int32 / float32 / int64 values, [5] merge_by_key, [6] sort jobs of one
+ * block-sorted tile, [7] the radix passes of the larger sort jobs; then the
+ * radix passes split by kernel: [8] downsweeps, [9] upsweeps, [10] counts
+ * scans.  Writes num_stages entries (zeros beyond 11). */
 int bulkx_batch_stage_timing(bulkx_batch* batch, float* stage_ms, int num_stages);
-/* Tile geometry of the batch's sort jobs: the block sort writes sorted runs
- * of *run_elements; each merge round then doubles them in tiles of
- * *merge_tile outputs, so a sort job of n elements takes
- * ceil(log2(ceil(n / run_elements))) rounds.  No counterpart in the
- * reference (its merge_sort_by_key.cu:163-171 block size is fixed). */
-int bulkx_batch_sort_geometry(int64_t* run_elements, int64_t* merge_tile);
+/* Sort geometry of the batch: a sort job of at most *run_elements keys is
+ * sorted by one block-sort tile in LDS; a larger one by four radix passes
+ * over tiles of *radix_tile keys (each pass reads and writes the job once).
+ * No counterpart in the reference (its merge_sort_by_key.cu:163-171 tile is
+ * fixed and its merge passes grow with log2 n). */
+int bulkx_batch_sort_geometry(int64_t* run_elements, int64_t* radix_tile);
 
 /* ------------------------------------------------------------------------
  * Serving: a request loop that admits jobs under an HBM budget, keeps their

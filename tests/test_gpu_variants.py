@@ -69,10 +69,10 @@ def test_reduce_variants(force, orc, v):
         assert abs(api.my_reduce(_dev(f), 0.0) - (hi + lo)) <= orc.float_tolerance(n, 2.0 ** -24, ab), n
 
 
-@pytest.mark.parametrize("v", [0, 1, 2, 3])
+@pytest.mark.parametrize("v", [0, 1, 2, 3, 4])
 def test_merge_and_sort_variants(force, orc, v):
     from bulk_amd import api
-    assert _variants(C.ALGO_SORT_BY_KEY) == 4 and _variants(C.ALGO_MERGE) == 2
+    assert _variants(C.ALGO_SORT_BY_KEY) == 5 and _variants(C.ALGO_MERGE) == 2
     for a in (C.ALGO_MERGE, C.ALGO_MERGE_BY_KEY):
         force(a, C.FLOAT32, min(v, 1))
         force(a, C.INT32, min(v, 1))
@@ -155,3 +155,50 @@ def test_reduce_by_key_last_run_key_zero(force, orc, v):
             rk, rv = orc.reduce_by_key(keys, vals, textbook=True)
             assert m == rk.size and np.array_equal(ko[:m].cpu().numpy(), rk) and \
                 np.array_equal(vo[:m].cpu().numpy(), rv), (n, v)
+
+
+RADIX_SIZES = [2, 8191, 8192, 8193, 16383, 16384, 16385, 3 * 8192 + 17, 100003, (1 << 20) + 5,
+               (1 << 22) + 3]
+
+
+@pytest.mark.parametrize("kt", ["int32", "uint32", "float32"])
+@pytest.mark.parametrize("payload", ["none", "u32", "u64"])
+def test_radix_sort_variant(force, orc, kt, payload):
+    """Sort variant 4 (the radix passes) equals a stable argsort bit for bit:
+    keys of every 4-byte type, keys only or with 4- or 8-byte payloads,
+    less and greater, many ties, and -0.0 / +0.0 as one key."""
+    from bulk_amd import api
+    code = {"int32": C.INT32, "uint32": C.UINT32, "float32": C.FLOAT32}[kt]
+    force(C.ALGO_SORT_BY_KEY, code, 4)
+    tdt = {"int32": torch.int32, "uint32": torch.uint32, "float32": torch.float32}[kt]
+    for n in RADIX_SIZES:
+        if kt == "uint32":
+            k = orc.gen_uniform_u32(n, n)
+        elif kt == "int32":
+            k = orc.gen_uniform_i32(n, -max(2, n // 8), max(2, n // 8), n)  # ties
+        else:
+            pool = np.array([-1.5, -0.0, 0.0, 2.0, 1e-30, -np.inf, np.inf], np.float32)
+            k = np.where(orc.gen_uniform_u32(n + 1, n) % 3 == 0,
+                         pool[orc.gen_uniform_u32(n + 2, n) % pool.size],
+                         orc.gen_normal(n, n, np.float32))
+            k = k.astype(np.float32)
+        for comp in ("less", "greater"):
+            dk = _dev(k.view(np.int32)).view(tdt)
+            idx = np.arange(n, dtype=np.uint64)
+            dv = None
+            if payload == "u32":
+                dv = _dev(idx.astype(np.int32))
+            elif payload == "u64":
+                dv = _dev((idx * 3 + (1 << 40)).astype(np.int64))
+            api.stable_merge_sort_by_key(dk, dv, comp=comp)
+            if kt == "float32":
+                key = k.astype(np.float64)  # -0.0 == 0.0 under both orders
+            else:
+                key = k.astype(np.int64)
+            order = np.argsort(key if comp == "less" else -key, kind="stable")
+            got = dk.view(torch.int32).cpu().numpy().view(k.dtype)
+            assert np.array_equal(got.view(np.uint32), k[order].view(np.uint32)), (n, comp)
+            if payload == "u32":
+                assert np.array_equal(dv.cpu().numpy(), order.astype(np.int32)), (n, comp)
+            elif payload == "u64":
+                assert np.array_equal(dv.cpu().numpy(), (order.astype(np.int64) * 3 + (1 << 40))), (n, comp)
