@@ -76,6 +76,7 @@ SIGNATURES = {
     "bulkx_load_tuning": (_I, [ctypes.c_char_p, ctypes.POINTER(_I)]),
     "bulkx_tuning_variants": (_I, [_I, ctypes.POINTER(_I)]),
     "bulkx_set_tuning": (_I, [_I, _I, _I, _I]),
+    "bulkx_tuned_variant": (_I, [_I, _I, _L, ctypes.POINTER(_I)]),
     "bulkx_workspace_size": (_I, [_I, _I, _I, _L, _L, ctypes.POINTER(_S)]),
     "bulkx_inclusive_scan": (_I, [_I, _I, _P, _P, _L, _P, _P, _S, _P]),
     "bulkx_exclusive_scan": (_I, [_I, _I, _P, _P, _L, _P, _P, _S, _P]),

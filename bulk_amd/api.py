@@ -265,6 +265,13 @@ def device_info() -> dict:
             "max_threads_per_group": mt.value, "hbm_bytes": hbm.value}
 
 
+def tuned_variant(algo: int, dtype: int, n: int) -> int:
+    """The tile-shape variant the tuning table selects for (algo, dtype, n)."""
+    v = ctypes.c_int()
+    C.check(C.lib().bulkx_tuned_variant(algo, dtype, n, ctypes.byref(v)), "bulkx_tuned_variant")
+    return v.value
+
+
 def choose_sizes(algo: int, dtype: int, n: int) -> tuple[int, int, int]:
     """bulk::choose_sizes counterpart: (num_groups, group_size, grain_size)."""
     g, s, r = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()

@@ -285,6 +285,13 @@ extern "C" int bulkx_set_tuning(int algo, int dtype, int size_class_, int varian
   return BULKX_SUCCESS;
 }
 
+extern "C" int bulkx_tuned_variant(int algo, int dtype, int64_t n, int* variant) {
+  if (algo < 0 || algo >= kAlgos || dtype < 0 || dtype >= kDtypes || n < 0 || !variant)
+    return BULKX_ERR_INVALID_ARGUMENT;
+  *variant = tuned_variant(algo, dtype, n);
+  return BULKX_SUCCESS;
+}
+
 extern "C" int bulkx_tuning_variants(int algo, int* count) {
   if (algo < 0 || algo >= kAlgos || !count) return BULKX_ERR_INVALID_ARGUMENT;
   *count = num_variants(algo);

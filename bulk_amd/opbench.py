@@ -70,6 +70,60 @@ def _two_sum_tree(x: torch.Tensor) -> float:
     return float(hi[0]) + float(lo[0])
 
 
+def _exact_sum(x: torch.Tensor) -> tuple[float, float]:
+    """Sum of a float64 tensor as (hi, lo): a pairwise tree of TwoSum steps
+    (Knuth) whose rounding errors are summed alongside; the error left is of
+    order n * 2^-106 * sum|x|."""
+    s = x
+    e = torch.zeros_like(x)
+    while s.numel() > 1:
+        if s.numel() % 2:
+            s = torch.cat([s, s.new_zeros(1)])
+            e = torch.cat([e, e.new_zeros(1)])
+        a, b = s[0::2], s[1::2]
+        t = a + b
+        bb = t - a
+        err = (a - (t - bb)) + (b - bb)
+        e = e[0::2] + e[1::2] + err
+        s = t
+    return float(s[0]), float(e[0])
+
+
+def _f64_prefix_error(x64: torch.Tensor, y: torch.Tensor, eps: float, samples: int = 32) -> dict:
+    """Error of a float64 scan at `samples` positions (and the last): the
+    exact prefix is the sum of exactly summed segments between positions."""
+    n = x64.numel()
+    g = torch.Generator(device="cpu").manual_seed(n)
+    pos = sorted(set(torch.randint(0, n, (samples,), generator=g).tolist()) | {n - 1})
+    worst_err, worst_ratio, start, parts, absum = 0.0, 0.0, 0, [], 0.0
+    for i in pos:
+        seg = x64[start:i + 1]
+        hi, lo = _exact_sum(seg)
+        parts += [hi, lo]
+        absum += float(seg.abs().sum())
+        start = i + 1
+        exact = math.fsum(parts)
+        err = abs(float(y[i]) - exact)
+        worst_err = max(worst_err, err)
+        worst_ratio = max(worst_ratio, err / _bound(i + 1, eps, absum))
+    return {"err": worst_err, "err_over_bound": worst_ratio, "err_sampled_positions": len(pos)}
+
+
+def _sort_ok(k: torch.Tensor, v: torch.Tensor, k0: torch.Tensor) -> bool:
+    """Sorted output of a stable sort by key with payload = input index:
+    keys nondecreasing, payload a permutation with key == input key at it,
+    equal keys in input order."""
+    ki = k.view(torch.int32).long() & 0xFFFFFFFF
+    vi = v.view(torch.int32).long()
+    n = ki.numel()
+    seen = torch.zeros(n, dtype=torch.uint8, device=k.device)
+    seen[vi] = 1
+    ties = ki[1:] == ki[:-1]
+    return bool((ki[1:] >= ki[:-1]).all()) and bool(seen.all()) and \
+        bool(torch.equal(k.view(torch.int32), k0.view(torch.int32)[vi])) and \
+        bool((vi[1:][ties] > vi[:-1][ties]).all())
+
+
 def _bound(n, eps, absum):
     return 4 * max(1, math.ceil(math.log2(max(n, 2)))) * eps * absum
 
@@ -138,6 +192,8 @@ def run(iters: int = 10, log=print, large: int = 28) -> list[dict]:
                 b = 4 * max(1, math.ceil(math.log2(n))) * eps * a
                 extra = {"err": float(e.max()), "err_over_bound": float((e / b.clamp_min(1e-300)).max())}
                 del p, a, e, b
+            else:  # float64: exact prefixes at sampled positions (a float64 cumsum is too coarse)
+                extra = _f64_prefix_error(x64, y, eps)
             rows.append(_row(f"inclusive_scan {tag}", ms, 2 * n * x.element_size(), n, **extra))
             del x, y, x64, wss, out
             torch.cuda.empty_cache()
@@ -182,11 +238,16 @@ def run(iters: int = 10, log=print, large: int = 28) -> list[dict]:
         api.stable_merge_sort_by_key(k, v, workspace=wst)
 
     ms = _timeit(srt, iters) - t_copy
-    _, gs, gr = api.choose_sizes(C.ALGO_SORT_BY_KEY, api.dtype_code(k), s)  # block-sorted run length
-    runs = (s + gs * gr - 1) // (gs * gr)
-    passes = max(0, math.ceil(math.log2(runs)))
-    rows.append(_row(f"stable_sort_by_key u32+u32 2^{large - 2}", ms, 2 * s * 8, s, merge_passes=passes,
-                     pass_bytes=(1 + passes) * 2 * s * 8,
+    variant = api.tuned_variant(C.ALGO_SORT_BY_KEY, api.dtype_code(k), s)
+    if variant == 4:  # radix: 4 passes, each an upsweep (keys) and a downsweep (keys + values)
+        form = {"form": "radix", "passes": 4, "pass_bytes": 4 * (4 * s + 2 * s * 8)}
+    else:
+        _, gs, gr = api.choose_sizes(C.ALGO_SORT_BY_KEY, api.dtype_code(k), s)  # block-sorted run
+        runs = (s + gs * gr - 1) // (gs * gr)
+        passes = max(0, math.ceil(math.log2(runs)))
+        form = {"form": "merge", "passes": passes, "pass_bytes": (1 + passes) * 2 * s * 8}
+    rows.append(_row(f"stable_sort_by_key u32+u32 2^{large - 2}", ms, 2 * s * 8, s,
+                     sorted_stable_permutation=_sort_ok(k, v, k0), **form,
                      note="time excludes restoring the unsorted input (timed separately)"))
     del k0, v0, k, v, wst
     torch.cuda.empty_cache()
@@ -215,6 +276,18 @@ def run(iters: int = 10, log=print, large: int = 28) -> list[dict]:
             if vdt == torch.int64:
                 ref = torch.zeros(R, dtype=torch.int64, device=dev).index_add_(0, keys.long(), vals)
                 extra["sums_exact"] = bool(torch.equal(vo[:R], ref))
+            else:  # float64 index_add errs by < 2^-37 sum|x| per run: far below the bound
+                idx = keys.long()
+                v64 = vals.double()
+                ref = torch.zeros(R, dtype=torch.float64, device=dev).index_add_(0, idx, v64)
+                absum = torch.zeros(R, dtype=torch.float64, device=dev).index_add_(0, idx, v64.abs())
+                lens = torch.bincount(idx, minlength=R).double()
+                bnd = 4 * torch.clamp(torch.ceil(torch.log2(torch.clamp(lens, min=2))), min=1) \
+                    * 2.0 ** -24 * absum
+                err = (vo[:R].double() - ref).abs()
+                extra["err"] = float(err.max())
+                extra["err_over_bound"] = float((err / bnd.clamp_min(1e-300)).max())
+                del idx, v64, ref, absum, lens, bnd, err
             rows.append(_row(f"reduce_by_key i32/{'f32' if vdt == torch.float32 else 'i64'} 2^{large} {runs}",
                              ms, n * (4 + es) + R * (4 + es) + 8, n, **extra))
             del vals, ko, vo, wsk

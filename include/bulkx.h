@@ -143,6 +143,9 @@ int bulkx_load_tuning(const char* path, int* entries);
  */
 int bulkx_tuning_variants(int algo, int* count);
 int bulkx_set_tuning(int algo, int dtype, int size_class, int variant);
+/* The variant the table (or the built-in default, 0) selects for (algo,
+ * dtype, n); for stable_sort_by_key, variant 4 is the radix form. */
+int bulkx_tuned_variant(int algo, int dtype, int64_t n, int* variant);
 
 /* Bytes of scratch `algo` needs for n1 (+ n2) elements; *bytes may be 0. */
 int bulkx_workspace_size(int algo, int dtype, int value_dtype, int64_t n1, int64_t n2,

@@ -410,27 +410,37 @@ void orc_head_flags_i32(const int32_t* k, int64_t n, uint8_t* f) {
 // ---- extended-precision references for the float tolerances ----
 // Prefix sums in long double (64-bit significand), returned as hi + lo
 // doubles, together with the prefix sum of |x|.
-void orc_prefix_sum_ld_f32(const float* x, int64_t n, double init, double* hi, double* lo,
-                           double* abs_prefix) {
-  long double s = init, a = fabsl((long double)init);
+// Prefix sums in long double with Neumaier's compensation (Kahan-Babuska:
+// the rounding error of every addition is kept in a second accumulator,
+// whichever operand is larger), so the prefix is exact to a few units of
+// the long double format however long the array is (plain accumulation
+// errs by up to n * 2^-64 * sum|x|, above the float64 bound at 2^28).
+}  // extern "C"
+template <class T>
+static void prefix_sum_neumaier(const T* x, int64_t n, double init, double* hi, double* lo,
+                                double* abs_prefix) {
+  long double s = init, c = 0, a = fabsl((long double)init);
   for (int64_t i = 0; i < n; ++i) {
-    s += (long double)x[i];
-    a += fabsl((long double)x[i]);
-    hi[i] = (double)s;
-    lo[i] = (double)(s - (long double)hi[i]);
+    const long double v = (long double)x[i];
+    const long double t = s + v;
+    if (fabsl(s) >= fabsl(v)) c += (s - t) + v;
+    else c += (v - t) + s;
+    s = t;
+    a += fabsl(v);
+    const long double p = s + c;
+    hi[i] = (double)p;
+    lo[i] = (double)(p - (long double)hi[i]);
     abs_prefix[i] = (double)a;
   }
 }
+extern "C" {
+void orc_prefix_sum_ld_f32(const float* x, int64_t n, double init, double* hi, double* lo,
+                           double* abs_prefix) {
+  prefix_sum_neumaier(x, n, init, hi, lo, abs_prefix);
+}
 void orc_prefix_sum_ld_f64(const double* x, int64_t n, double init, double* hi, double* lo,
                            double* abs_prefix) {
-  long double s = init, a = fabsl((long double)init);
-  for (int64_t i = 0; i < n; ++i) {
-    s += (long double)x[i];
-    a += fabsl((long double)x[i]);
-    hi[i] = (double)s;
-    lo[i] = (double)(s - (long double)hi[i]);
-    abs_prefix[i] = (double)a;
-  }
+  prefix_sum_neumaier(x, n, init, hi, lo, abs_prefix);
 }
 // Sum and sum |x| in long double with Kahan compensation on top.
 void orc_sum_ld_f64(const double* x, int64_t n, double* hi, double* lo, double* abs_sum) {

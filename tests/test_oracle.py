@@ -79,6 +79,32 @@ def test_prefix_sum_exact_f64_brute_force(orc):
         assert err <= Fraction(1, 2 ** 60) * (1 + abs(acc))
 
 
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_prefix_sum_exact_compensated(orc, dt):
+    """The prefix oracle keeps the rounding error of every addition
+    (Neumaier): [2^70, 1, -2^70] has prefixes 2^70, 2^70 + 1, 1, where plain
+    long-double accumulation (64-bit mantissa) loses the 1 and ends at 0."""
+    x = np.array([2.0 ** 70, 1.0, -(2.0 ** 70), 0.5, 2.0 ** -30], dtype=dt)
+    hi, lo, ab = orc.prefix_sum_exact(x)
+    assert hi[2] + lo[2] == 1.0
+    assert hi[1] + lo[1] == 2.0 ** 70  # 2^70 + 1 rounded once to the long double grid
+    assert hi[4] + lo[4] == 1.5 + 2.0 ** -30
+
+
+@pytest.mark.parametrize("seed,n", [(21, 300000), (22, 1 << 20)])
+def test_prefix_sum_exact_matches_fsum_at_samples(orc, seed, n):
+    """At sampled positions the compensated prefix equals math.fsum of the
+    slice (the exact sum rounded once) to within 2^-60 of sum|x|, on data
+    with heavy cancellation (normal values scaled over 30 binades)."""
+    import math
+    x = orc.gen_normal(seed, n) * np.exp2(orc.gen_uniform_i32(seed + 1, -15, 15, n).astype(np.float64))
+    hi, lo, ab = orc.prefix_sum_exact(x)
+    rng = np.random.default_rng(seed)
+    for i in sorted(rng.choice(n, 12, replace=False)) + [n - 1]:
+        exact = math.fsum(x[: i + 1])
+        assert abs((hi[i] + lo[i]) - exact) <= 2.0 ** -60 * ab[i] + 2.0 ** -52 * abs(exact), i
+
+
 # ---------------------------------------------------------------- reduce
 def test_reduce_known_answer_sequence_plus_13(orc):
     # reduce.cu:132-148: sequence(0..123456788) + 13 in int32 (wrapping)
