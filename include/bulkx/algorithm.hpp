@@ -35,6 +35,14 @@
 #include <utility>
 
 #include "../bulkx.h"
+#include "generic.hpp"
+
+// Thrust's functors and device pointers are recognised when Thrust is in
+// use (any Thrust header included before this one).
+#ifdef THRUST_VERSION
+#include <thrust/functional.h>
+#include <thrust/memory.h>
+#endif
 
 namespace bulkx {
 
@@ -162,18 +170,48 @@ auto raw(It it) -> decltype(thrust::raw_pointer_cast(&*it)) {
 template <class It>
 using value_t = std::remove_cv_t<std::remove_pointer_t<decltype(raw(std::declval<It>()))>>;
 
+// Element types the C-ABI carries (everything else takes the generic path).
+template <class T>
+struct has_dtype : std::false_type {};
+template <> struct has_dtype<int32_t> : std::true_type {};
+template <> struct has_dtype<uint32_t> : std::true_type {};
+template <> struct has_dtype<int64_t> : std::true_type {};
+template <> struct has_dtype<uint64_t> : std::true_type {};
+template <> struct has_dtype<float> : std::true_type {};
+template <> struct has_dtype<double> : std::true_type {};
+template <> struct has_dtype<long long> : std::true_type {};
+template <> struct has_dtype<unsigned long long> : std::true_type {};
+
+// Whether a call maps onto the C-ABI: an enum operator over enum types.
+template <class Op, class... Ts>
+constexpr bool abi_op() {
+  return op_code<Op>() >= 0 && (has_dtype<Ts>::value && ...);
+}
+template <class Cmp, class... Ts>
+constexpr bool abi_cmp() {
+  return cmp_code<Cmp>() >= 0 && (has_dtype<Ts>::value && ...);
+}
+
 }  // namespace detail
+
+// Each driver takes the C-ABI path when its operator (plus / minimum /
+// maximum, less / greater, equal_to) and element types map onto the enums,
+// and the header-only generic path of generic.hpp otherwise (any
+// associative operator, any comparator or predicate, any element type).
 
 template <class It1, class It2, class T, class Op>
 It2 inclusive_scan(It1 first, It1 last, It2 result, T init, Op op) {
   using V = detail::value_t<It1>;
-  static_assert(std::is_same<V, detail::value_t<It2>>::value, "input and output types match");
-  static_assert(detail::op_code<Op>() >= 0, "op must be plus, minimum or maximum");
+  using R = detail::value_t<It2>;
   const int64_t n = int64_t(last - first);
-  const V iv = V(init);
-  check(bulkx_inclusive_scan(detail::dtype_of<V>::value, detail::op_code<Op>(), detail::raw(first),
-                             detail::raw(result), n, &iv, nullptr, 0, default_stream()),
-        "inclusive_scan");
+  if constexpr (detail::abi_op<Op, V>() && std::is_same<V, R>::value) {
+    const V iv = V(init);
+    check(bulkx_inclusive_scan(detail::dtype_of<V>::value, detail::op_code<Op>(), detail::raw(first),
+                               detail::raw(result), n, &iv, nullptr, 0, default_stream()),
+          "inclusive_scan");
+  } else {
+    generic::scan<true>(detail::raw(first), n, detail::raw(result), R(init), op, default_stream());
+  }
   return result + n;
 }
 
@@ -183,31 +221,39 @@ It2 inclusive_scan(It1 first, It1 last, It2 result, T init, Op op) {
 template <class It1, class It2, class T, class Op>
 It2 exclusive_scan(It1 first, It1 last, It2 result, T init, Op op) {
   using V = detail::value_t<It1>;
-  static_assert(std::is_same<V, detail::value_t<It2>>::value, "input and output types match");
-  static_assert(detail::op_code<Op>() >= 0, "op must be plus, minimum or maximum");
+  using R = detail::value_t<It2>;
   const int64_t n = int64_t(last - first);
-  const V iv = V(init);
-  check(bulkx_exclusive_scan(detail::dtype_of<V>::value, detail::op_code<Op>(), detail::raw(first),
-                             detail::raw(result), n, &iv, nullptr, 0, default_stream()),
-        "exclusive_scan");
+  if constexpr (detail::abi_op<Op, V>() && std::is_same<V, R>::value) {
+    const V iv = V(init);
+    check(bulkx_exclusive_scan(detail::dtype_of<V>::value, detail::op_code<Op>(), detail::raw(first),
+                               detail::raw(result), n, &iv, nullptr, 0, default_stream()),
+          "exclusive_scan");
+  } else {
+    generic::scan<false>(detail::raw(first), n, detail::raw(result), R(init), op, default_stream());
+  }
   return result + n;
 }
 
 template <class It, class T, class Op>
 T my_reduce(It first, It last, T init, Op op) {
   using V = detail::value_t<It>;
-  static_assert(detail::op_code<Op>() >= 0, "op must be plus, minimum or maximum");
-  const V iv = V(init);
-  V* d = nullptr;
-  check(hipMallocAsync(reinterpret_cast<void**>(&d), sizeof(V), default_stream()) == hipSuccess
+  using A = typename std::conditional<detail::abi_op<Op, V>(), V, T>::type;  // device result type
+  const A iv = A(init);
+  A* d = nullptr;
+  check(hipMallocAsync(reinterpret_cast<void**>(&d), sizeof(A), default_stream()) == hipSuccess
             ? BULKX_SUCCESS
             : BULKX_ERR_OUT_OF_MEMORY,
         "my_reduce");
-  int st = bulkx_reduce(detail::dtype_of<V>::value, detail::op_code<Op>(), detail::raw(first),
-                        int64_t(last - first), &iv, d, nullptr, 0, default_stream());
-  V h = iv;
+  int st = BULKX_SUCCESS;
+  if constexpr (detail::abi_op<Op, V>()) {
+    st = bulkx_reduce(detail::dtype_of<V>::value, detail::op_code<Op>(), detail::raw(first),
+                      int64_t(last - first), &iv, d, nullptr, 0, default_stream());
+  } else {
+    generic::reduce_into(detail::raw(first), int64_t(last - first), iv, op, d, default_stream());
+  }
+  A h = iv;
   if (st == BULKX_SUCCESS &&
-      hipMemcpyAsync(&h, d, sizeof(V), hipMemcpyDeviceToHost, default_stream()) != hipSuccess)
+      hipMemcpyAsync(&h, d, sizeof(A), hipMemcpyDeviceToHost, default_stream()) != hipSuccess)
     st = BULKX_ERR_HIP;
   if (hipStreamSynchronize(default_stream()) != hipSuccess && st == BULKX_SUCCESS) st = BULKX_ERR_HIP;
   (void)hipFreeAsync(d, default_stream());
@@ -218,11 +264,17 @@ T my_reduce(It first, It last, T init, Op op) {
 template <class It1, class It2, class It3, class Compare>
 It3 my_merge(It1 first1, It1 last1, It2 first2, It2 last2, It3 result, Compare comp) {
   using K = detail::value_t<It1>;
-  static_assert(detail::cmp_code<Compare>() >= 0, "comp must be less or greater");
   const int64_t n1 = int64_t(last1 - first1), n2 = int64_t(last2 - first2);
-  check(bulkx_merge(detail::dtype_of<K>::value, detail::cmp_code<Compare>(), detail::raw(first1), n1,
-                    detail::raw(first2), n2, detail::raw(result), nullptr, 0, default_stream()),
-        "my_merge");
+  if constexpr (detail::abi_cmp<Compare, K>() && std::is_same<K, detail::value_t<It2>>::value &&
+                std::is_same<K, detail::value_t<It3>>::value) {
+    check(bulkx_merge(detail::dtype_of<K>::value, detail::cmp_code<Compare>(), detail::raw(first1),
+                      n1, detail::raw(first2), n2, detail::raw(result), nullptr, 0,
+                      default_stream()),
+          "my_merge");
+  } else {
+    generic::merge(detail::raw(first1), n1, detail::raw(first2), n2, detail::raw(result), comp,
+                   default_stream());
+  }
   return result + (n1 + n2);
 }
 
@@ -232,14 +284,22 @@ std::pair<KIt3, VIt3> merge_by_key(KIt1 keys_first1, KIt1 keys_last1, VIt1 value
                                    KIt3 keys_result, VIt3 values_result, Compare comp) {
   using K = detail::value_t<KIt1>;
   using V = detail::value_t<VIt1>;
-  static_assert(detail::cmp_code<Compare>() >= 0, "comp must be less or greater");
   const int64_t n1 = int64_t(keys_last1 - keys_first1), n2 = int64_t(keys_last2 - keys_first2);
-  check(bulkx_merge_by_key(detail::dtype_of<K>::value, detail::dtype_of<V>::value,
-                           detail::cmp_code<Compare>(), detail::raw(keys_first1),
-                           detail::raw(values_first1), n1, detail::raw(keys_first2),
-                           detail::raw(values_first2), n2, detail::raw(keys_result),
-                           detail::raw(values_result), nullptr, 0, default_stream()),
-        "merge_by_key");
+  if constexpr (detail::abi_cmp<Compare, K, V>() &&
+                std::is_same<K, detail::value_t<KIt2>>::value &&
+                std::is_same<V, detail::value_t<VIt2>>::value) {
+    check(bulkx_merge_by_key(detail::dtype_of<K>::value, detail::dtype_of<V>::value,
+                             detail::cmp_code<Compare>(), detail::raw(keys_first1),
+                             detail::raw(values_first1), n1, detail::raw(keys_first2),
+                             detail::raw(values_first2), n2, detail::raw(keys_result),
+                             detail::raw(values_result), nullptr, 0, default_stream()),
+          "merge_by_key");
+  } else {
+    generic::merge_by_key(detail::raw(keys_first1), n1, detail::raw(values_first1),
+                          detail::raw(keys_first2), n2, detail::raw(values_first2),
+                          detail::raw(keys_result), detail::raw(values_result), comp,
+                          default_stream());
+  }
   return {keys_result + (n1 + n2), values_result + (n1 + n2)};
 }
 
@@ -247,12 +307,16 @@ template <class KIt, class VIt, class Compare>
 void stable_merge_sort_by_key(KIt keys_first, KIt keys_last, VIt values_first, Compare comp) {
   using K = detail::value_t<KIt>;
   using V = detail::value_t<VIt>;
-  static_assert(detail::cmp_code<Compare>() >= 0, "comp must be less or greater");
-  check(bulkx_stable_sort_by_key(detail::dtype_of<K>::value, detail::dtype_of<V>::value,
-                                 detail::cmp_code<Compare>(), detail::raw(keys_first),
-                                 detail::raw(values_first), int64_t(keys_last - keys_first),
-                                 nullptr, 0, default_stream()),
-        "stable_merge_sort_by_key");
+  const int64_t n = int64_t(keys_last - keys_first);
+  if constexpr (detail::abi_cmp<Compare, K, V>()) {
+    check(bulkx_stable_sort_by_key(detail::dtype_of<K>::value, detail::dtype_of<V>::value,
+                                   detail::cmp_code<Compare>(), detail::raw(keys_first),
+                                   detail::raw(values_first), n, nullptr, 0, default_stream()),
+          "stable_merge_sort_by_key");
+  } else {
+    generic::stable_sort_by_key(detail::raw(keys_first), detail::raw(values_first), n, comp,
+                                default_stream());
+  }
 }
 
 template <class KIt1, class VIt1, class KIt2, class VIt2, class Pred, class Op>
@@ -260,18 +324,24 @@ std::pair<KIt2, VIt2> my_reduce_by_key(KIt1 keys_first, KIt1 keys_last, VIt1 val
                                        KIt2 keys_result, VIt2 values_result, Pred pred, Op op) {
   using K = detail::value_t<KIt1>;
   using V = detail::value_t<VIt1>;
-  static_assert(detail::pred_code<Pred>() >= 0, "pred must be equal_to");
-  static_assert(detail::op_code<Op>() >= 0, "op must be plus, minimum or maximum");
   int64_t* d = nullptr;
   check(hipMallocAsync(reinterpret_cast<void**>(&d), sizeof(int64_t), default_stream()) == hipSuccess
             ? BULKX_SUCCESS
             : BULKX_ERR_OUT_OF_MEMORY,
         "my_reduce_by_key");
-  int st = bulkx_reduce_by_key(detail::dtype_of<K>::value, detail::dtype_of<V>::value,
-                               detail::pred_code<Pred>(), detail::op_code<Op>(),
-                               detail::raw(keys_first), detail::raw(values_first),
-                               int64_t(keys_last - keys_first), detail::raw(keys_result),
-                               detail::raw(values_result), d, nullptr, 0, default_stream());
+  int st = BULKX_SUCCESS;
+  const int64_t n = int64_t(keys_last - keys_first);
+  if constexpr (detail::pred_code<Pred>() >= 0 && detail::abi_op<Op, K, V>()) {
+    st = bulkx_reduce_by_key(detail::dtype_of<K>::value, detail::dtype_of<V>::value,
+                             detail::pred_code<Pred>(), detail::op_code<Op>(),
+                             detail::raw(keys_first), detail::raw(values_first), n,
+                             detail::raw(keys_result), detail::raw(values_result), d, nullptr, 0,
+                             default_stream());
+  } else {
+    generic::reduce_by_key(detail::raw(keys_first), detail::raw(values_first), n,
+                           detail::raw(keys_result), detail::raw(values_result), pred, op,
+                           reinterpret_cast<long long*>(d), default_stream());
+  }
   int64_t m = 0;
   if (st == BULKX_SUCCESS &&
       hipMemcpyAsync(&m, d, sizeof(m), hipMemcpyDeviceToHost, default_stream()) != hipSuccess)
@@ -388,13 +458,15 @@ template <class It, class T, class Op>
 scalar_future<T> async_reduce(It first, It last, T init, Op op,
                               hipStream_t stream = default_stream()) {
   using V = detail::value_t<It>;
-  static_assert(std::is_same<V, T>::value, "init has the element type");
-  static_assert(detail::op_code<Op>() >= 0, "op must be plus, minimum or maximum");
-  const V iv = V(init);
-  detail::stream_scalar<V> d(stream);
-  check(bulkx_reduce(detail::dtype_of<V>::value, detail::op_code<Op>(), detail::raw(first),
-                     int64_t(last - first), &iv, d.p, nullptr, 0, stream),
-        "async_reduce");
+  detail::stream_scalar<T> d(stream);
+  if constexpr (detail::abi_op<Op, V>() && std::is_same<V, T>::value) {
+    const V iv = V(init);
+    check(bulkx_reduce(detail::dtype_of<V>::value, detail::op_code<Op>(), detail::raw(first),
+                       int64_t(last - first), &iv, d.p, nullptr, 0, stream),
+          "async_reduce");
+  } else {
+    generic::reduce_into(detail::raw(first), int64_t(last - first), init, op, d.p, stream);
+  }
   return scalar_future<T>::from_device(d.p, stream);
 }
 
@@ -429,17 +501,86 @@ reduce_by_key_future<KIt2, VIt2> async_reduce_by_key(KIt1 keys_first, KIt1 keys_
                                                      hipStream_t stream = default_stream()) {
   using K = detail::value_t<KIt1>;
   using V = detail::value_t<VIt1>;
-  static_assert(detail::pred_code<Pred>() >= 0, "pred must be equal_to");
-  static_assert(detail::op_code<Op>() >= 0, "op must be plus, minimum or maximum");
   detail::stream_scalar<int64_t> d(stream);
-  check(bulkx_reduce_by_key(detail::dtype_of<K>::value, detail::dtype_of<V>::value,
-                            detail::pred_code<Pred>(), detail::op_code<Op>(),
-                            detail::raw(keys_first), detail::raw(values_first),
-                            int64_t(keys_last - keys_first), detail::raw(keys_result),
-                            detail::raw(values_result), d.p, nullptr, 0, stream),
-        "async_reduce_by_key");
+  if constexpr (detail::pred_code<Pred>() >= 0 && detail::abi_op<Op, K, V>()) {
+    check(bulkx_reduce_by_key(detail::dtype_of<K>::value, detail::dtype_of<V>::value,
+                              detail::pred_code<Pred>(), detail::op_code<Op>(),
+                              detail::raw(keys_first), detail::raw(values_first),
+                              int64_t(keys_last - keys_first), detail::raw(keys_result),
+                              detail::raw(values_result), d.p, nullptr, 0, stream),
+          "async_reduce_by_key");
+  } else {
+    generic::reduce_by_key(detail::raw(keys_first), detail::raw(values_first),
+                           int64_t(keys_last - keys_first), detail::raw(keys_result),
+                           detail::raw(values_result), pred, op,
+                           reinterpret_cast<long long*>(d.p), stream);
+  }
   return reduce_by_key_future<KIt2, VIt2>(scalar_future<int64_t>::from_device(d.p, stream),
                                           keys_result, values_result);
+}
+
+// Decompositions of [0, n) into partitions (decomposition.hpp:5-218): each
+// has operator[](i) -> [first, second), size() (the number of partitions)
+// and n().
+//
+// trivial_decomposition(n): one partition, [0, n).
+template <class Size>
+class trivial_decomposition {
+ public:
+  typedef Size size_type;
+  typedef std::pair<Size, Size> range;
+  trivial_decomposition() : m_n(0) {}
+  explicit trivial_decomposition(Size n) : m_n(n) {}
+  __host__ __device__ range operator[](Size) const { return range(0, m_n); }
+  __host__ __device__ Size size() const { return 1; }
+  __host__ __device__ Size n() const { return m_n; }
+
+ private:
+  Size m_n;
+};
+template <class Size>
+trivial_decomposition<Size> make_trivial_decomposition(Size n) {
+  return trivial_decomposition<Size>(n);
+}
+
+// blocked_decomposition(n, block): partitions of `block` elements, the last
+// one short.
+template <class Size>
+class blocked_decomposition {
+ public:
+  typedef Size size_type;
+  typedef std::pair<Size, Size> range;
+  blocked_decomposition() : m_n(0), m_block(0), m_p(0) {}
+  blocked_decomposition(Size n, Size block_size)
+      : m_n(n), m_block(block_size), m_p(block_size > 0 ? (n + block_size - 1) / block_size : 0) {}
+  __host__ __device__ range operator[](Size i) const {
+    const Size b = i * m_block;
+    return range(b, b + m_block < m_n ? b + m_block : m_n);
+  }
+  __host__ __device__ Size size() const { return m_p; }
+  __host__ __device__ Size n() const { return m_n; }
+  __host__ __device__ Size block_size() const { return m_block; }
+
+ private:
+  Size m_n, m_block, m_p;
+};
+template <class Size>
+blocked_decomposition<Size> make_blocked_decomposition(Size n, Size block_size) {
+  return blocked_decomposition<Size>(n, block_size);
+}
+
+// uniform_decomposition(n, P): blocks of n / P elements (so ceil(n / (n / P))
+// partitions, the last one short), as the reference defines it.
+template <class Size>
+class uniform_decomposition : public blocked_decomposition<Size> {
+ public:
+  uniform_decomposition() = default;
+  uniform_decomposition(Size n, Size num_partitions)
+      : blocked_decomposition<Size>(n, num_partitions > 0 ? n / num_partitions : 0) {}
+};
+template <class Size>
+uniform_decomposition<Size> make_uniform_decomposition(Size n, Size num_partitions) {
+  return uniform_decomposition<Size>(n, num_partitions);
 }
 
 // aligned_decomposition(n, num_partitions, tile): ceil(n / tile) tiles split
@@ -452,7 +593,7 @@ class aligned_decomposition {
   typedef std::pair<Size, Size> range;
   aligned_decomposition(Size n, Size num_partitions, Size aligned_size)
       : m_n(n), m_p(num_partitions), m_tile(aligned_size) {}
-  range operator[](Size i) const {
+  __host__ __device__ range operator[](Size i) const {
     const Size tiles = (m_n + m_tile - 1) / m_tile;
     const Size per = tiles / m_p, extra = tiles % m_p;
     Size b = per * i + (i < extra ? i : extra);
@@ -461,9 +602,9 @@ class aligned_decomposition {
     e = e * m_tile < m_n ? e * m_tile : m_n;
     return range(b, e);
   }
-  Size size() const { return m_p; }
-  Size n() const { return m_n; }
-  Size tile_size() const { return m_tile; }
+  __host__ __device__ Size size() const { return m_p; }
+  __host__ __device__ Size n() const { return m_n; }
+  __host__ __device__ Size tile_size() const { return m_tile; }
 
  private:
   Size m_n, m_p, m_tile;
@@ -474,26 +615,50 @@ aligned_decomposition<Size> make_aligned_decomposition(Size n, Size num_partitio
   return aligned_decomposition<Size>(n, num_partitions, aligned_size);
 }
 
+namespace detail {
+template <class D>
+struct is_aligned_decomposition : std::false_type {};
+template <class Size>
+struct is_aligned_decomposition<aligned_decomposition<Size>> : std::true_type {};
+}  // namespace detail
+
 // reduce_intervals(first, decomp, result, op): result[g] = the op-reduction
-// of partition g of the decomposition.
-template <class It1, class Size, class It2, class Op>
-It2 reduce_intervals(It1 first, const aligned_decomposition<Size>& decomp, It2 result, Op op) {
+// of partition g of the decomposition (reduce_intervals.hpp:31-40).
+// Aligned, blocked, uniform and trivial decompositions under an enum
+// operator run the library kernel (empty partitions give the identity);
+// any other decomposition or operator runs the generic path (empty
+// partitions leave their slot untouched).
+template <class It1, class Decomposition, class It2, class Op>
+It2 reduce_intervals(It1 first, const Decomposition& decomp, It2 result, Op op) {
   using V = detail::value_t<It1>;
-  static_assert(detail::op_code<Op>() >= 0, "op must be plus, minimum or maximum");
-  check(bulkx_reduce_intervals(detail::dtype_of<V>::value, detail::op_code<Op>(), detail::raw(first),
-                               int64_t(decomp.n()), int64_t(decomp.size()),
-                               int64_t(decomp.tile_size()), detail::raw(result), default_stream()),
-        "reduce_intervals");
+  using Size = typename Decomposition::size_type;
+  constexpr bool aligned = detail::is_aligned_decomposition<Decomposition>::value;
+  constexpr bool blocked = std::is_base_of<blocked_decomposition<Size>, Decomposition>::value;
+  constexpr bool trivial = std::is_same<Decomposition, trivial_decomposition<Size>>::value;
+  if constexpr ((aligned || blocked || trivial) && detail::abi_op<Op, V>() &&
+                std::is_same<V, detail::value_t<It2>>::value) {
+    int64_t tile;
+    if constexpr (aligned) tile = int64_t(decomp.tile_size());
+    else if constexpr (blocked) tile = int64_t(decomp.block_size());
+    else tile = decomp.n() > 0 ? int64_t(decomp.n()) : 1;
+    if (tile > 0) {
+      check(bulkx_reduce_intervals(detail::dtype_of<V>::value, detail::op_code<Op>(),
+                                   detail::raw(first), int64_t(decomp.n()), int64_t(decomp.size()),
+                                   tile, detail::raw(result), default_stream()),
+            "reduce_intervals");
+      return result + decomp.size();
+    }
+  }
+  generic::reduce_intervals(detail::raw(first), decomp, detail::raw(result), op, default_stream());
   return result + decomp.size();
 }
 
-// reduce_intervals(first, last, interval_size, result, op): fixed-size
-// intervals (one tile per partition of an aligned decomposition).
+// reduce_intervals(first, last, interval_size, result, op): the blocked
+// decomposition of [first, last) (reduce_intervals.hpp:43-47).
 template <class It1, class Size, class It2, class Op>
 It2 reduce_intervals(It1 first, It1 last, Size interval_size, It2 result, Op op) {
-  const Size n = Size(last - first);
-  const Size p = (n + interval_size - 1) / interval_size;
-  return reduce_intervals(first, aligned_decomposition<Size>(n, p, interval_size), result, op);
+  return reduce_intervals(first, make_blocked_decomposition<Size>(Size(last - first), interval_size),
+                          result, op);
 }
 
 }  // namespace bulkx

@@ -16,9 +16,11 @@
 //   * concurrent_group<agent, S>       -> one workgroup of S work-items (S <= 1024,
 //                                         16 wavefronts of 64); wait() is a workgroup barrier
 //   * parallel_group<concurrent_group> -> one workgroup per group (grid)
-//   * bulk::malloc(g, bytes)           -> a bump arena in the workgroup's dynamic LDS
-//                                         (heap_size bytes, up to 160 KB per CU on gfx950),
-//                                         falling back to device malloc when exhausted
+//   * bulk::malloc(g, bytes)           -> a first-fit heap in the workgroup's dynamic LDS
+//                                         (heap_size bytes, or by default the LDS share
+//                                         that keeps the kernel's occupancy; up to 160 KB
+//                                         per CU on gfx950), falling back to device
+//                                         malloc when exhausted
 // Group algorithms (copy_n, reduce, accumulate, inclusive_scan,
 // exclusive_scan, merge_path, merge, merge_by_key, inplace_merge,
 // stable_sort_by_key, reduce_by_key, scatter_if, gather,
@@ -27,6 +29,9 @@
 // bounded agent (bulk::bound<N>(self)) are the sequential bodies of one
 // work-item.  head_flags / tail_flags are index-computed flag ranges and
 // join_iterator views two ranges as one.
+// Also: choose_sizes (bulk/choose_sizes.hpp), strided_iterator
+// (bulk/iterator/strided_iterator.hpp) and uninitialized /
+// uninitialized_array (bulk/uninitialized.hpp).
 // Contracts: bulk/algorithm/*.hpp of the reference.
 #pragma once
 
@@ -35,6 +40,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <iterator>
+#include <new>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -48,6 +54,9 @@ namespace bulk {
 static constexpr std::size_t use_default = ~std::size_t(0);
 static constexpr std::size_t dynamic_group_size = 0;
 static constexpr int invalid_index = -1;
+// A group or grid size left to the launcher (use_default at construction):
+// resolved at launch from the kernel's occupancy.
+static constexpr int default_size = -1;
 
 // ------------------------------------------------------------------ errors
 class system_error : public std::runtime_error {
@@ -65,6 +74,149 @@ inline void throw_on_error(hipError_t e, const char* where) {
   if (e != hipSuccess) throw system_error(e, where);
 }
 }  // namespace detail
+
+// ------------------------------------------------------------ raw storage
+// uninitialized<T>: storage for one T whose constructor has not run
+// (bulk/uninitialized.hpp:30-160), so that types with constructors can
+// live in __shared__ memory; construct(args...) / destroy() run them
+// explicitly.
+template <class T>
+class uninitialized {
+ public:
+  __host__ __device__ T& get() { return *ptr(); }
+  __host__ __device__ const T& get() const { return *ptr(); }
+  __host__ __device__ operator T&() { return get(); }
+  __host__ __device__ operator const T&() const { return get(); }
+  __host__ __device__ uninitialized& operator=(const T& other) {
+    get() = other;
+    return *this;
+  }
+  template <class... Args>
+  __host__ __device__ void construct(const Args&... args) {
+    ::new (static_cast<void*>(m_storage)) T(args...);
+  }
+  __host__ __device__ void destroy() { get().~T(); }
+
+ private:
+  __host__ __device__ T* ptr() { return reinterpret_cast<T*>(m_storage); }
+  __host__ __device__ const T* ptr() const { return reinterpret_cast<const T*>(m_storage); }
+  alignas(T) unsigned char m_storage[sizeof(T)];
+};
+
+// uninitialized_array<T, N>: N unconstructed T with array access
+// (bulk/uninitialized.hpp:163-296).
+template <class T, std::size_t N>
+class uninitialized_array {
+ public:
+  typedef T value_type;
+  typedef T& reference;
+  typedef const T& const_reference;
+  typedef T* pointer;
+  typedef const T* const_pointer;
+  typedef pointer iterator;
+  typedef const_pointer const_iterator;
+  typedef std::size_t size_type;
+  __host__ __device__ iterator begin() { return data(); }
+  __host__ __device__ const_iterator begin() const { return data(); }
+  __host__ __device__ iterator end() { return begin() + size(); }
+  __host__ __device__ const_iterator end() const { return begin() + size(); }
+  __host__ __device__ const_iterator cbegin() const { return begin(); }
+  __host__ __device__ const_iterator cend() const { return end(); }
+  __host__ __device__ size_type size() const { return N; }
+  __host__ __device__ bool empty() const { return N == 0; }
+  __host__ __device__ T* data() { return reinterpret_cast<T*>(m_storage); }
+  __host__ __device__ const T* data() const { return reinterpret_cast<const T*>(m_storage); }
+  __host__ __device__ reference operator[](size_type i) { return data()[i]; }
+  __host__ __device__ const_reference operator[](size_type i) const { return data()[i]; }
+  __host__ __device__ reference front() { return data()[0]; }
+  __host__ __device__ const_reference front() const { return data()[0]; }
+  __host__ __device__ reference back() { return data()[N - 1]; }
+  __host__ __device__ const_reference back() const { return data()[N - 1]; }
+
+ private:
+  alignas(T) unsigned char m_storage[sizeof(T) * (N ? N : 1)];
+};
+
+// ------------------------------------------------------------- iterators
+// strided_iterator: an iterator that steps `stride` elements of its base
+// per increment (bulk/iterator/strided_iterator.hpp:28-105); the distance
+// between two strided iterators rounds away from zero, as the reference's
+// distance_to does.
+template <class Iterator,
+          class Size = typename std::iterator_traits<Iterator>::difference_type>
+class strided_iterator {
+ public:
+  typedef typename std::iterator_traits<Iterator>::value_type value_type;
+  typedef typename std::iterator_traits<Iterator>::reference reference;
+  typedef typename std::iterator_traits<Iterator>::pointer pointer;
+  typedef typename std::iterator_traits<Iterator>::difference_type difference_type;
+  typedef std::random_access_iterator_tag iterator_category;
+  typedef Size stride_type;
+
+  __host__ __device__ strided_iterator() : m_base(), m_stride(1) {}
+  __host__ __device__ strided_iterator(const Iterator& base, stride_type stride)
+      : m_base(base), m_stride(stride) {}
+  __host__ __device__ stride_type stride() const { return m_stride; }
+  __host__ __device__ Iterator base() const { return m_base; }
+
+  __host__ __device__ reference operator*() const { return *m_base; }
+  __host__ __device__ reference operator[](difference_type n) const {
+    return m_base[n * difference_type(m_stride)];
+  }
+  __host__ __device__ strided_iterator& operator++() {
+    m_base += m_stride;
+    return *this;
+  }
+  __host__ __device__ strided_iterator operator++(int) {
+    strided_iterator t = *this;
+    ++*this;
+    return t;
+  }
+  __host__ __device__ strided_iterator& operator--() {
+    m_base -= m_stride;
+    return *this;
+  }
+  __host__ __device__ strided_iterator operator--(int) {
+    strided_iterator t = *this;
+    --*this;
+    return t;
+  }
+  __host__ __device__ strided_iterator& operator+=(difference_type n) {
+    m_base += n * difference_type(m_stride);
+    return *this;
+  }
+  __host__ __device__ strided_iterator& operator-=(difference_type n) { return *this += -n; }
+  __host__ __device__ strided_iterator operator+(difference_type n) const {
+    strided_iterator t = *this;
+    return t += n;
+  }
+  __host__ __device__ strided_iterator operator-(difference_type n) const {
+    strided_iterator t = *this;
+    return t -= n;
+  }
+  template <class OtherIterator, class OtherSize>
+  __host__ __device__ difference_type operator-(
+      const strided_iterator<OtherIterator, OtherSize>& other) const {
+    const difference_type d = m_base - other.base();
+    const difference_type s = difference_type(other.stride());
+    return d >= 0 ? (d + (s - 1)) / s : (d - (s - 1)) / s;
+  }
+  __host__ __device__ bool operator==(const strided_iterator& o) const { return m_base == o.m_base; }
+  __host__ __device__ bool operator!=(const strided_iterator& o) const { return m_base != o.m_base; }
+  __host__ __device__ bool operator<(const strided_iterator& o) const { return m_base < o.m_base; }
+  __host__ __device__ bool operator>(const strided_iterator& o) const { return m_base > o.m_base; }
+  __host__ __device__ bool operator<=(const strided_iterator& o) const { return m_base <= o.m_base; }
+  __host__ __device__ bool operator>=(const strided_iterator& o) const { return m_base >= o.m_base; }
+
+ private:
+  Iterator m_base;
+  stride_type m_stride;
+};
+
+template <class Iterator, class Size>
+__host__ __device__ strided_iterator<Iterator, Size> make_strided_iterator(Iterator it, Size stride) {
+  return strided_iterator<Iterator, Size>(it, stride);
+}
 
 // ------------------------------------------------------------ agents/groups
 template <std::size_t grainsize_ = 1>
@@ -259,7 +411,7 @@ inline async_launch<parallel_group<>> par(future<void>& before, std::size_t n) {
 }
 
 inline concurrent_group<> con(std::size_t size, std::size_t heap_size = use_default) {
-  return concurrent_group<>(int(size == use_default ? 256 : size),
+  return concurrent_group<>(size == use_default ? default_size : int(size),
                             heap_size == use_default ? concurrent_group<>::default_heap
                                                      : int(heap_size));
 }
@@ -268,7 +420,7 @@ template <class ExecutionAgent>
 concurrent_group<ExecutionAgent> con(ExecutionAgent exec, std::size_t size,
                                      std::size_t heap_size = use_default) {
   return concurrent_group<ExecutionAgent>(
-      int(size == use_default ? 256 : size),
+      size == use_default ? default_size : int(size),
       heap_size == use_default ? concurrent_group<ExecutionAgent>::default_heap : int(heap_size),
       exec);
 }
@@ -281,8 +433,8 @@ concurrent_group<agent<grainsize>, groupsize> con(std::size_t heap_size) {
 inline parallel_group<concurrent_group<>> grid(std::size_t num_groups = use_default,
                                                std::size_t group_size = use_default,
                                                std::size_t heap_size = use_default) {
-  if (num_groups == use_default) num_groups = std::size_t(concurrent_group<>::hardware_concurrency());
-  return par(con(group_size, heap_size), num_groups);
+  return parallel_group<concurrent_group<>>(
+      num_groups == use_default ? default_size : int(num_groups), con(group_size, heap_size));
 }
 
 inline async_launch<parallel_group<concurrent_group<>>> grid(std::size_t num_groups,
@@ -647,8 +799,54 @@ void launch(const parallel_group<A, S>& g, hipStream_t s, F f, Args... args) {
       g, f, args...);
 }
 
+// Group size the launcher picks for `kernel` when the group's size was
+// left to it: the block size of maximum potential occupancy
+// (choose_group_size, bulk/detail/cuda_launcher/cuda_launcher.hpp:147-159).
+template <class K>
+inline int default_group_size(K kernel) {
+  int min_grid = 0, block = 0;
+  throw_on_error(hipOccupancyMaxPotentialBlockSize(&min_grid, &block,
+                                                   reinterpret_cast<const void*>(kernel), 0, 1024),
+                 "hipOccupancyMaxPotentialBlockSize in bulk::async");
+  return block > 0 ? block : 256;
+}
+
+// Number of groups when left to the launcher: enough groups of `group_size`
+// to fill every CU's thread slots (choose_num_groups / choose_subscription,
+// cuda_launcher.hpp:163-185).
+inline int default_num_groups(int group_size) {
+  int dev = 0, cus = 0, per_cu = 0;
+  throw_on_error(hipGetDevice(&dev), "hipGetDevice in bulk::async");
+  throw_on_error(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+                 "hipDeviceGetAttribute in bulk::async");
+  throw_on_error(
+      hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxThreadsPerMultiProcessor, dev),
+      "hipDeviceGetAttribute in bulk::async");
+  const int subscription = group_size > 0 ? per_cu / group_size : 0;
+  return subscription > 0 ? subscription * cus : cus;
+}
+
+// The group with its launcher-chosen sizes filled in.
+template <class A, std::size_t S, class K>
+concurrent_group<A, S> resolve(const concurrent_group<A, S>& g, K kernel) {
+  if constexpr (S == 0) {
+    if (g.size() == default_size)
+      return concurrent_group<A, S>(default_group_size(kernel), g.heap_size(), g.this_exec,
+                                    g.index());
+  }
+  return g;
+}
+template <class A, std::size_t S, std::size_t PS, class K>
+parallel_group<concurrent_group<A, S>, PS> resolve(const parallel_group<concurrent_group<A, S>, PS>& g,
+                                                   K kernel) {
+  const concurrent_group<A, S> cg = resolve(g.this_exec, kernel);
+  const int n = g.size() == default_size ? default_num_groups(cg.size()) : g.size();
+  return parallel_group<concurrent_group<A, S>, PS>(n, cg, g.index());
+}
+
 template <class A, std::size_t S, class F, class... Args>
-void launch(const concurrent_group<A, S>& g, hipStream_t s, F f, Args... args) {
+void launch(const concurrent_group<A, S>& g0, hipStream_t s, F f, Args... args) {
+  const concurrent_group<A, S> g = resolve(g0, con_kernel<A, S, F, Args...>);
   if (g.size() <= 0 || g.size() > 1024)
     throw system_error(hipErrorInvalidValue, "bulk::async: a concurrent_group holds 1..1024 agents");
   const std::size_t lds = heap_bytes(con_kernel<A, S, F, Args...>,
@@ -658,11 +856,12 @@ void launch(const concurrent_group<A, S>& g, hipStream_t s, F f, Args... args) {
 }
 
 template <class A, std::size_t S, std::size_t PS, class F, class... Args>
-void launch(const parallel_group<concurrent_group<A, S>, PS>& g, hipStream_t s, F f, Args... args) {
+void launch(const parallel_group<concurrent_group<A, S>, PS>& g0, hipStream_t s, F f, Args... args) {
+  typedef parallel_group<concurrent_group<A, S>, PS> G;
+  const G g = resolve(g0, grid_kernel<A, S, PS, F, Args...>);
   if (g.size() <= 0) return;
   if (g.this_exec.size() <= 0 || g.this_exec.size() > 1024)
     throw system_error(hipErrorInvalidValue, "bulk::async: a concurrent_group holds 1..1024 agents");
-  typedef parallel_group<concurrent_group<A, S>, PS> G;
   const std::size_t lds = heap_bytes(grid_kernel<A, S, PS, F, Args...>, launch_traits<G>::heap(g),
                                      g.this_exec.size());
   allow_lds(grid_kernel<A, S, PS, F, Args...>, lds);
@@ -702,6 +901,18 @@ future<void> async(G g, F f, Args... args) {
   } else {
     return detail::async_in_stream(g, nullptr, nullptr, false, f, args...);
   }
+}
+
+// choose_sizes(g, f, args...): the (number of groups, group size) a launch
+// of f(args...) over the grid g would use, sizes g leaves to the launcher
+// (use_default) filled in from the kernel's occupancy
+// (bulk/choose_sizes.hpp:29-75, bulk/detail/choose_sizes.inl:32-117).
+template <class A, std::size_t S, std::size_t PS, class F, class... Args>
+thrust::pair<int, int> choose_sizes(parallel_group<concurrent_group<A, S>, PS> g, F f,
+                                    Args... args) {
+  (void)f;
+  const auto r = detail::resolve(g, detail::grid_kernel<A, S, PS, F, Args...>);
+  return thrust::make_pair(r.size(), r.this_exec.size());
 }
 
 // ------------------------------------------------------------ group algorithms
@@ -895,6 +1106,45 @@ __device__ It2 exclusive_scan(concurrent_group<Agent, S>& g, It1 first, It1 last
     g.wait();
   }
   return result + n;
+}
+
+// inclusive_scan without init (scan.hpp:489-526): the first input is the
+// init, written through unchanged; in place (result == first) allowed.
+template <class Agent, std::size_t S, class It1, class It2, class Op>
+__device__ It2 inclusive_scan(concurrent_group<Agent, S>& g, It1 first, It1 last, It2 result,
+                              Op op) {
+  if (first < last) {
+    const typename std::iterator_traits<It1>::value_type init = *first;
+    g.wait();  // every agent has read *first before result[0] may overwrite it
+    if (g.this_exec.index() == 0) *result = init;
+    inclusive_scan(g, first + 1, last, result + 1, init, op);
+  }
+  return result + (last - first);
+}
+
+// The bounded-group forms (scan.hpp:222-313, 391-452, 529-550): a group
+// whose input is statically bounded by one tile (bound <= size * grain)
+// scans it in one pass.  The reference's bounded exclusive_scan forwards to
+// its inclusive form (scan.hpp:548); this one computes the exclusive scan
+// its name and the unbounded form (scan.hpp:553-591) promise.
+template <std::size_t B, std::size_t S, std::size_t G, class It1, class It2, class T, class Op>
+__device__ typename std::enable_if<(B <= S * G), It2>::type inclusive_scan(
+    bounded<B, concurrent_group<agent<G>, S>>& g, It1 first, It1 last, It2 result, T init, Op op) {
+  return inclusive_scan(static_cast<concurrent_group<agent<G>, S>&>(g), first, last, result, init,
+                        op);
+}
+
+template <std::size_t B, std::size_t S, std::size_t G, class It1, class It2, class Op>
+__device__ typename std::enable_if<(B <= S * G), It2>::type inclusive_scan(
+    bounded<B, concurrent_group<agent<G>, S>>& g, It1 first, It1 last, It2 result, Op op) {
+  return inclusive_scan(static_cast<concurrent_group<agent<G>, S>&>(g), first, last, result, op);
+}
+
+template <std::size_t B, std::size_t S, std::size_t G, class It1, class It2, class T, class Op>
+__device__ typename std::enable_if<(B <= S * G), It2>::type exclusive_scan(
+    bounded<B, concurrent_group<agent<G>, S>>& g, It1 first, It1 last, It2 result, T init, Op op) {
+  return exclusive_scan(static_cast<concurrent_group<agent<G>, S>&>(g), first, last, result, init,
+                        op);
 }
 
 // merge_path(a, n1, b, n2, diag, comp): how many of the first diag outputs of

@@ -1,6 +1,6 @@
 """The header-only call surface (include/bulkx/bulk.hpp) and the C++ driver
 facade (include/bulkx/algorithm.hpp): they compile on their own (CPU), and
-the test program built from tests/cpp/surface_test.hip passes on the GPU."""
+the test programs built from tests/cpp/*.hip pass on the GPU."""
 import os
 import shutil
 import subprocess
@@ -25,7 +25,7 @@ def test_header_compiles_alone(tmp_path, header):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("program", ["surface_test", "heap_test"])
+@pytest.mark.parametrize("program", ["surface_test", "heap_test", "generic_test"])
 def test_surface_program_passes(program):
     exe = os.path.join(BIN, program)
     assert os.path.exists(exe), f"tests/cpp/_bin/{program} is not built (run the build first)"
