@@ -539,7 +539,8 @@ __device__ __forceinline__ void striped_load(const T* chunk, int count, T (&x)[I
   }
 }
 
-template <class T, int ITEMS, int VE = 16 / int(sizeof(T))>
+// NT = nontemporal stores (outputs not read again by this kernel).
+template <class T, int ITEMS, int VE = 16 / int(sizeof(T)), bool NT = false>
 __device__ __forceinline__ void striped_store(T* chunk, int count, const T (&x)[ITEMS]) {
   using S = Striped<T, ITEMS, VE>;
   using RV = typename RawVec<VE * sizeof(T)>::type;
@@ -552,7 +553,8 @@ __device__ __forceinline__ void striped_store(T* chunk, int count, const T (&x)[
       T* e = reinterpret_cast<T*>(&t);
 #pragma unroll
       for (int k = 0; k < S::VE; ++k) e[k] = x[j * S::VE + k];
-      q[j * kWave + lane] = t;
+      if constexpr (NT) __builtin_nontemporal_store(t, q + j * kWave + lane);
+      else q[j * kWave + lane] = t;
     }
   } else {
 #pragma unroll

@@ -36,7 +36,9 @@ __device__ __forceinline__ void scan_tile(ScanSmem<T, Op, BLOCK>& sm, const T* i
   const A aid = AOp::template identity<A>();
 
   T x[ITEMS];
-  striped_load<T, ITEMS>(in + (count > 0 ? chunk_base : 0), count, x, id);
+  // Nontemporal loads and stores: every element is read once and written
+  // once, so neither stream is worth a place in the caches.
+  striped_load<T, ITEMS, SL::VE, true>(in + (count > 0 ? chunk_base : 0), count, x, id);
 
   // Lane-local inclusive scan of each 16-byte vector (the bounded-agent
   // body of bulk::inclusive_scan, scan.hpp:35-55, over VE elements).
@@ -108,7 +110,7 @@ __device__ __forceinline__ void scan_tile(ScanSmem<T, Op, BLOCK>& sm, const T* i
     }
   }
   if constexpr (WRITE) {
-    if (count > 0) striped_store<T, ITEMS>(out + chunk_base, count, x);
+    if (count > 0) striped_store<T, ITEMS, SL::VE, true>(out + chunk_base, count, x);
   } else {
     if (threadIdx.x == 0 && tile * TILE + TILE >= n)
       *total = AccP::to(AOp::apply(tile_prefix, tile_agg));

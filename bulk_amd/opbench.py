@@ -47,6 +47,26 @@ def _timeit(fn, iters=10, warm=2, stream=None):
     return ts[len(ts) // 2]
 
 
+def _timeit_back_to_back(fn, calls=20, iters=5):
+    """Milliseconds per call of `calls` calls enqueued back to back between
+    two events (median over `iters`): the device-side cost of a call once
+    the host's launch latency is hidden behind the previous call."""
+    fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(iters):
+        s = torch.cuda.Event(enable_timing=True)
+        e = torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(calls):
+            fn()
+        e.record()
+        torch.cuda.synchronize()
+        ts.append(s.elapsed_time(e) / calls)
+    ts.sort()
+    return ts[len(ts) // 2]
+
+
 def _row(name, ms, nbytes, elements, **extra):
     gbs = nbytes / (ms * 1e-3) / 1e9
     return {"op": name, "ms": ms, "bytes": nbytes, "gbs": gbs, "frac_peak": gbs / HBM_PEAK_GBS,
@@ -159,8 +179,10 @@ def run(iters: int = 10, log=print, large: int = 28) -> list[dict]:
     x = torch.randint(-1000, 1001, (n,), generator=g, device=dev, dtype=torch.int32)
     y = torch.empty_like(x)
     ws = api.Workspace.for_algo(C.ALGO_INCLUSIVE_SCAN, C.INT32, -1, n)
-    rows.append(_row("inclusive_scan int32 2^24", _timeit(lambda: api.inclusive_scan(x, y, workspace=ws), iters),
-                     2 * n * 4, n, exact=bool(torch.equal(y, torch.cumsum(x, 0, dtype=torch.int32)))))
+    call = lambda: api.inclusive_scan(x, y, workspace=ws)  # noqa: E731
+    rows.append(_row("inclusive_scan int32 2^24", _timeit(call, iters), 2 * n * 4, n,
+                     exact=bool(torch.equal(y, torch.cumsum(x, 0, dtype=torch.int32))),
+                     ms_back_to_back=_timeit_back_to_back(call, 20, iters)))
     del x, y, ws
 
     # reduce / scan at 2^large in f32 / f64, uniform and normal
@@ -294,7 +316,8 @@ def run(iters: int = 10, log=print, large: int = 28) -> list[dict]:
         del keys
         torch.cuda.empty_cache()
     for r in rows:
-        log(f"{r['op']:46s} {r['ms'] * 1e3:9.1f} us {r['gbs']:7.0f} GB/s {100 * r['frac_peak']:5.1f}%")
+        b2b = f"  (back to back {r['ms_back_to_back'] * 1e3:.1f} us)" if "ms_back_to_back" in r else ""
+        log(f"{r['op']:46s} {r['ms'] * 1e3:9.1f} us {r['gbs']:7.0f} GB/s {100 * r['frac_peak']:5.1f}%{b2b}")
     return rows
 
 
@@ -304,5 +327,11 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser(description="single-op table at full size (one MI355X)")
     ap.add_argument("--iters", type=int, default=10)
     ap.add_argument("--large", type=int, default=28)
+    ap.add_argument("--json", default=None, help="also write the rows to this file")
     a = ap.parse_args()
-    run(iters=a.iters, large=a.large)
+    rows = run(iters=a.iters, large=a.large)
+    if a.json:
+        import json
+
+        with open(a.json, "w") as f:
+            json.dump(rows, f, indent=1)
