@@ -86,6 +86,7 @@ constexpr int kRbkShapes = 6;
 constexpr int kRbkVariants = 9;
 // tile shape of a variant
 constexpr int rbk_shape(int variant) { return variant == 6 ? 0 : variant == 7 ? 2 : variant == 8 ? 5 : variant; }
+enum RbkForm { kRbkSinglePass = 0, kRbkThreePass = 1 };
 constexpr int kRbkMinTile = 4096;  // smallest tile of any variant (workspace sizing)
 
 // Look-back window of 64*Q tiles (Q tiles per lane).  Q = 1 measured
@@ -94,8 +95,6 @@ constexpr int kRbkMinTile = 4096;  // smallest tile of any variant (workspace si
 #define BULKX_RBK_LOOKBACK_Q 1
 #endif
 constexpr int kRbkLookbackQ = BULKX_RBK_LOOKBACK_Q;
-
-
 constexpr size_t kRbkHeaderBytes = 256;
 constexpr int kRbkMaxStatusWords = 6;  // Seg<dd> = 24 bytes
 
@@ -140,79 +139,85 @@ constexpr int rbk_scan_seg() {
   return 256 * rbk_scan_items<S>();
 }
 
-// One tile of reduce_by_key (shared by the device-wide kernel and the batch
-// executor).
+// One wavefront's chunk of a tile while its loads are in flight: the keys
+// and values as coalesced 16-byte vectors (full aligned chunks; a partial
+// chunk is read element by element by rbk_lane_pass), plus the keys just
+// before and after the chunk.
+template <class K, class V, int ITEMS>
+struct RbkChunk {
+  u32x4 tk[WaveTile<K, ITEMS>::NV];
+  u32x4 tv[WaveTile<V, ITEMS>::NV];
+  K gprev, gnext;
+  int64_t base;  // index of the chunk's first element
+  int count;     // elements of the chunk inside [0, n)
+};
 
-template <class K, class V, class Op, int BLOCK, int ITEMS>
-__device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, const K* keys,
-                                         const V* vals, int64_t n, K* kout, V* vout,
-                                         int64_t* num_runs, int64_t tile, uint64_t* words,
-                                         LookbackGuard* guard, bool single_tile,
-                                         const Seg<typename Acc<V, Op>::type>* prefixes = nullptr,
-                                         const Seg<typename Acc<V, Op>::type>* group_prefixes = nullptr,
-                                         const uint8_t* skip = nullptr, int64_t skip_tiles = 0) {
-  using AccP = Acc<V, Op>;
-  using A = typename AccP::type;
-  using AOp = typename AccP::op;
-  using S = Seg<A>;
-  using SOp = SegOp<AOp, A>;
-  using RS = RowSeg<V>;
-  using RSOp = RowSegOp<Op, V>;
+// Issues the key loads of the chunk at `chunk_base` (no wait).
+template <class K, class V, int ITEMS>
+__device__ __forceinline__ void rbk_load_keys(const K* keys, int64_t n, int64_t chunk_base,
+                                              RbkChunk<K, V, ITEMS>& c, int lane) {
   using WK = WaveTile<K, ITEMS>;
-  using WV = WaveTile<V, ITEMS>;
-  constexpr int TILE = BLOCK * ITEMS;
-  constexpr int NW = BLOCK / kWave;
   constexpr int N = WK::N;
-  const TileStatus<S> st{words};
-  const int wid = threadIdx.x / kWave;
-  const int lane = lane_id();
-  // Three-pass form: prefix of the tile's scan segment, then the tile's
-  // exclusive prefix inside it, read before the tile's loads so that the
-  // round trip overlaps them.
-  S known = SOp::template identity<S>();
-  if (prefixes)
-    known = group_prefixes ? SOp::apply(group_prefixes[tile / rbk_scan_seg<S>()], prefixes[tile])
-                           : prefixes[tile];
-  const int64_t chunk_base = tile * TILE + int64_t(wid) * N;
   const int64_t rem = n - chunk_base;
-  const int count = rem <= 0 ? 0 : (rem >= N ? N : int(rem));
-  const int64_t cb = count > 0 ? chunk_base : 0;
-  K* lk = reinterpret_cast<K*>(sm.region[wid]);
-  V* lv = reinterpret_cast<V*>(sm.region[wid]);
-
-  // Coalesced 16-byte loads of keys and values, all in flight before the
-  // first LDS write, then transposed in turn through the region to ITEMS
-  // consecutive elements per lane.
-  const bool fullk = count >= N && aligned16(keys + cb);
-  const bool fullv = count >= N && aligned16(vals + cb);
-  u32x4 tk[WK::NV], tv[WV::NV];
-  if (fullk) {
+  c.base = chunk_base;
+  c.count = rem <= 0 ? 0 : (rem >= N ? N : int(rem));
+  const int64_t cb = c.count > 0 ? chunk_base : 0;
+  if (c.count >= N && aligned16(keys + cb)) {
     const u32x4* q = reinterpret_cast<const u32x4*>(keys + cb);
 #pragma unroll
-    for (int j = 0; j < WK::NV; ++j) tk[j] = __builtin_nontemporal_load(q + j * kWave + lane);
+    for (int j = 0; j < WK::NV; ++j) c.tk[j] = __builtin_nontemporal_load(q + j * kWave + lane);
   }
-  // Three-pass form with `skip` (the upsweep's flags): a tile in which no
-  // run ends writes nothing, so it stops here, while its key loads are in
-  // flight and before its values are read.
-  if (skip != nullptr && tile + 1 < skip_tiles && !(skip[tile] & 2u) && !(skip[tile + 1] & 1u))
-    return;
-  if (fullv) {
+  c.gprev = K(0);
+  c.gnext = K(0);
+  if (lane == 0 && c.count > 0 && chunk_base > 0) c.gprev = keys[chunk_base - 1];
+  if (lane == kWave - 1 && c.count >= N && chunk_base + N < n) c.gnext = keys[chunk_base + N];
+}
+
+// Issues the value loads of the chunk whose keys rbk_load_keys requested.
+template <class K, class V, int ITEMS>
+__device__ __forceinline__ void rbk_load_values(const V* vals, RbkChunk<K, V, ITEMS>& c, int lane) {
+  using WV = WaveTile<V, ITEMS>;
+  const int64_t cb = c.count > 0 ? c.base : 0;
+  if (c.count >= WV::N && aligned16(vals + cb)) {
     const u32x4* q = reinterpret_cast<const u32x4*>(vals + cb);
 #pragma unroll
-    for (int j = 0; j < WV::NV; ++j) tv[j] = __builtin_nontemporal_load(q + j * kWave + lane);
+    for (int j = 0; j < WV::NV; ++j) c.tv[j] = __builtin_nontemporal_load(q + j * kWave + lane);
   }
-  K gprev = K(0), gnext = K(0);
-  if (lane == 0 && count > 0 && chunk_base > 0) gprev = keys[chunk_base - 1];
-  if (lane == kWave - 1 && count >= N && chunk_base + N < n) gnext = keys[chunk_base + N];
+}
+
+// The lane's part of a chunk after its loads: keys and values transposed
+// through the wave's LDS region to ITEMS consecutive elements per lane,
+// head flags h and tail flags t (bit i: element i), each v[i] replaced by
+// the fold since its run head (for a tail, the run's output value once the
+// lane's prefix is folded in where no head precedes it in the lane), and the
+// lane aggregate (heads, value since the last head).  Values past the end
+// of the array only extend the last run's aggregate, which nothing reads.
+template <class K, class V, class Op, int ITEMS>
+struct RbkLane {
   K k[ITEMS];
   V v[ITEMS];
-  if (fullk) {
+  uint32_t h, t;
+  RowSeg<V> a;
+};
+
+template <class K, class V, class Op, int ITEMS>
+__device__ __forceinline__ void rbk_lane_pass(const K* keys, const V* vals, int64_t n,
+                                              const RbkChunk<K, V, ITEMS>& c, char* region,
+                                              RbkLane<K, V, Op, ITEMS>& L, int lane) {
+  using WK = WaveTile<K, ITEMS>;
+  using WV = WaveTile<V, ITEMS>;
+  constexpr int N = WK::N;
+  const int count = c.count;
+  const int64_t cb = count > 0 ? c.base : 0;
+  K* lk = reinterpret_cast<K*>(region);
+  V* lv = reinterpret_cast<V*>(region);
+  if (count >= N && aligned16(keys + cb)) {
 #pragma unroll
     for (int j = 0; j < WK::NV; ++j) {
-      const K* e = reinterpret_cast<const K*>(&tk[j]);
+      const K* e = reinterpret_cast<const K*>(&c.tk[j]);
       const int base = WK::pos((j * kWave + lane) * WK::VE);
 #pragma unroll
-      for (int c = 0; c < WK::VE; ++c) lk[base + c] = e[c];
+      for (int q = 0; q < WK::VE; ++q) lk[base + q] = e[q];
     }
   } else {
 #pragma unroll
@@ -223,17 +228,17 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
   }
   wave_lds_sync();
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i) k[i] = lk[WK::pos(lane * ITEMS + i)];
-  const K kprev = lane > 0 ? lk[WK::pos(lane * ITEMS - 1)] : gprev;
-  const K knext = lane < kWave - 1 ? lk[WK::pos(lane * ITEMS + ITEMS)] : gnext;
+  for (int i = 0; i < ITEMS; ++i) L.k[i] = lk[WK::pos(lane * ITEMS + i)];
+  const K kprev = lane > 0 ? lk[WK::pos(lane * ITEMS - 1)] : c.gprev;
+  const K knext = lane < kWave - 1 ? lk[WK::pos(lane * ITEMS + ITEMS)] : c.gnext;
   wave_lds_sync();  // key reads done before the values reuse the region
-  if (fullv) {
+  if (count >= N && aligned16(vals + cb)) {
 #pragma unroll
     for (int j = 0; j < WV::NV; ++j) {
-      const V* e = reinterpret_cast<const V*>(&tv[j]);
+      const V* e = reinterpret_cast<const V*>(&c.tv[j]);
       const int base = WV::pos((j * kWave + lane) * WV::VE);
 #pragma unroll
-      for (int c = 0; c < WV::VE; ++c) lv[base + c] = e[c];
+      for (int q = 0; q < WV::VE; ++q) lv[base + q] = e[q];
     }
   } else {
 #pragma unroll
@@ -244,8 +249,8 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
   }
   wave_lds_sync();
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i) v[i] = lv[WV::pos(lane * ITEMS + i)];
-  wave_lds_sync();  // the region is free again (used as the output stages below)
+  for (int i = 0; i < ITEMS; ++i) L.v[i] = lv[WV::pos(lane * ITEMS + i)];
+  wave_lds_sync();  // the region is free again (used as the output stages)
 
   // Head / tail flags (head_flags.hpp, tail_flags.hpp): one key comparison
   // per element; a tail is the element before a head.
@@ -254,39 +259,171 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
   const uint32_t vmask = valid >= 32 ? ~0u : ((1u << valid) - 1);
   uint32_t h = 0;
 #pragma unroll
-  for (int i = 0; i < ITEMS; ++i) h |= uint32_t(!(k[i] == (i ? k[i - 1] : kprev))) << i;
-  uint32_t t = (h >> 1) | (uint32_t(!(k[ITEMS - 1] == knext)) << (ITEMS - 1));
-  if (chunk_base == 0 && lane == 0) h |= 1u;  // the array's first element
+  for (int i = 0; i < ITEMS; ++i) h |= uint32_t(!(L.k[i] == (i ? L.k[i - 1] : kprev))) << i;
+  uint32_t t = (h >> 1) | (uint32_t(!(L.k[ITEMS - 1] == knext)) << (ITEMS - 1));
+  if (c.base == 0 && lane == 0) h |= 1u;  // the array's first element
   // The array's last element ends a run wherever it falls in the lane (the
   // element after it in the lane's slice is fill, which may equal its key).
-  if (valid > 0 && chunk_base + int64_t(lane) * ITEMS + valid == n) t |= 1u << (valid - 1);
-  h &= vmask;
-  t &= vmask;
-
-  // Lane aggregate (count of heads, value since the last head), DPP scan
-  // across the wave, LDS combine across waves.  Values past the end of the
-  // array only extend the last run's aggregate, which nothing reads.
-  // The fold since each element's run head replaces v[i] (for a tail it is
-  // the run's output value, once the lane's prefix is folded in where no
-  // head precedes it in the lane).
-  RS a;
-  a.c = uint32_t(__builtin_popcount(h));
-  a.v = Op::template identity<V>();
+  if (valid > 0 && c.base + int64_t(lane) * ITEMS + valid == n) t |= 1u << (valid - 1);
+  L.h = h & vmask;
+  L.t = t & vmask;
+  L.a.c = uint32_t(__builtin_popcount(L.h));
+  L.a.v = Op::template identity<V>();
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
-    a.v = (h >> i) & 1u ? v[i] : Op::apply(a.v, v[i]);
-    v[i] = a.v;
+    L.a.v = (L.h >> i) & 1u ? L.v[i] : Op::apply(L.a.v, L.v[i]);
+    L.v[i] = L.a.v;
   }
-  const RS inc = wave_inclusive_scan<RSOp>(a);
+}
+
+template <class V, class Op>
+__device__ __forceinline__ Seg<typename Acc<V, Op>::type> rbk_lift(RowSeg<V> r) {
+  Seg<typename Acc<V, Op>::type> x;
+  x.c = r.c;
+  x.v = Acc<V, Op>::from(r.v);
+  return x;
+}
+
+// The walk, first half: a tail's run has slot (heads so far) - 1.  The
+// chunk's tails hold consecutive slots from base (the run open at the
+// chunk's first element), so slot - base < 64*ITEMS is a 32-bit stage
+// index.  Slots are kept relative to the tile (base_rel) until the tile's
+// prefix arrives.  This half stages the output keys.  It has no branches
+// and no chain: a tail's slot is sbase + (heads up to it in the lane),
+// masked into the stage; an element that is no tail writes to a spare slot
+// past the N staged ones (one per lane, or lane pair).  The lane's one
+// tail that may precede its first head closes the run carried into the
+// lane; its value needs the tile's prefix and is patched in the second half.
+template <class K, class V, class Op, int ITEMS>
+struct RbkWalk {
+  Seg<typename Acc<V, Op>::type> q;  // in-tile prefix of the lane
+  int64_t base_rel;
+  uint32_t ntails, sbase;
+  bool pre;
+  V pre_local;
+};
+
+template <class K, class V, class Op, int ITEMS>
+__device__ __forceinline__ RbkWalk<K, V, Op, ITEMS> rbk_stage_keys(
+    const RbkLane<K, V, Op, ITEMS>& L, Seg<typename Acc<V, Op>::type> q, char* region, int lane) {
+  constexpr int N = WaveTile<K, ITEMS>::N;
+  RbkWalk<K, V, Op, ITEMS> w;
+  w.q = q;
+  w.base_rel = readlane(int64_t(q.c) + int64_t(L.h & 1u) - 1, 0);
+  w.ntails = wave_reduce<OpPlus>(uint32_t(__builtin_popcount(L.t)));
+  K* sk = reinterpret_cast<K*>(region);
+  constexpr uint32_t KSPARE = uint32_t(RbkLds<K, V, ITEMS>::BYTES / int(sizeof(K)) - N);
+  static_assert(KSPARE >= 1, "the region has spare slots past the stage");
+  static_assert((N & (N - 1)) == 0, "the stage index is masked with N - 1");
+  const uint32_t kspare = uint32_t(N) + uint32_t(lane) % KSPARE;
+  const uint32_t e0 = uint32_t(int64_t(q.c) - w.base_rel);
+  w.sbase = e0 - 1;  // slot of a tail with no head before it in the lane
+  w.pre = false;     // the lane has such a tail
+  w.pre_local = Op::template identity<V>();
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint32_t upto = L.h & (i == 31 ? ~0u : ((2u << i) - 1u));  // heads up to element i
+    const uint32_t slot = (uint32_t(__builtin_popcount(upto)) + w.sbase) & uint32_t(N - 1);
+    const bool ti = (L.t >> i) & 1u;
+    sk[ti ? slot : kspare] = L.k[i];
+    const bool p = ti && upto == 0;
+    w.pre_local = p ? L.v[i] : w.pre_local;
+    w.pre |= p;
+  }
+  return w;
+}
+
+// The walk, second half, once the tile's exclusive prefix is known: the
+// staged keys go out, the values are staged (the pre-head tail's value with
+// the prefix folded in) and go out, all with coalesced stores.
+template <class K, class V, class Op, int ITEMS>
+__device__ __forceinline__ void rbk_write_runs(const RbkLane<K, V, Op, ITEMS>& L,
+                                               const RbkWalk<K, V, Op, ITEMS>& w,
+                                               Seg<typename Acc<V, Op>::type> tile_prefix,
+                                               int64_t n, K* kout, V* vout, char* region,
+                                               int lane) {
+  using AccP = Acc<V, Op>;
+  using AOp = typename AccP::op;
+  constexpr int N = WaveTile<K, ITEMS>::N;
+  K* sk = reinterpret_cast<K*>(region);
+  V* sv = reinterpret_cast<V*>(region);
+  constexpr uint32_t VSPARE = uint32_t(RbkLds<K, V, ITEMS>::BYTES / int(sizeof(V)) - N);
+  static_assert(VSPARE >= 1, "the region has spare slots past the stage");
+  const uint32_t vspare = uint32_t(N) + uint32_t(lane) % VSPARE;
+  // The pre-head tail's value: the lane's in-tile prefix folded in, and the
+  // tile's prefix too when no head precedes the lane in the tile (same
+  // association as folding the full prefix first: (tile_prefix . q) . local).
+  V pre_val = Op::template identity<V>();
+  if (w.pre)
+    pre_val = w.q.c > 0
+                  ? AccP::to(AOp::apply(w.q.v, AccP::from(w.pre_local)))
+                  : AccP::to(AOp::apply(AOp::apply(tile_prefix.v, w.q.v), AccP::from(w.pre_local)));
+  const int64_t base = int64_t(tile_prefix.c) + w.base_rel;
+  // bounds guard: slots stay below n when the scan is consistent
+  uint32_t cnt = w.ntails < uint32_t(N) ? w.ntails : uint32_t(N);
+  if (base < 0 || base >= n) cnt = 0;
+  else if (base + int64_t(cnt) > n) cnt = uint32_t(n - base);
+  for (uint32_t x = lane; x < cnt; x += kWave) kout[base + x] = sk[x];
+  wave_lds_sync();  // key stage read before the value stage overwrites it
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint32_t upto = L.h & (i == 31 ? ~0u : ((2u << i) - 1u));
+    const uint32_t slot = (uint32_t(__builtin_popcount(upto)) + w.sbase) & uint32_t(N - 1);
+    sv[(L.t >> i) & 1u ? slot : vspare] = L.v[i];
+  }
+  if (w.pre) sv[w.sbase & uint32_t(N - 1)] = pre_val;
+  wave_lds_sync();
+  for (uint32_t x = lane; x < cnt; x += kWave) vout[base + x] = sv[x];
+}
+
+// One tile of reduce_by_key (shared by the device-wide kernels and the
+// batch executor).
+template <class K, class V, class Op, int BLOCK, int ITEMS>
+__device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, const K* keys,
+                                         const V* vals, int64_t n, K* kout, V* vout,
+                                         int64_t* num_runs, int64_t tile, uint64_t* words,
+                                         LookbackGuard* guard, bool single_tile,
+                                         const Seg<typename Acc<V, Op>::type>* prefixes = nullptr,
+                                         const Seg<typename Acc<V, Op>::type>* group_prefixes = nullptr,
+                                         const uint8_t* skip = nullptr, int64_t skip_tiles = 0) {
+  using AccP = Acc<V, Op>;
+  using A = typename AccP::type;
+  using S = Seg<A>;
+  using SOp = SegOp<typename AccP::op, A>;
+  using RS = RowSeg<V>;
+  using RSOp = RowSegOp<Op, V>;
+  constexpr int TILE = BLOCK * ITEMS;
+  constexpr int NW = BLOCK / kWave;
+  constexpr int N = WaveTile<K, ITEMS>::N;
+  const TileStatus<S> st{words};
+  const int wid = threadIdx.x / kWave;
+  // Three-pass form: prefix of the tile's scan segment, then the tile's
+  // exclusive prefix inside it, read before the tile's loads so that the
+  // round trip overlaps them.
+  S known = SOp::template identity<S>();
+  if (prefixes)
+    known = group_prefixes ? SOp::apply(group_prefixes[tile / rbk_scan_seg<S>()], prefixes[tile])
+                           : prefixes[tile];
+  char* region = sm.region[wid];
+  // Coalesced 16-byte loads of keys and values, all in flight before the
+  // first LDS write.
+  const int lane = lane_id();
+  RbkChunk<K, V, ITEMS> c;
+  rbk_load_keys(keys, n, tile * TILE + int64_t(wid) * N, c, lane);
+  // Three-pass form with `skip` (the upsweep's flags): a tile in which no
+  // run ends writes nothing, so it stops here, while its key loads are in
+  // flight and before its values are read.
+  if (skip != nullptr && tile + 1 < skip_tiles && !(skip[tile] & 2u) && !(skip[tile + 1] & 1u))
+    return;
+  rbk_load_values(vals, c, lane);
+  RbkLane<K, V, Op, ITEMS> L;
+  rbk_lane_pass(keys, vals, n, c, region, L, lane);
+
+  // Lane aggregates: DPP scan across the wave, LDS combine across waves.
+  const RS inc = wave_inclusive_scan<RSOp>(L.a);
   const RS lane_prefix = wave_shift_right<RSOp>(inc);
   const RS wave_total = readlane(inc, kWave - 1);
-  auto lift = [](RS r) {
-    S x;
-    x.c = r.c;
-    x.v = AccP::from(r.v);
-    return x;
-  };
-  if (lane == 0) sm.wave[wid] = lift(wave_total);
+  if (lane == 0) sm.wave[wid] = rbk_lift<V, Op>(wave_total);
   __syncthreads();
   S wave_prefix = SOp::template identity<S>(), tile_agg = SOp::template identity<S>();
 #pragma unroll
@@ -297,12 +434,10 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
   }
 
   // Publish this tile's aggregate (or, for tile 0, its inclusive prefix),
-  // then let wavefront 0 look back while the other wavefronts already walk
-  // their chunks: the walk only needs the in-tile prefix, except for the
-  // value of the tile's first run when that run began in an earlier tile,
-  // which is patched once the tile's prefix is known.
-  // With `prefixes` (the three-pass form) the tile's exclusive prefix is
-  // already known: nothing is published and nothing looked back.
+  // then let wavefront 0 look back while the other wavefronts already stage
+  // their keys: that only needs the in-tile prefix.  With `prefixes` (the
+  // three-pass form) the tile's exclusive prefix is already known: nothing
+  // is published and nothing looked back.
   const bool chained = !single_tile && tile > 0 && prefixes == nullptr;
   if (!single_tile && prefixes == nullptr && threadIdx.x == 0)
     st.publish(tile, tile == 0 ? kStatusPrefix : kStatusAggregate, tile_agg);
@@ -313,70 +448,14 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
       sm.prefix = excl;
     }
   }
-
-  // Walk: a tail's run has slot (heads so far) - 1.  The chunk's tails hold
-  // consecutive slots from base (the run open at the chunk's first element),
-  // so slot - base < 64*ITEMS is a 32-bit stage index.  Slots are kept
-  // relative to the tile (base_rel) until the tile's prefix arrives.  The
-  // walk stages the output keys.  It has no branches and no chain: a tail's
-  // slot is sbase + (heads up to it in the lane), masked into the stage; an
-  // element that is no tail writes to a spare slot past the N staged ones
-  // (one per lane, or lane pair).  The lane's one tail that may precede its
-  // first head closes the run carried into the lane; its value needs the
-  // prefix and is patched once, after the walk.
-  const S q = SOp::apply(wave_prefix, lift(lane_prefix));  // in-tile prefix of the lane
-  const int64_t base_rel = readlane(int64_t(q.c) + int64_t(h & 1u) - 1, 0);
-  const uint32_t ntails = wave_reduce<OpPlus>(uint32_t(__builtin_popcount(t)));
-  K* sk = lk;
-  V* sv = lv;
-  constexpr uint32_t KSPARE = uint32_t(RbkLds<K, V, ITEMS>::BYTES / int(sizeof(K)) - N);
-  constexpr uint32_t VSPARE = uint32_t(RbkLds<K, V, ITEMS>::BYTES / int(sizeof(V)) - N);
-  static_assert(KSPARE >= 1 && VSPARE >= 1, "the region has spare slots past the stage");
-  static_assert((N & (N - 1)) == 0, "the stage index is masked with N - 1");
-  const uint32_t kspare = uint32_t(N) + uint32_t(lane) % KSPARE;
-  const uint32_t vspare = uint32_t(N) + uint32_t(lane) % VSPARE;
-  const uint32_t e0 = uint32_t(int64_t(q.c) - base_rel);
-  const uint32_t sbase = e0 - 1;  // slot of a tail with no head before it in the lane
-  bool pre = false;               // the lane has such a tail
-  V pre_local = Op::template identity<V>();
-#pragma unroll
-  for (int i = 0; i < ITEMS; ++i) {
-    const uint32_t upto = h & (i == 31 ? ~0u : ((2u << i) - 1u));  // heads up to element i
-    const uint32_t slot = (uint32_t(__builtin_popcount(upto)) + sbase) & uint32_t(N - 1);
-    const bool ti = (t >> i) & 1u;
-    sk[ti ? slot : kspare] = k[i];
-    const bool p = ti && upto == 0;
-    pre_local = p ? v[i] : pre_local;
-    pre |= p;
-  }
+  const RbkWalk<K, V, Op, ITEMS> w =
+      rbk_stage_keys(L, SOp::apply(wave_prefix, rbk_lift<V, Op>(lane_prefix)), region, lane);
   __syncthreads();  // the tile's prefix is in sm.prefix
   const S tile_prefix = prefixes ? known : chained ? sm.prefix : SOp::template identity<S>();
   // The tile holding element n-1 reports the run count.
   if (threadIdx.x == 0 && tile * TILE + TILE >= n)
     *num_runs = int64_t(tile_prefix.c + tile_agg.c);
-  // The pre-head tail's value: the lane's in-tile prefix folded in, and the
-  // tile's prefix too when no head precedes the lane in the tile (same
-  // association as folding the full prefix first: (tile_prefix . q) . local).
-  V pre_val = Op::template identity<V>();
-  if (pre)
-    pre_val = q.c > 0 ? AccP::to(AOp::apply(q.v, AccP::from(pre_local)))
-                      : AccP::to(AOp::apply(AOp::apply(tile_prefix.v, q.v), AccP::from(pre_local)));
-  const int64_t base = int64_t(tile_prefix.c) + base_rel;
-  // bounds guard: slots stay below n when the scan is consistent
-  uint32_t cnt = ntails < uint32_t(N) ? ntails : uint32_t(N);
-  if (base < 0 || base >= n) cnt = 0;
-  else if (base + int64_t(cnt) > n) cnt = uint32_t(n - base);
-  for (uint32_t x = lane; x < cnt; x += kWave) kout[base + x] = sk[x];
-  wave_lds_sync();  // key stage read before the value stage overwrites it
-#pragma unroll
-  for (int i = 0; i < ITEMS; ++i) {
-    const uint32_t upto = h & (i == 31 ? ~0u : ((2u << i) - 1u));
-    const uint32_t slot = (uint32_t(__builtin_popcount(upto)) + sbase) & uint32_t(N - 1);
-    sv[(t >> i) & 1u ? slot : vspare] = v[i];
-  }
-  if (pre) sv[sbase & uint32_t(N - 1)] = pre_val;
-  wave_lds_sync();
-  for (uint32_t x = lane; x < cnt; x += kWave) vout[base + x] = sv[x];
+  rbk_write_runs(L, w, tile_prefix, n, kout, vout, region, lane);
 }
 
 
@@ -667,13 +746,12 @@ __global__ __launch_bounds__(BLOCK) void rbk_downsweep_kernel(
 __global__ void rbk_empty_kernel(int64_t* num_runs);
 __global__ void rbk_prepare_kernel(uint32_t* header, uint64_t* words, int64_t num_words);
 
-template <class K, class V, int VAR>
+template <class C>
 inline int64_t rbk_num_tiles(int64_t n) {
-  constexpr int TILE = RbkConfig<K, V, VAR>::TILE;
-  return (n + TILE - 1) / TILE;
+  return (n + C::TILE - 1) / C::TILE;
 }
 
-template <class K, class V, class Op, int VAR, bool THREE_PASS = false>
+template <class K, class V, class Op, int VAR, int FORM = kRbkSinglePass>
 int rbk_launch(const K* keys, const V* vals, int64_t n, K* kout, V* vout, int64_t* num_runs,
                char* ws, hipStream_t stream) {
   using C = RbkConfig<K, V, VAR>;
@@ -683,14 +761,14 @@ int rbk_launch(const K* keys, const V* vals, int64_t n, K* kout, V* vout, int64_
     BULKX_LAUNCHED();
     return BULKX_SUCCESS;
   }
-  const int64_t tiles = rbk_num_tiles<K, V, VAR>(n);
+  const int64_t tiles = rbk_num_tiles<C>(n);
   if (tiles == 1) {
     reduce_by_key_kernel<K, V, Op, C::BLOCK, C::ITEMS><<<1, C::BLOCK, 0, stream>>>(
         keys, vals, n, kout, vout, num_runs, nullptr, nullptr, nullptr, true);
     BULKX_LAUNCHED();
     return BULKX_SUCCESS;
   }
-  if constexpr (THREE_PASS) {
+  if constexpr (FORM == kRbkThreePass) {
     S* aggs = reinterpret_cast<S*>(ws + kRbkHeaderBytes);
     // the segment totals follow the aggregates, then one flag byte per tile
     const int64_t groups = (tiles + rbk_scan_seg<S>() - 1) / rbk_scan_seg<S>();
@@ -762,9 +840,9 @@ int rbk_dispatch_values(int vdtype, int op, const void* keys, const void* vals, 
           break;
         case 5: st = rbk_launch<K, V, OP, 5>(k, v, n, ko, vo, num_runs, ws, stream); break;
         // three-pass forms of shapes 0, 2 and 5
-        case 6: st = rbk_launch<K, V, OP, 0, true>(k, v, n, ko, vo, num_runs, ws, stream); break;
-        case 7: st = rbk_launch<K, V, OP, 2, true>(k, v, n, ko, vo, num_runs, ws, stream); break;
-        case 8: st = rbk_launch<K, V, OP, 5, true>(k, v, n, ko, vo, num_runs, ws, stream); break;
+        case 6: st = rbk_launch<K, V, OP, 0, kRbkThreePass>(k, v, n, ko, vo, num_runs, ws, stream); break;
+        case 7: st = rbk_launch<K, V, OP, 2, kRbkThreePass>(k, v, n, ko, vo, num_runs, ws, stream); break;
+        case 8: st = rbk_launch<K, V, OP, 5, kRbkThreePass>(k, v, n, ko, vo, num_runs, ws, stream); break;
         default: st = rbk_launch<K, V, OP, 0>(k, v, n, ko, vo, num_runs, ws, stream); break;
       }
     } else {
