@@ -52,6 +52,20 @@ def _log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def _cpu_model():
+    """The host CPU's model name and the cores this process may run on."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": name, "visible_cores": len(os.sched_getaffinity(0)), "logical_cpus": os.cpu_count()}
+
+
 def _pct(xs, q):
     import numpy as np
     return float(np.percentile(np.asarray(xs, dtype=np.float64), q)) if len(xs) else None
@@ -109,20 +123,13 @@ def main():
     njobs = len(jobs)
     _log(f"generated {len(jobs)} jobs, {total} elements in {time.perf_counter() - t0:.1f}s")
 
-    # device copy bandwidth (read 1 GiB + write 1 GiB)
-    src = torch.empty(1 << 28, dtype=torch.int32, device=dev).fill_(1)
-    dst = torch.empty_like(src)
-    for _ in range(3):
-        dst.copy_(src)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(10):
-        dst.copy_(src)
-    e1.record()
-    torch.cuda.synchronize()
-    copy_gbs = 2 * src.numel() * 4 * 10 / (e0.elapsed_time(e1) * 1e-3) / 1e9
-    del src, dst
-    _log(f"device copy {copy_gbs:.0f} GB/s")
+    # HBM ceilings: the hand-written nontemporal copy and read kernels of
+    # bulk_amd/hbm_ceiling.hip on 1 GiB streams (run as a separate process)
+    from bulk_amd.opbench import calibration_copy, hbm_ceiling
+    ceiling = hbm_ceiling()
+    calibration_copy(dev)  # the traffic counters' scale under rocprofv3 --pmc
+    copy_gbs = ceiling["copy_nt_gbs"]
+    _log(f"device copy {copy_gbs:.0f} GB/s, read {ceiling['read_nt_gbs']:.0f} GB/s")
 
     # ----------------------------------------------------- per-call baseline
     def timed(fn, steps):
@@ -297,6 +304,15 @@ def main():
         arr = W.poisson_arrivals(len(jobs), rate, seed=args.seed + 1)
         pois = serve_run(0, arr)
         pois_pc = serve_run(SERVE_PER_CALL, arr)
+        # latency against offered load: Poisson arrivals at fractions of the
+        # saturated rate
+        sweep = []
+        for load in (0.25, 0.5, 0.75, 0.9, 1.0):
+            r = load * len(jobs) / (sat["span_ms"] * 1e-3)
+            res = serve_run(0, W.poisson_arrivals(len(jobs), r, seed=args.seed + 2))
+            sweep.append({"load": load, "rate_jobs_per_s": r, "p50_ms": res["p50_ms"],
+                          "p99_ms": res["p99_ms"], "batches": res["batches"]})
+            _log(f"poisson load {load:.2f}: p50 {res['p50_ms']:.3f} ms p99 {res['p99_ms']:.3f} ms")
         sat_pc = serve_run(SERVE_PER_CALL, zero)
         # Host-staged requests (data starts and ends in pinned host memory):
         # a sample of the stream served at once, PCIe copies included.
@@ -341,6 +357,7 @@ def main():
             "poisson_rate_jobs_per_s": rate,
             "poisson": pois,
             "poisson_per_call": pois_pc,
+            "poisson_sweep": sweep,
         }
         _log(f"served saturated {sat_melem:.0f} Melem/s p99 {sat['p99_ms']:.2f} ms; "
              f"poisson p50 {pois['p50_ms']:.3f} p99 {pois['p99_ms']:.3f} ms; "
@@ -396,6 +413,8 @@ def main():
             else:  # sorts in place: sort a copy so the inputs stay unsorted
                 O.stable_sort_by_key(h["out0"].copy(), h["out1"].copy())
 
+        # 16 threads: the host cores a single-GPU share of the machine gets
+        # (one GPU of eight on a node); fewer when fewer are visible
         threads = min(16, len(os.sched_getaffinity(0)))
         with ThreadPoolExecutor(threads) as ex:
             a = time.perf_counter()
@@ -425,9 +444,22 @@ def main():
         t1 = float(np.median(ts[1:]))
         cpu["scan_int32_2^24"] = {"ms": t1 * 1e3, "melem_per_s": (1 << 24) / t1 / 1e6, "cores": 1,
                                   "gpu_equals_cpu": bool(np.array_equal(gpu1, ref1))}
-        del x1, gpu1, h1, ref1
-        _log(f"cpu int32 scan 2^24: {t1 * 1e3:.2f} ms, GPU result equal: "
-             f"{cpu['scan_int32_2^24']['gpu_equals_cpu']}")
+        # The same scan on `threads` threads through the reference's own
+        # three-phase decomposition (scan.cu:109-145: per-partition sums,
+        # a scan of the carries, per-partition scans seeded by them).
+        ts = []
+        for _ in range(6):
+            a = time.perf_counter()
+            mt1 = O.mt_inclusive_scan_i32(h1, 0, threads)
+            ts.append(time.perf_counter() - a)
+        tm = float(np.median(ts[1:]))
+        cpu["scan_int32_2^24_threads"] = {"ms": tm * 1e3, "melem_per_s": (1 << 24) / tm / 1e6,
+                                          "cores": threads,
+                                          "equals_one_thread": bool(np.array_equal(mt1, ref1))}
+        cpu["host_cpu"] = _cpu_model()
+        del x1, gpu1, h1, ref1, mt1
+        _log(f"cpu int32 scan 2^24: {t1 * 1e3:.2f} ms on 1 thread, {tm * 1e3:.2f} ms on {threads}; "
+             f"GPU result equal: {cpu['scan_int32_2^24']['gpu_equals_cpu']}")
 
     # ------------------------------------------------- single ops at full size
     ops = None
@@ -473,6 +505,7 @@ def main():
         "p99_ms": serve["poisson"]["p99_ms"] if serve else None,
         "p50_ms": serve["poisson"]["p50_ms"] if serve else None,
         "copy_gbs": copy_gbs,
+        "hbm_ceiling": ceiling,
         "roofline": {"bound": "hbm", "kernel": f"{kernel_of[dom]} ({dom})",
                      "achieved": dom_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": dom_gbs / HBM_PEAK_GBS, "frac_of_copy": dom_gbs / copy_gbs,

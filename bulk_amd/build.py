@@ -65,6 +65,23 @@ def build_library(verbose: bool = True) -> str:
     return LIB
 
 
+CEILING_SRC = os.path.join(ROOT, "bulk_amd", "hbm_ceiling.hip")
+CEILING = os.path.join(LIBDIR, "hbm_ceiling")
+
+
+def build_ceiling(verbose: bool = True) -> str:
+    """Compile the HBM ceiling program (read / copy / read-two-write-an-eighth
+    kernels) that the benchmarks quote their roofline fractions against."""
+    os.makedirs(LIBDIR, exist_ok=True)
+    cmd = [HIPCC, *[c for c in COMMON if c != "-fPIC"], CEILING_SRC, "-o", CEILING]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"[bulk_amd.build] built {CEILING}", flush=True)
+    return CEILING
+
+
 CPP_TESTS = os.path.join(ROOT, "tests", "cpp")
 CPP_BIN = os.path.join(CPP_TESTS, "_bin")
 
@@ -91,5 +108,6 @@ def build_cpp_tests(verbose: bool = True) -> list[str]:
 
 if __name__ == "__main__":
     build_library()
+    build_ceiling()
     build_cpp_tests()
     sys.exit(0)

@@ -30,6 +30,33 @@ from . import api
 HBM_PEAK_GBS = 8000.0
 
 
+def hbm_ceiling() -> dict:
+    """Run the HBM ceiling program (bulk_amd/lib/hbm_ceiling, built from
+    bulk_amd/hbm_ceiling.hip) and return its JSON line: the GB/s of the best
+    hand-written nontemporal read, copy and read-two-write-an-eighth kernels
+    on 1 GiB streams, the practical ceilings of the kernels measured here."""
+    import json
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "hbm_ceiling")
+    if not os.path.exists(exe):
+        raise RuntimeError(f"{exe} is not built (python -m bulk_amd.build)")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120, check=True)
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def calibration_copy(dev, reps: int = 3) -> None:
+    """Copies of 1 GiB (2^28 int32) in this process: a kernel of known
+    traffic (1 GiB read, 1 GiB written) that scales the FETCH_SIZE and
+    WRITE_SIZE counters of a rocprofv3 --pmc run of it."""
+    src = torch.ones(1 << 28, dtype=torch.int32, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
 def _timeit(fn, iters=10, warm=2, stream=None):
     for _ in range(warm):
         fn()
@@ -172,6 +199,11 @@ def run(iters: int = 10, log=print, large: int = 28) -> list[dict]:
     dev = torch.device("cuda")
     g = torch.Generator(device=dev)
     rows = []
+    ceil = hbm_ceiling()
+    calibration_copy(dev)
+    log(f"HBM ceilings: read {ceil['read_nt_gbs']:.0f} GB/s, copy {ceil['copy_nt_gbs']:.0f} GB/s "
+        f"(plain stores {ceil['copy_plain_gbs']:.0f}), read two + write an eighth "
+        f"{ceil['read2_write8_gbs']:.0f} GB/s")
 
     # scan int32 2^24
     g.manual_seed(0)
@@ -316,8 +348,12 @@ def run(iters: int = 10, log=print, large: int = 28) -> list[dict]:
         del keys
         torch.cuda.empty_cache()
     for r in rows:
+        r["frac_of_copy_ceiling"] = r["gbs"] / ceil["copy_nt_gbs"]
+    rows.insert(0, {"op": "hbm_ceiling", **ceil})
+    for r in rows[1:]:
         b2b = f"  (back to back {r['ms_back_to_back'] * 1e3:.1f} us)" if "ms_back_to_back" in r else ""
-        log(f"{r['op']:46s} {r['ms'] * 1e3:9.1f} us {r['gbs']:7.0f} GB/s {100 * r['frac_peak']:5.1f}%{b2b}")
+        log(f"{r['op']:46s} {r['ms'] * 1e3:9.1f} us {r['gbs']:7.0f} GB/s {100 * r['frac_peak']:5.1f}% "
+            f"({100 * r['frac_of_copy_ceiling']:5.1f}% of copy){b2b}")
     return rows
 
 

@@ -11,8 +11,9 @@ sources.
   python -m bulk_amd.rocprof_summary traffic <fetch_dir> <write_dir> --jobs J --seed S
       HBM bytes per dispatch from a FETCH_SIZE run and a WRITE_SIZE run
       (*_counter_collection.csv).  The counters are scaled by what they read
-      for the 1 GiB device copy the benchmark makes (1 GiB read + 1 GiB
-      written): factor = known bytes / counter bytes.
+      for the 1 GiB device copies the profiled process makes
+      (bulk_amd.opbench.calibration_copy: 1 GiB read + 1 GiB written):
+      factor = known bytes / counter bytes.
 """
 from __future__ import annotations
 
@@ -24,7 +25,7 @@ import os
 import re
 import sys
 
-COPY_BYTES = 1 << 30  # the benchmark's calibration copy: 2^28 int32
+COPY_BYTES = 1 << 30  # the calibration copy: 2^28 int32
 
 
 def _short(name: str, width: int = 96) -> str:
