@@ -21,6 +21,8 @@ LESS, GREATER = range(2)
 EQUAL_TO = 0
 (ALGO_INCLUSIVE_SCAN, ALGO_EXCLUSIVE_SCAN, ALGO_REDUCE, ALGO_MERGE, ALGO_MERGE_BY_KEY,
  ALGO_SORT_BY_KEY, ALGO_REDUCE_BY_KEY, ALGO_REDUCE_INTERVALS) = range(8)
+# tuning-table keys of the batch executor's op families
+ALGO_BATCH_SCAN, ALGO_BATCH_REDUCE_BY_KEY, ALGO_BATCH_MERGE = 8, 9, 10
 (JOB_INCLUSIVE_SCAN, JOB_REDUCE_BY_KEY, JOB_MERGE_BY_KEY, JOB_SORT_BY_KEY, JOB_REDUCE) = range(5)
 
 STATUS_NAMES = {
@@ -99,6 +101,7 @@ SIGNATURES = {
     "bulkx_batch_last_timing": (_I, [_P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
     "bulkx_batch_stage_timing": (_I, [_P, ctypes.POINTER(ctypes.c_float), _I]),
     "bulkx_batch_sort_geometry": (_I, [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),
+    "bulkx_batch_family_shapes": (_I, [ctypes.c_void_p, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_int64)]),
     "bulkx_batch_prepare_graphs": (_I, [_P]),
     "bulkx_server_create": (_I, [_L, _I, _I, _L, _I, ctypes.POINTER(_P)]),
     "bulkx_server_destroy": (_I, [_P]),

@@ -8,7 +8,9 @@ For each (algorithm, dtype, size class) the representative input (2^14,
 2^18, 2^22, 2^24, 2^26 elements) is timed with HIP events under every variant
 (median of `--iters` calls, workspace preallocated); the fastest variant is
 stored.  Exclusive scan shares the inclusive scan's choice and merge (keys
-only) the merge_by_key choice.
+only) the merge_by_key choice.  The batch executor's op families are tuned
+on the benchmark's mixed-job stream (tune_batch): each family keeps the tile
+shape under which its stage ran fastest.
 """
 from __future__ import annotations
 
@@ -147,14 +149,80 @@ def tune(iters: int = 10, log=print) -> dict:
             "command": "python -m bulk_amd.autotune", "iters": iters, "entries": entries}
 
 
+# The batch executor's families: stage name -> (tuning algo, dtype key).
+BATCH_FAMILIES = {
+    "scan_i32": (C.ALGO_BATCH_SCAN, C.INT32),
+    "scan_f32": (C.ALGO_BATCH_SCAN, C.FLOAT32),
+    "reduce_by_key_i32": (C.ALGO_BATCH_REDUCE_BY_KEY, C.INT32),
+    "reduce_by_key_f32": (C.ALGO_BATCH_REDUCE_BY_KEY, C.FLOAT32),
+    "reduce_by_key_i64": (C.ALGO_BATCH_REDUCE_BY_KEY, C.INT64),
+    "merge_by_key": (C.ALGO_BATCH_MERGE, C.INT32),
+}
+
+
+def tune_batch(iters: int = 5, jobs: int = 4096, seed: int = 5, log=print) -> list[dict]:
+    """The batch executor's family tile shapes, timed on the mixed-job
+    stream the benchmark runs (`jobs` jobs, `seed`): every family's stage
+    time (HIP events around its kernels in an eager launch, median of
+    `iters`) under each shape; a family keeps its fastest shape."""
+    from . import workload as W
+    from .batch import Batch
+    specs = W.make_specs(jobs, seed=seed)
+    data = W.build_jobs(specs, seed=seed + 2)
+    total = W.total_elements(data)
+    nshapes = _variants(C.ALGO_BATCH_SCAN)
+    times = {f: [] for f in BATCH_FAMILIES}
+    for sh in range(nshapes):
+        for algo, dt in BATCH_FAMILIES.values():
+            _set(algo, dt, 0, sh)
+        b = Batch(max_jobs=len(data), max_elements=total)
+        b.reserve(total)
+        b.set_jobs([j.job for j in data])
+        runs = {f: [] for f in BATCH_FAMILIES}
+        for _ in range(iters + 1):
+            W.reset_all(data)
+            torch.cuda.synchronize()
+            b.launch(use_graph=False)
+            st = b.stage_timing()
+            for f in BATCH_FAMILIES:
+                runs[f].append(st[f])
+        assert b.status() == 0
+        b.close()
+        for f in BATCH_FAMILIES:
+            r = sorted(runs[f][1:])
+            times[f].append(r[len(r) // 2])
+        log(f"batch shape {sh}: " + " ".join(f"{f} {times[f][-1]:.3f}ms" for f in BATCH_FAMILIES))
+    entries = []
+    for f, (algo, dt) in BATCH_FAMILIES.items():
+        best = min(range(nshapes), key=lambda i: times[f][i])
+        _set(algo, dt, 0, best)
+        entries.append({"algo": algo, "dtype": dt, "size_class": 0, "family": f, "jobs": jobs,
+                        "seed": seed, "variant": best, "ms": [round(t, 5) for t in times[f]]})
+        log(f"batch family {f}: {' '.join(f'{t:.3f}ms' for t in times[f])} -> shape {best}")
+    del data
+    torch.cuda.empty_cache()
+    return entries
+
+
 def main(argv=None):
     p = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     p.add_argument("--out", default=DEFAULT_OUT)
     p.add_argument("--iters", type=int, default=10)
+    p.add_argument("--no-batch", action="store_true", help="skip the batch executor's families")
+    p.add_argument("--batch-only", action="store_true",
+                   help="tune only the batch executor's families; keep the other entries of --out")
     a = p.parse_args(argv)
     C.lib()
     C.check(C.lib().bulkx_load_tuning(b"none", None), "bulkx_load_tuning")  # start from defaults
-    table = tune(a.iters, log=lambda m: print(m, file=sys.stderr, flush=True))
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    if a.batch_only:
+        with open(a.out) as f:
+            table = json.load(f)
+        table["entries"] = [e for e in table["entries"] if e["algo"] < C.ALGO_BATCH_SCAN]
+    else:
+        table = tune(a.iters, log=log)
+    if not a.no_batch:
+        table["entries"] += tune_batch(log=log)
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
     with open(a.out, "w") as f:
         json.dump(table, f, indent=1)

@@ -103,6 +103,15 @@ class Batch:
                 "bulkx_batch_last_timing")
         return a.value, b.value
 
+    def family_shapes(self) -> dict:
+        """{family stage name: (shape, tile elements)} of the first seven
+        stages: the tile shape each op family of this batch runs (read from
+        the tuning table when the batch was created)."""
+        sh = (ctypes.c_int * 7)()
+        te = (ctypes.c_int64 * 7)()
+        C.check(C.lib().bulkx_batch_family_shapes(self._h, sh, te), "bulkx_batch_family_shapes")
+        return {name: (sh[i], te[i]) for i, name in enumerate(self.STAGES[:7])}
+
     STAGES = ("scan_i32", "scan_f32", "reduce_by_key_i32", "reduce_by_key_f32",
               "reduce_by_key_i64", "merge_by_key", "sort_block", "sort_radix",
               "sort_downsweep_kernels", "sort_upsweep_kernels", "sort_count_scan_kernels")

@@ -19,6 +19,10 @@
 // tiles together.  The graph forks into four concurrent branches (sort,
 // merge, reduce_by_key, scan) after one clear kernel.
 //
+// Each family is compiled in two tile shapes (batch_shapes.hpp); a batch
+// takes each family's shape from the tuning table when it is created, and
+// the autotuner times both on the mixed-job stream.
+//
 // Everything a launch needs (tile counts, tables, counters) lives in device
 // memory, so the captured graph stays valid when the job list changes: a new
 // list is uploaded with stream-ordered copies, the size-dependent grids are
@@ -26,34 +30,26 @@
 #include <algorithm>
 #include <vector>
 
+#include "batch_shapes.hpp"
 #include "host_common.hpp"
 #include "merge_impl.hpp"
 #include "reduce_by_key_impl.hpp"
 #include "scan_impl.hpp"
 #include "sort_impl.hpp"
+#include "tuning.hpp"
 
 namespace bulkx {
 
-constexpr int kBatchBlock = 512;
-constexpr int kBatchScanItems = 32;   // 16384 elements per scan / reduce tile
-constexpr int kBatchRbkItems = 16;    // 8192 elements per reduce_by_key tile
-constexpr int kBatchMergeItems = 8;   // 4096 outputs per merge / sort tile
-constexpr int64_t kBatchScanTile = int64_t(kBatchBlock) * kBatchScanItems;
-constexpr int64_t kBatchRbkTile = int64_t(kBatchBlock) * kBatchRbkItems;
-constexpr int64_t kBatchMergeTile = int64_t(kBatchBlock) * kBatchMergeItems;
+// Tile shapes: fam_block / fam_items (batch_shapes.hpp), two per family.
 // Sort jobs of at most one block-sort tile (512 x 16 = 8192 keys, in LDS)
 // are sorted by the block sort alone; larger ones by the radix passes
 // (radix_impl.hpp: four digit passes, each an upsweep of per-tile digit
 // counts, a scan of the job's counts and a downsweep that scatters keys and
 // values), which read and write a job 4 times however large it is.
-constexpr int kBatchSortBlock = 512;
-constexpr int kBatchSortItems = 16;
 constexpr int64_t kBatchSortRun = int64_t(kBatchSortBlock) * kBatchSortItems;
 using BatchRadix = RadixConfig<uint32_t>;
 constexpr int64_t kBatchRadixTile = BatchRadix::TILE;
-constexpr int64_t kBatchMinTile = kBatchScanTile < kBatchRbkTile
-                                      ? (kBatchScanTile < kBatchMergeTile ? kBatchScanTile : kBatchMergeTile)
-                                      : (kBatchRbkTile < kBatchMergeTile ? kBatchRbkTile : kBatchMergeTile);
+constexpr int64_t kBatchMinTile = batch_min_tile();
 constexpr int kStatusWordsPerTile = 6;  // widest look-back record used here (24 bytes)
 
 enum BatchOp : int32_t {
@@ -87,13 +83,10 @@ struct BatchJobDev {
   int64_t sort_tiles;       // radix tiles
 };
 
-// Phase-0 work is split by op family; each family is one kernel
-// compiled for its own register and LDS footprint (a single kernel over all
-// ops would run at the occupancy of the heaviest body).
-enum BatchFamily : int32_t {
-  FAM_SCAN_I32 = 0, FAM_SCAN_F32, FAM_RBK_I32, FAM_RBK_F32, FAM_RBK_I64, FAM_MERGE, FAM_SORT,
-  FAM_COUNT
-};
+// Phase-0 work is split by op family (BatchFamily, batch_shapes.hpp); each
+// family is one kernel compiled for its own register and LDS footprint (a
+// single kernel over all ops would run at the occupancy of the heaviest
+// body), in each of its tile shapes.
 constexpr int kBatchStages = FAM_COUNT + 1;  // the family kernels, then the sort radix passes
 // Launch-grid kinds: one per phase-0 family, the radix passes' upsweep, scan
 // and downsweep, the aggregate scans of the three reduce_by_key families,
@@ -119,38 +112,39 @@ struct BatchDesc {
   LookbackGuard guard;  // the look-back waits of every family; err cleared per launch
 };
 
-template <int FAM>
+template <int FAM, int SH>
 struct FamilySmem;
-template <>
-struct FamilySmem<FAM_SCAN_I32> {
-  ScanSmem<int32_t, OpPlus, kBatchBlock> s;
+template <int SH>
+struct FamilySmem<FAM_SCAN_I32, SH> {
+  ScanSmem<int32_t, OpPlus, fam_block(FAM_SCAN_I32, SH)> s;
 };
-template <>
-struct FamilySmem<FAM_SCAN_F32> {
-  ScanSmem<float, OpPlus, kBatchBlock> s;
+template <int SH>
+struct FamilySmem<FAM_SCAN_F32, SH> {
+  ScanSmem<float, OpPlus, fam_block(FAM_SCAN_F32, SH)> s;
 };
-template <>
-struct FamilySmem<FAM_RBK_I32> {
-  RbkSmem<int32_t, int32_t, OpPlus, kBatchBlock, kBatchRbkItems> s;
+template <int SH>
+struct FamilySmem<FAM_RBK_I32, SH> {
+  RbkSmem<int32_t, int32_t, OpPlus, fam_block(FAM_RBK_I32, SH), fam_items(FAM_RBK_I32, SH)> s;
 };
-template <>
-struct FamilySmem<FAM_RBK_F32> {
-  RbkSmem<int32_t, float, OpPlus, kBatchBlock, kBatchRbkItems> s;
+template <int SH>
+struct FamilySmem<FAM_RBK_F32, SH> {
+  RbkSmem<int32_t, float, OpPlus, fam_block(FAM_RBK_F32, SH), fam_items(FAM_RBK_F32, SH)> s;
 };
-template <>
-struct FamilySmem<FAM_RBK_I64> {
-  RbkSmem<int32_t, int64_t, OpPlus, kBatchBlock, kBatchRbkItems> s;
+template <int SH>
+struct FamilySmem<FAM_RBK_I64, SH> {
+  RbkSmem<int32_t, int64_t, OpPlus, fam_block(FAM_RBK_I64, SH), fam_items(FAM_RBK_I64, SH)> s;
 };
-template <>
-struct FamilySmem<FAM_MERGE> {
+template <int SH>
+struct FamilySmem<FAM_MERGE, SH> {
+  static constexpr int B = fam_block(FAM_MERGE, SH), I = fam_items(FAM_MERGE, SH);
   union {
-    MergeSmem<int32_t, uint32_t, true, kBatchBlock, kBatchMergeItems> i;
-    MergeSmem<float, uint32_t, true, kBatchBlock, kBatchMergeItems> f;
-    MergeSmem<uint32_t, uint32_t, true, kBatchBlock, kBatchMergeItems> u;
+    MergeSmem<int32_t, uint32_t, true, B, I> i;
+    MergeSmem<float, uint32_t, true, B, I> f;
+    MergeSmem<uint32_t, uint32_t, true, B, I> u;
   };
 };
-template <>
-struct FamilySmem<FAM_SORT> {
+template <int SH>
+struct FamilySmem<FAM_SORT, SH> {
   union {
     SortSmem<int32_t, uint32_t, true, kBatchSortBlock, kBatchSortItems> i;
     SortSmem<uint32_t, uint32_t, true, kBatchSortBlock, kBatchSortItems> u;
@@ -192,47 +186,48 @@ using RbkValue = typename std::conditional<
 template <int FAM>
 using RbkAgg = Seg<typename Acc<RbkValue<FAM>, OpPlus>::type>;
 
-// One phase-0 tile of a job of family FAM.
-template <int FAM>
-__device__ __forceinline__ void run_tile(FamilySmem<FAM>& sm, BatchDesc* d, const BatchJobDev& job,
-                                         int64_t lt) {
+// One phase-0 tile of a job of family FAM in shape SH.
+template <int FAM, int SH>
+__device__ __forceinline__ void run_tile(FamilySmem<FAM, SH>& sm, BatchDesc* d,
+                                         const BatchJobDev& job, int64_t lt) {
+  constexpr int B = fam_block(FAM, SH), I = fam_items(FAM, SH);
   const bool single = job.tiles0 == 1;
   if constexpr (FAM == FAM_SCAN_I32) {
     if (job.op == OP_SCAN_I32)
-      scan_tile<int32_t, OpPlus, kBatchBlock, kBatchScanItems, false>(
+      scan_tile<int32_t, OpPlus, B, I, false>(
           sm.s, static_cast<const int32_t*>(job.in0), static_cast<int32_t*>(job.out0), job.n0, 0,
           false, lt, job.status, &d->guard, single, nullptr);
     else
-      scan_tile<int32_t, OpPlus, kBatchBlock, kBatchScanItems, false, false>(
+      scan_tile<int32_t, OpPlus, B, I, false, false>(
           sm.s, static_cast<const int32_t*>(job.in0), nullptr, job.n0, 0, false, lt, job.status,
           &d->guard, single, static_cast<int32_t*>(job.out0));
   } else if constexpr (FAM == FAM_SCAN_F32) {
     if (job.op == OP_SCAN_F32)
-      scan_tile<float, OpPlus, kBatchBlock, kBatchScanItems, false>(
+      scan_tile<float, OpPlus, B, I, false>(
           sm.s, static_cast<const float*>(job.in0), static_cast<float*>(job.out0), job.n0, 0.f,
           false, lt, job.status, &d->guard, single, nullptr);
     else
-      scan_tile<float, OpPlus, kBatchBlock, kBatchScanItems, false, false>(
+      scan_tile<float, OpPlus, B, I, false, false>(
           sm.s, static_cast<const float*>(job.in0), nullptr, job.n0, 0.f, false, lt, job.status,
           &d->guard, single, static_cast<float*>(job.out0));
   } else if constexpr (FAM == FAM_RBK_I32 || FAM == FAM_RBK_F32 || FAM == FAM_RBK_I64) {
     // the third pass: the tile's exclusive prefix was scanned by
     // batch_rbk_scan_kernel into the job's status words
     using V = RbkValue<FAM>;
-    rbk_tile<int32_t, V, OpPlus, kBatchBlock, kBatchRbkItems>(
+    rbk_tile<int32_t, V, OpPlus, B, I>(
         sm.s, static_cast<const int32_t*>(job.in0), static_cast<const V*>(job.in1), job.n0,
         static_cast<int32_t*>(job.out0), static_cast<V*>(job.out1),
         static_cast<int64_t*>(job.out2), lt, nullptr, &d->guard, single,
         single ? nullptr : reinterpret_cast<const RbkAgg<FAM>*>(job.status));
   } else if constexpr (FAM == FAM_MERGE) {
     if (job.op == OP_MERGE_I32)
-      merge_two_tile<CmpLess, int32_t, uint32_t, true, kBatchBlock, kBatchMergeItems>(
+      merge_two_tile<CmpLess, int32_t, uint32_t, true, B, I>(
           sm.i, static_cast<const int32_t*>(job.in0), static_cast<const uint32_t*>(job.in1),
           job.n0, static_cast<const int32_t*>(job.in2), static_cast<const uint32_t*>(job.in3),
           job.n1, static_cast<int32_t*>(job.out0), static_cast<uint32_t*>(job.out1), lt,
           job.splits);
     else
-      merge_two_tile<CmpLess, float, uint32_t, true, kBatchBlock, kBatchMergeItems>(
+      merge_two_tile<CmpLess, float, uint32_t, true, B, I>(
           sm.f, static_cast<const float*>(job.in0), static_cast<const uint32_t*>(job.in1),
           job.n0, static_cast<const float*>(job.in2), static_cast<const uint32_t*>(job.in3),
           job.n1, static_cast<float*>(job.out0), static_cast<uint32_t*>(job.out1), lt,
@@ -258,14 +253,11 @@ __device__ __forceinline__ void run_tile(FamilySmem<FAM>& sm, BatchDesc* d, cons
 // tiles are independent and use the workgroup index directly.  (A
 // persistent loop here would keep the tile body's per-lane address math live
 // across iterations and cost ~40 registers per lane.)
-// Threads per workgroup of a family's phase-0 kernel.
-constexpr int fam_block(int fam) { return fam == FAM_SORT ? kBatchSortBlock : kBatchBlock; }
-
-template <int FAM>
-__global__ __launch_bounds__(fam_block(FAM)) void batch_phase0_kernel(
+template <int FAM, int SH>
+__global__ __launch_bounds__(fam_block(FAM, SH)) void batch_phase0_kernel(
     BatchDesc* d, const BatchJobDev* __restrict__ jobs, const uint32_t* __restrict__ tile_job,
     uint32_t* __restrict__ job_ctr) {
-  __shared__ FamilySmem<FAM> sm;
+  __shared__ FamilySmem<FAM, SH> sm;
   const int64_t begin = d->fam_begin[FAM];
   const int64_t count = d->fam_begin[FAM + 1] - begin;
   const int64_t g = int64_t(blockIdx.x);
@@ -282,12 +274,13 @@ __global__ __launch_bounds__(fam_block(FAM)) void batch_phase0_kernel(
     __syncthreads();
     lt = wave_uniform(s_t);
   }
-  run_tile<FAM>(sm, d, job, lt);
+  run_tile<FAM, SH>(sm, d, job, lt);
 }
 
 // The merge path of every merge tile's first diagonal, one lane per tile
 // (as merge_split_kernel); the last tile of a job also writes the end
 // boundary (all of A).  Searched once here instead of by each tile.
+template <int SH>
 __global__ __launch_bounds__(256) void batch_merge_partition_kernel(
     BatchDesc* d, const BatchJobDev* __restrict__ jobs, const uint32_t* __restrict__ tile_job) {
   const int64_t begin = d->fam_begin[FAM_MERGE];
@@ -296,7 +289,8 @@ __global__ __launch_bounds__(256) void batch_merge_partition_kernel(
   const BatchJobDev& job = jobs[tile_job[begin + g]];
   const int64_t lt = begin + g - job.tile_begin0;
   const int64_t n = job.n0 + job.n1;
-  const int64_t diag = lt * kBatchMergeTile < n ? lt * kBatchMergeTile : n;
+  constexpr int64_t T = fam_tile(FAM_MERGE, SH);
+  const int64_t diag = lt * T < n ? lt * T : n;
   int64_t r;
   if (job.op == OP_MERGE_I32)
     r = lane_merge_path<CmpLess>(static_cast<const int32_t*>(job.in0), job.n0,
@@ -314,8 +308,8 @@ __global__ __launch_bounds__(256) void batch_merge_partition_kernel(
 // job's aggregates in place (one workgroup per job), pass 3 is the family's
 // phase-0 kernel with the tile's prefix read, not looked back.  No tile ever
 // waits on another, and the tiles need no claim counter.
-template <int FAM>
-__global__ __launch_bounds__(kBatchBlock) void batch_rbk_upsweep_kernel(
+template <int FAM, int SH>
+__global__ __launch_bounds__(fam_block(FAM, SH)) void batch_rbk_upsweep_kernel(
     BatchDesc* d, const BatchJobDev* __restrict__ jobs, const uint32_t* __restrict__ tile_job) {
   const int64_t begin = d->fam_begin[FAM];
   const int64_t g = int64_t(blockIdx.x);
@@ -323,7 +317,7 @@ __global__ __launch_bounds__(kBatchBlock) void batch_rbk_upsweep_kernel(
   const BatchJobDev& job = jobs[tile_job[begin + g]];
   if (job.tiles0 <= 1) return;
   using V = RbkValue<FAM>;
-  rbk_upsweep_tile<int32_t, V, OpPlus, kBatchBlock, kBatchRbkItems>(
+  rbk_upsweep_tile<int32_t, V, OpPlus, fam_block(FAM, SH), fam_items(FAM, SH)>(
       static_cast<const int32_t*>(job.in0), static_cast<const V*>(job.in1), job.n0,
       reinterpret_cast<RbkAgg<FAM>*>(job.status), begin + g - job.tile_begin0);
 }
@@ -529,6 +523,7 @@ struct bulkx_batch {
   int64_t max_elements = 0;
   int64_t max_tiles = 0;
   int64_t fam_tiles[FAM_COUNT] = {};  // phase-0 grid of each family = its tile count
+  int shape[FAM_COUNT] = {};  // each family's tile shape (tuning table, at create)
   // device
   BatchDesc* d_desc = nullptr;
   BatchJobDev* d_jobs = nullptr;
@@ -591,40 +586,48 @@ static unsigned batch_grid(const bulkx_batch* b, int kind) {
 }
 
 using Phase0Kernel = void (*)(BatchDesc*, const BatchJobDev*, const uint32_t*, uint32_t*);
-static const Phase0Kernel kPhase0[FAM_COUNT] = {
-    batch_phase0_kernel<FAM_SCAN_I32>, batch_phase0_kernel<FAM_SCAN_F32>,
-    batch_phase0_kernel<FAM_RBK_I32>,  batch_phase0_kernel<FAM_RBK_F32>,
-    batch_phase0_kernel<FAM_RBK_I64>,  batch_phase0_kernel<FAM_MERGE>,
-    batch_phase0_kernel<FAM_SORT>};
+#define BULKX_PHASE0(SH)                                                              \
+  {batch_phase0_kernel<FAM_SCAN_I32, SH>, batch_phase0_kernel<FAM_SCAN_F32, SH>,      \
+   batch_phase0_kernel<FAM_RBK_I32, SH>,  batch_phase0_kernel<FAM_RBK_F32, SH>,       \
+   batch_phase0_kernel<FAM_RBK_I64, SH>,  batch_phase0_kernel<FAM_MERGE, SH>,         \
+   batch_phase0_kernel<FAM_SORT, 0>}  /* one shape */
+static const Phase0Kernel kPhase0[kBatchShapes][FAM_COUNT] = {BULKX_PHASE0(0), BULKX_PHASE0(1)};
+#undef BULKX_PHASE0
 
 using RbkUpKernel = void (*)(BatchDesc*, const BatchJobDev*, const uint32_t*);
-static const RbkUpKernel kRbkUp[kRbkFamilies] = {batch_rbk_upsweep_kernel<FAM_RBK_I32>,
-                                                 batch_rbk_upsweep_kernel<FAM_RBK_F32>,
-                                                 batch_rbk_upsweep_kernel<FAM_RBK_I64>};
+static const RbkUpKernel kRbkUp[kBatchShapes][kRbkFamilies] = {
+    {batch_rbk_upsweep_kernel<FAM_RBK_I32, 0>, batch_rbk_upsweep_kernel<FAM_RBK_F32, 0>,
+     batch_rbk_upsweep_kernel<FAM_RBK_I64, 0>},
+    {batch_rbk_upsweep_kernel<FAM_RBK_I32, 1>, batch_rbk_upsweep_kernel<FAM_RBK_F32, 1>,
+     batch_rbk_upsweep_kernel<FAM_RBK_I64, 1>}};
+using PartitionKernel = void (*)(BatchDesc*, const BatchJobDev*, const uint32_t*);
+static const PartitionKernel kMergePartition[kBatchShapes] = {batch_merge_partition_kernel<0>,
+                                                              batch_merge_partition_kernel<1>};
 static const RbkUpKernel kRbkScan[kRbkFamilies] = {batch_rbk_scan_kernel<FAM_RBK_I32>,
                                                    batch_rbk_scan_kernel<FAM_RBK_F32>,
                                                    batch_rbk_scan_kernel<FAM_RBK_I64>};
 
-static bool is_rbk_family(int f) { return f == FAM_RBK_I32 || f == FAM_RBK_F32 || f == FAM_RBK_I64; }
 
 // The passes of family f on stream q: for reduce_by_key the aggregates and
 // their per-job scans come first.
 static int batch_enqueue_family(bulkx_batch* b, int f, hipStream_t q) {
+  const int sh = b->shape[f];
   if (f == FAM_MERGE) {
-    batch_merge_partition_kernel<<<batch_grid(b, kGridMergePartition), 256, 0, q>>>(
-        b->d_desc, b->d_jobs, b->d_tile_job);
+    kMergePartition[sh]<<<batch_grid(b, kGridMergePartition), 256, 0, q>>>(b->d_desc, b->d_jobs,
+                                                                          b->d_tile_job);
     BULKX_LAUNCHED();
   }
-  if (is_rbk_family(f)) {
+  if (fam_is_rbk(f)) {
     const int r = f - FAM_RBK_I32;
-    kRbkUp[r]<<<batch_grid(b, f), kBatchBlock, 0, q>>>(b->d_desc, b->d_jobs, b->d_tile_job);
+    kRbkUp[sh][r]<<<batch_grid(b, f), fam_block(f, sh), 0, q>>>(b->d_desc, b->d_jobs,
+                                                                b->d_tile_job);
     BULKX_LAUNCHED();
     kRbkScan[r]<<<batch_grid(b, grid_kind_rbk_scan(f)), 256, 0, q>>>(b->d_desc, b->d_jobs,
                                                                      b->d_rbk_jobs);
     BULKX_LAUNCHED();
   }
-  kPhase0[f]<<<batch_grid(b, f), fam_block(f), 0, q>>>(b->d_desc, b->d_jobs, b->d_tile_job,
-                                                       b->d_job_ctr);
+  kPhase0[sh][f]<<<batch_grid(b, f), fam_block(f, sh), 0, q>>>(b->d_desc, b->d_jobs,
+                                                               b->d_tile_job, b->d_job_ctr);
   BULKX_LAUNCHED();
   return BULKX_SUCCESS;
 }
@@ -700,6 +703,11 @@ extern "C" int bulkx_batch_create(int max_jobs, int64_t max_elements, void* stre
   b->stream = static_cast<hipStream_t>(stream);
   b->max_jobs = max_jobs;
   b->max_elements = max_elements;
+  for (int f = 0; f < FAM_COUNT; ++f) {
+    const int algo = fam_tuning_algo(f);
+    const int v = algo < 0 ? 0 : tuned_variant(algo, fam_tuning_dtype(f), 0);
+    b->shape[f] = v >= 0 && v < kBatchShapes ? v : 0;
+  }
   // every job can add one partial tile
   b->max_tiles = max_elements / kBatchMinTile + 2 * int64_t(max_jobs) + 1;
   // radix positions are 32-bit per job; a batch never holds a larger job
@@ -868,18 +876,17 @@ extern "C" int bulkx_batch_set_jobs(bulkx_batch* b, const bulkx_job* jobs, int n
     elements += n;
     ws_need = std::max(ws_need, per_call_workspace(j));
     if (n == 0 && j.kind != BULKX_JOB_REDUCE && j.kind != BULKX_JOB_REDUCE_BY_KEY) continue;
+    const int fam = host_family(op);
+    const int64_t T = fam_tile(fam, b->shape[fam]);
     int64_t t;
     switch (op) {
       case OP_SCAN_I32: case OP_SCAN_F32: case OP_REDUCE_I32: case OP_REDUCE_F32:
-        t = (n + kBatchScanTile - 1) / kBatchScanTile;
-        if (t > 1) status_words += size_t(t) * kStatusWordsPerTile;
-        break;
       case OP_RBK_I32_I32: case OP_RBK_I32_F32: case OP_RBK_I32_I64:
-        t = (n + kBatchRbkTile - 1) / kBatchRbkTile;
+        t = (n + T - 1) / T;
         if (t > 1) status_words += size_t(t) * kStatusWordsPerTile;
         break;
       default:
-        t = (n + kBatchMergeTile - 1) / kBatchMergeTile;
+        t = (n + T - 1) / T;
         if (op >= OP_SORT_I32) {
           t = 1;  // one block-sorted tile, or the radix passes (no phase-0 tile)
           if (n > kBatchSortRun) {
@@ -904,12 +911,11 @@ extern "C" int bulkx_batch_set_jobs(bulkx_batch* b, const bulkx_job* jobs, int n
   b->jobs.assign(jobs, jobs + num_jobs);
   b->num_jobs = num_jobs;
   // Phase-0 tiles are grouped by family, jobs in list order within a family.
-  auto job_tiles = [](int op, int64_t n) {
-    int64_t t;
-    if (op <= OP_REDUCE_F32) t = (n + kBatchScanTile - 1) / kBatchScanTile;
-    else if (op <= OP_RBK_I32_I64) t = (n + kBatchRbkTile - 1) / kBatchRbkTile;
-    else if (op >= OP_SORT_I32) return n <= kBatchSortRun ? int64_t(1) : int64_t(0);
-    else t = (n + kBatchMergeTile - 1) / kBatchMergeTile;
+  auto job_tiles = [b](int op, int64_t n) {
+    if (op >= OP_SORT_I32) return n <= kBatchSortRun ? int64_t(1) : int64_t(0);
+    const int fam = host_family(op);
+    const int64_t T = fam_tile(fam, b->shape[fam]);
+    const int64_t t = (n + T - 1) / T;
     return t < 1 ? int64_t(1) : t;  // an empty reduce / reduce_by_key still writes its result
   };
   auto job_n = [](const bulkx_job& j, int op) {
@@ -1074,10 +1080,11 @@ static int batch_capture(bulkx_batch* b, BatchGraph& G) {
     BULKX_HIP(hipGraphKernelNodeGetParams(node, &kp));
     int kind = -1;
     for (int f = 0; f < FAM_COUNT; ++f)
-      if (kp.func == reinterpret_cast<void*>(kPhase0[f])) kind = f;
-    if (kp.func == reinterpret_cast<void*>(batch_merge_partition_kernel)) kind = kGridMergePartition;
+      if (kp.func == reinterpret_cast<void*>(kPhase0[b->shape[f]][f])) kind = f;
+    if (kp.func == reinterpret_cast<void*>(kMergePartition[b->shape[FAM_MERGE]]))
+      kind = kGridMergePartition;
     for (int r = 0; r < kRbkFamilies; ++r) {
-      if (kp.func == reinterpret_cast<void*>(kRbkUp[r])) kind = FAM_RBK_I32 + r;
+      if (kp.func == reinterpret_cast<void*>(kRbkUp[b->shape[FAM_RBK_I32 + r]][r])) kind = FAM_RBK_I32 + r;
       if (kp.func == reinterpret_cast<void*>(kRbkScan[r])) kind = grid_kind_rbk_scan(FAM_RBK_I32 + r);
     }
     if (kp.func == reinterpret_cast<void*>(batch_radix_upsweep_kernel)) kind = kGridRadixUp;
@@ -1200,6 +1207,15 @@ extern "C" int bulkx_batch_sort_geometry(int64_t* run_elements, int64_t* radix_t
   if (!run_elements || !radix_tile) return BULKX_ERR_INVALID_ARGUMENT;
   *run_elements = kBatchSortRun;
   *radix_tile = kBatchRadixTile;
+  return BULKX_SUCCESS;
+}
+
+extern "C" int bulkx_batch_family_shapes(bulkx_batch* b, int* shape, int64_t* tile_elements) {
+  if (!b) return BULKX_ERR_INVALID_ARGUMENT;
+  for (int f = 0; f < FAM_COUNT; ++f) {
+    if (shape) shape[f] = b->shape[f];
+    if (tile_elements) tile_elements[f] = fam_tile(f, b->shape[f]);
+  }
   return BULKX_SUCCESS;
 }
 

@@ -14,6 +14,7 @@
 #include <mutex>
 #include <string>
 
+#include "batch_shapes.hpp"
 #include "host_common.hpp"
 #include "merge_impl.hpp"
 #include "reduce_by_key_impl.hpp"
@@ -25,7 +26,7 @@ namespace bulkx {
 
 namespace {
 
-constexpr int kAlgos = 8;
+constexpr int kAlgos = 11;  // the plain algorithms, then the batch families
 constexpr int kDtypes = 6;
 
 struct Table {
@@ -129,6 +130,9 @@ int num_variants(int algo) {
     case BULKX_ALGO_MERGE_BY_KEY: return kMergeVariants;
     case BULKX_ALGO_SORT_BY_KEY: return kSortVariants;
     case BULKX_ALGO_REDUCE_BY_KEY: return kRbkVariants;
+    case BULKX_ALGO_BATCH_SCAN:
+    case BULKX_ALGO_BATCH_REDUCE_BY_KEY:
+    case BULKX_ALGO_BATCH_MERGE: return kBatchShapes;
     default: return 1;
   }
 }
@@ -138,7 +142,9 @@ int tuned_variant(int algo, int dtype, int64_t n) {
   Table& t = table();
   std::lock_guard<std::mutex> lock(t.mu);
   ensure_loaded(t);
-  const int v = t.v[algo][dtype][size_class(n)];
+  // the batch families are tuned once per batch, in size class 0
+  const int sc = algo >= BULKX_ALGO_BATCH_SCAN ? 0 : size_class(n);
+  const int v = t.v[algo][dtype][sc];
   return v < 0 ? 0 : v;
 }
 
@@ -235,6 +241,20 @@ int variant_shape(int algo, int dtype, int64_t n, int v, LaunchShape* out) {
       s.group_size = 256;
       s.grain_size = 1;
       s.num_groups = n;  // one workgroup per interval (capped at 65536 and looped)
+      break;
+    }
+    case BULKX_ALGO_BATCH_SCAN:
+    case BULKX_ALGO_BATCH_REDUCE_BY_KEY:
+    case BULKX_ALGO_BATCH_MERGE: {
+      // the family's tile (one workgroup per tile of a job of n elements)
+      const int fam = algo == BULKX_ALGO_BATCH_MERGE ? int(FAM_MERGE)
+                      : algo == BULKX_ALGO_BATCH_SCAN ? (dtype == BULKX_FLOAT32 ? int(FAM_SCAN_F32) : int(FAM_SCAN_I32))
+                      : dtype == BULKX_FLOAT32 ? int(FAM_RBK_F32)
+                      : dtype == BULKX_INT64 ? int(FAM_RBK_I64) : int(FAM_RBK_I32);
+      s.group_size = fam_block(fam, v);
+      s.grain_size = fam_items(fam, v);
+      const int64_t tile = fam_tile(fam, v);
+      s.num_groups = (n + tile - 1) / tile;
       break;
     }
     default:

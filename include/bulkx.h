@@ -55,7 +55,13 @@ enum bulkx_algo {
   BULKX_ALGO_MERGE_BY_KEY = 4,
   BULKX_ALGO_SORT_BY_KEY = 5,
   BULKX_ALGO_REDUCE_BY_KEY = 6,
-  BULKX_ALGO_REDUCE_INTERVALS = 7
+  BULKX_ALGO_REDUCE_INTERVALS = 7,
+  /* tuning-table keys of the batch executor's families (no workspace):
+   * scan / reduce jobs (dtype INT32 or FLOAT32), reduce_by_key jobs (dtype =
+   * the value dtype: INT32, FLOAT32 or INT64) and merge jobs (dtype INT32) */
+  BULKX_ALGO_BATCH_SCAN = 8,
+  BULKX_ALGO_BATCH_REDUCE_BY_KEY = 9,
+  BULKX_ALGO_BATCH_MERGE = 10
 };
 
 /* ---- status codes ---- */
@@ -305,7 +311,9 @@ typedef struct bulkx_job {
 typedef struct bulkx_batch bulkx_batch;
 
 /* Create a batch executor that can hold up to max_jobs jobs and up to
- * max_elements elements in total, bound to `stream`. */
+ * max_elements elements in total, bound to `stream`.  Each op family's tile
+ * shape is read from the tuning table now (BULKX_ALGO_BATCH_*, size class 0;
+ * shape 0 without an entry) and fixed for the batch's life. */
 int bulkx_batch_create(int max_jobs, int64_t max_elements, void* stream, bulkx_batch** batch);
 int bulkx_batch_destroy(bulkx_batch* batch);
 /* Replace the batch's job list (host array, copied). */
@@ -343,6 +351,13 @@ int bulkx_batch_stage_timing(bulkx_batch* batch, float* stage_ms, int num_stages
  * No counterpart in the reference (its merge_sort_by_key.cu:163-171 tile is
  * fixed and its merge passes grow with log2 n). */
 int bulkx_batch_sort_geometry(int64_t* run_elements, int64_t* radix_tile);
+/* The tile shape each op family of this batch runs, in the order of
+ * bulkx_batch_stage_timing's first seven stages (int32 scan/reduce, float32
+ * scan/reduce, reduce_by_key with int32 / float32 / int64 values, merge,
+ * one-tile sorts): shape[i] (0 or 1) and tile_elements[i] (elements per
+ * workgroup).  Either pointer may be NULL.  No counterpart in the
+ * reference, whose tile sizes are constants in each driver. */
+int bulkx_batch_family_shapes(bulkx_batch* batch, int* shape, int64_t* tile_elements);
 
 /* ------------------------------------------------------------------------
  * Serving: a request loop that admits jobs under an HBM budget, keeps their

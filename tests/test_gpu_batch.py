@@ -40,6 +40,8 @@ def _make_jobs(orc, sizes, seed=0):
             keys, runs = orc.gen_runs_geometric(seed + i, n, 8.0)
             if i % 12 == 2:
                 vals = orc.gen_uniform_i64(seed + i, -(1 << 20), 1 << 20, n)
+            elif i % 18 == 8:
+                vals = orc.gen_uniform_f32(seed + i, n, -1.0, 1.0)
             else:
                 vals = orc.gen_uniform_i32(seed + i, -50, 50, n)
             dk, dv = _dev(keys), _dev(vals)
@@ -47,11 +49,25 @@ def _make_jobs(orc, sizes, seed=0):
             vo = torch.empty(n, dtype=dv.dtype, device="cuda")
             cnt = torch.full((1,), -1, dtype=torch.int64, device="cuda")
             jobs.append(B.reduce_by_key_job(dk, dv, ko, vo, cnt))
-            rk, rv = orc.reduce_by_key(keys, vals, textbook=True)
-            checks.append(lambda ko=ko, vo=vo, cnt=cnt, rk=rk, rv=rv:
-                          int(cnt.item()) == rk.size
-                          and np.array_equal(ko.cpu().numpy()[:rk.size], rk)
-                          and np.array_equal(vo.cpu().numpy()[:rk.size], rv))
+            if vals.dtype == np.float32:  # keys and count exact, sums within the bound
+                v64 = vals.astype(np.float64)
+                starts = np.flatnonzero(np.r_[True, keys[1:] != keys[:-1]])
+                exact = np.add.reduceat(v64, starts)
+                lens = np.diff(np.r_[starts, n])
+                bound = 4 * np.maximum(1, np.ceil(np.log2(np.maximum(lens, 2)))) * 2.0 ** -24 * \
+                    np.add.reduceat(np.abs(v64), starts)
+                rk = keys[starts + lens - 1]
+                checks.append(lambda ko=ko, vo=vo, cnt=cnt, rk=rk, ex=exact, bd=bound:
+                              int(cnt.item()) == rk.size
+                              and np.array_equal(ko.cpu().numpy()[:rk.size], rk)
+                              and bool(np.all(np.abs(vo.cpu().numpy()[:rk.size].astype(np.float64) - ex)
+                                              <= bd)))
+            else:
+                rk, rv = orc.reduce_by_key(keys, vals, textbook=True)
+                checks.append(lambda ko=ko, vo=vo, cnt=cnt, rk=rk, rv=rv:
+                              int(cnt.item()) == rk.size
+                              and np.array_equal(ko.cpu().numpy()[:rk.size], rk)
+                              and np.array_equal(vo.cpu().numpy()[:rk.size], rv))
         elif kind == 3:  # merge_by_key, float32 keys with ties, int32 payload
             pool = orc.gen_uniform_f32(seed + 99, 64)
             n1 = n // 2 + 1
@@ -113,6 +129,36 @@ def test_batch_matches_oracle(bx, orc, mode):
     bad = [i for i, c in enumerate(checks) if not c()]
     assert bad == [], f"jobs {bad} differ (sizes {[SIZES[i] for i in bad]})"
     b.close()
+
+
+@pytest.mark.parametrize("shape", [0, 1])
+@pytest.mark.parametrize("mode", ["graph", "eager"])
+def test_batch_family_shapes(bx, orc, shape, mode):
+    """Every family in each of its tile shapes (forced through the tuning
+    table before the batch is created) matches the oracle."""
+    from bulk_amd.batch import Batch
+    keys = [(C.ALGO_BATCH_SCAN, C.INT32), (C.ALGO_BATCH_SCAN, C.FLOAT32),
+            (C.ALGO_BATCH_REDUCE_BY_KEY, C.INT32), (C.ALGO_BATCH_REDUCE_BY_KEY, C.FLOAT32),
+            (C.ALGO_BATCH_REDUCE_BY_KEY, C.INT64), (C.ALGO_BATCH_MERGE, C.INT32)]
+    try:
+        for algo, dt in keys:
+            C.check(C.lib().bulkx_set_tuning(algo, dt, 0, shape), "bulkx_set_tuning")
+        jobs, checks = _make_jobs(orc, SIZES + [3 * 16384 + 5, 5 * 8192 + 1], seed=7)
+        b = Batch(max_jobs=64, max_elements=1 << 23)
+        got = b.family_shapes()
+        assert all(got[f][0] == shape for f in ("scan_i32", "scan_f32", "reduce_by_key_i32",
+                                                 "reduce_by_key_f32", "reduce_by_key_i64",
+                                                 "merge_by_key")), got
+        b.set_jobs(jobs)
+        print(f"batch shape={shape} mode={mode} tiles={got}", flush=True)
+        b.launch(use_graph=(mode == "graph"))
+        torch.cuda.synchronize()
+        bad = [i for i, c in enumerate(checks) if not c()]
+        assert bad == [], f"jobs {bad} differ"
+        assert b.status() == 0
+        b.close()
+    finally:
+        C.check(C.lib().bulkx_load_tuning(None, None), "bulkx_load_tuning")
 
 
 def test_batch_graph_replays_and_new_job_lists(bx, orc):
