@@ -8,7 +8,7 @@ sources.
       graph replays run independent branches concurrently, eager launches
       (the benchmark's per-stage HIP-event timing) one kernel at a time
 
-  python -m bulk_amd.rocprof_summary traffic <fetch_dir> <write_dir> --jobs J --seed S
+  python -m bulk_amd.rocprof_summary traffic <fetch_dir> <write_dir> [--jobs J --seed S]
       HBM bytes per dispatch from a FETCH_SIZE run and a WRITE_SIZE run
       (*_counter_collection.csv).  The counters are scaled by what they read
       for the 1 GiB device copies the profiled process makes
@@ -117,9 +117,11 @@ def traffic(fetch_dir: str, write_dir: str, jobs: int, seed: int, command: str |
     per = {}
     for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
         disp = _per_dispatch(d, counter)
-        # calibration: the largest copy dispatches are the 1 GiB copies
-        copies = [v for (n, g, v) in disp.values() if "copyBuffer" in n or "copy" in n.lower()]
-        big = sorted(copies)[-3:] if copies else []
+        # calibration: the first three large copies of the run (the process
+        # makes its 1 GiB calibration copies before any other large copy)
+        copies = [(did, v) for did, (n, g, v) in disp.items() if "copy" in n.lower()]
+        top = max((v for _, v in copies), default=0.0)
+        big = [v for _, v in sorted(c for c in copies if c[1] >= 0.25 * top)][:3]
         kib = sum(big) / len(big) if big else float("nan")
         factor = COPY_BYTES / (kib * 1024.0) if big else float("nan")
         out.append(f"{counter}: copy of {COPY_BYTES} bytes reads {kib:.0f} KiB -> factor {factor:.4f}")
@@ -129,9 +131,10 @@ def traffic(fetch_dir: str, write_dir: str, jobs: int, seed: int, command: str |
                 groups[_short(n, 200)].append(v * 1024.0 * factor)
         per[counter] = groups
     out.append("")
-    out.append(f"jobs: {jobs}")
-    out.append(f"seed: {seed}")
-    out.append("")
+    if jobs is not None:  # the workload of a bench.py run
+        out.append(f"jobs: {jobs}")
+        out.append(f"seed: {seed}")
+        out.append("")
     names = sorted(set(per["FETCH_SIZE"]) | set(per["WRITE_SIZE"]))
     for n in names:
         f = per["FETCH_SIZE"].get(n, [])
@@ -158,8 +161,8 @@ def main(argv=None):
     t = sub.add_parser("traffic")
     t.add_argument("fetch_dir")
     t.add_argument("write_dir")
-    t.add_argument("--jobs", type=int, required=True)
-    t.add_argument("--seed", type=int, required=True)
+    t.add_argument("--jobs", type=int, default=None)
+    t.add_argument("--seed", type=int, default=None)
     t.add_argument("--command")
     a = p.parse_args(argv)
     if a.mode == "stats":
