@@ -76,7 +76,8 @@ struct BatchJobDev {
   int64_t* splits;   // merge: the merge path of every tile boundary (tiles0 + 1)
   void* tmp_k;       // sort ping-pong buffers
   void* tmp_v;
-  uint32_t* counts;  // sort: digit counts of the radix tiles (256 per tile, digit-major)
+  uint32_t* counts;  // sort: digit counts of the radix tiles (256 per tile, tile-major)
+  uint32_t* totals;  // sort: keys per digit over the job (256), from the row scan
   int64_t tiles0;       // phase-0 tiles
   int64_t tile_begin0;  // first global phase-0 tile
   int64_t sort_tile_begin;  // first global radix tile
@@ -356,7 +357,8 @@ struct RadixBuffers {
   }
 };
 
-// Digit counts of every radix tile of pass `pass` (one workgroup per tile).
+// Digit counts of every radix tile of pass `pass` (one workgroup per tile),
+// each tile's 256 counts one coalesced row (tile-major).
 __global__ __launch_bounds__(BatchRadix::BLOCK) void batch_radix_upsweep_kernel(
     BatchDesc* d, const BatchJobDev* __restrict__ jobs, const uint32_t* __restrict__ sort_tile_job,
     int pass) {
@@ -368,30 +370,33 @@ __global__ __launch_bounds__(BatchRadix::BLOCK) void batch_radix_upsweep_kernel(
   const int shift = pass * kRadixBits;
   switch (job.op) {
     case OP_SORT_I32:
-      radix_count_tiles<int32_t, false, BatchRadix::BLOCK, BatchRadix::ITEMS, 1>(
+      radix_count_tiles<int32_t, false, BatchRadix::BLOCK, BatchRadix::ITEMS, 1, true>(
           s_hist, RadixBuffers<int32_t>(job, pass).sk, job.n0, shift, job.counts, job.sort_tiles, lt);
       break;
     case OP_SORT_U32:
-      radix_count_tiles<uint32_t, false, BatchRadix::BLOCK, BatchRadix::ITEMS, 1>(
+      radix_count_tiles<uint32_t, false, BatchRadix::BLOCK, BatchRadix::ITEMS, 1, true>(
           s_hist, RadixBuffers<uint32_t>(job, pass).sk, job.n0, shift, job.counts, job.sort_tiles, lt);
       break;
     default:
-      radix_count_tiles<float, false, BatchRadix::BLOCK, BatchRadix::ITEMS, 1>(
+      radix_count_tiles<float, false, BatchRadix::BLOCK, BatchRadix::ITEMS, 1, true>(
           s_hist, RadixBuffers<float>(job, pass).sk, job.n0, shift, job.counts, job.sort_tiles, lt);
       break;
   }
 }
 
-// Exclusive scan of each radix job's counts (one workgroup per job): the
-// global position of every (digit, tile) run within the job.
-__global__ __launch_bounds__(1024) void batch_radix_scan_kernel(BatchDesc* d,
-                                                                const BatchJobDev* __restrict__ jobs,
-                                                                const uint32_t* __restrict__ radix_jobs) {
-  __shared__ uint32_t s_wave[1024 / kWave];
-  const int64_t b = blockIdx.x;
+// Row scan of each radix job's counts: sixteen workgroups per job, one per
+// group of 16 digits (radix_scan_rows), so a large job's scan is spread over
+// the device; each count becomes the keys of its digit in the job's earlier
+// tiles, and the job's per-digit totals are written beside them.
+constexpr int kRadixRowGroups = kRadixDigits / 16;
+__global__ __launch_bounds__(256) void batch_radix_scan_kernel(BatchDesc* d,
+                                                               const BatchJobDev* __restrict__ jobs,
+                                                               const uint32_t* __restrict__ radix_jobs) {
+  __shared__ uint32_t s_part[16][16];
+  const int64_t b = blockIdx.x / kRadixRowGroups;
   if (b >= d->num_radix_jobs) return;
   const BatchJobDev& job = jobs[radix_jobs[b]];
-  radix_scan_counts_block(job.counts, job.sort_tiles * kRadixDigits, s_wave);
+  radix_scan_rows(job.counts, job.sort_tiles, job.totals, int(blockIdx.x % kRadixRowGroups), s_part);
 }
 
 template <class K>
@@ -400,7 +405,7 @@ __device__ __forceinline__ void radix_down_job(BatchRadixSmem& sm, const BatchJo
   const RadixBuffers<K> rb(job, pass);
   radix_scatter_tile<K, uint32_t, true, false, BatchRadix::BLOCK, BatchRadix::ITEMS>(
       radix_smem<K>(sm), rb.sk, rb.sv, job.n0, pass * kRadixBits, job.counts, job.sort_tiles,
-      rb.dk, rb.dv, lt);
+      rb.dk, rb.dv, lt, job.totals);
 }
 
 // Scatter of pass `pass` for every radix tile; each XCD takes a contiguous
@@ -578,7 +583,7 @@ static unsigned batch_grid(const bulkx_batch* b, int kind) {
   } else if (kind == kGridRadixUp || kind == kGridRadixDown) {
     g = b->total_sort_tiles;
   } else if (kind == kGridRadixScan) {
-    g = b->num_radix_jobs;
+    g = b->num_radix_jobs * kRadixRowGroups;
   } else {
     g = b->rbk_jobs[kind - grid_kind_rbk_scan(FAM_RBK_I32)];
   }
@@ -641,7 +646,7 @@ static int batch_enqueue_radix(bulkx_batch* b, hipStream_t s, const hipEvent_t* 
         b->d_desc, b->d_jobs, b->d_sort_tile_job, p);
     BULKX_LAUNCHED();
     if (rev) BULKX_HIP(hipEventRecord(rev[3 * p + 1], s));
-    batch_radix_scan_kernel<<<batch_grid(b, kGridRadixScan), 1024, 0, s>>>(b->d_desc, b->d_jobs,
+    batch_radix_scan_kernel<<<batch_grid(b, kGridRadixScan), 256, 0, s>>>(b->d_desc, b->d_jobs,
                                                                            b->d_radix_jobs);
     BULKX_LAUNCHED();
     if (rev) BULKX_HIP(hipEventRecord(rev[3 * p + 2], s));
@@ -814,10 +819,11 @@ extern "C" int bulkx_batch_destroy(bulkx_batch* b) {
 }
 
 // Scratch of one radix sort job of n elements: ping-pong keys and values,
-// and the digit counts of its tiles.
+// the digit counts of its tiles and its per-digit totals.
 static size_t radix_tmp_bytes(int64_t n) {
   const int64_t rt = (n + kBatchRadixTile - 1) / kBatchRadixTile;
-  return 2 * align_up(size_t(n) * 4, 256) + align_up(size_t(rt) * kRadixDigits * sizeof(uint32_t), 256);
+  return 2 * align_up(size_t(n) * 4, 256) + align_up(size_t(rt) * kRadixDigits * sizeof(uint32_t), 256) +
+         align_up(kRadixDigits * sizeof(uint32_t), 256);
 }
 
 // Grows the sort ping-pong space and the per-call workspace to at least the
@@ -843,11 +849,12 @@ static int batch_grow(bulkx_batch* b, size_t tmp_bytes, size_t ws_need) {
 
 extern "C" int bulkx_batch_reserve(bulkx_batch* b, int64_t elements) {
   if (!b || elements < 0 || elements > b->max_elements) return BULKX_ERR_INVALID_ARGUMENT;
-  // sort jobs of `elements` elements in all: ping-pong keys and values and
-  // the digit counts, each array 256-byte aligned
+  // sort jobs of `elements` elements in all: ping-pong keys and values, the
+  // digit counts and the digit totals, each array 256-byte aligned (per
+  // job: a partial tile's counts, the totals and four alignments, < 4 KB)
   const size_t tmp = 2 * align_up(size_t(elements) * 4, 256) +
                      size_t(elements / kBatchRadixTile) * kRadixDigits * sizeof(uint32_t) +
-                     size_t(b->max_jobs) * (kRadixDigits * sizeof(uint32_t) + 3 * 256);
+                     size_t(b->max_jobs) * 4096;
   size_t ws = 256;
   const int64_t n = elements;
   for (int dt : {int(BULKX_INT32), int(BULKX_FLOAT32)}) {
@@ -983,6 +990,8 @@ extern "C" int bulkx_batch_set_jobs(bulkx_batch* b, const bulkx_job* jobs, int n
       tb += align_up(size_t(n) * 4, 256);
       dj.counts = reinterpret_cast<uint32_t*>(b->d_tmp + tb);
       tb += align_up(size_t(dj.sort_tiles) * kRadixDigits * sizeof(uint32_t), 256);
+      dj.totals = reinterpret_cast<uint32_t*>(b->d_tmp + tb);
+      tb += align_up(kRadixDigits * sizeof(uint32_t), 256);
       sort_jobs.push_back(i);
     }
     for (int64_t k = 0; k < t; ++k) b->h_tile_job[cursor[f] + k] = uint32_t(i);

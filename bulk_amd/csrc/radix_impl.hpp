@@ -9,7 +9,10 @@
 //   upsweep   every tile counts its keys per digit into counts[d * T + t]
 //             (digit-major), reading the keys only;
 //   scan      an exclusive scan of counts[] turns every (digit, tile) count
-//             into the global position of the tile's first key of that digit;
+//             into the global position of the tile's first key of that digit
+//             (the batch executor keeps the counts tile-major instead, scans
+//             each digit's row over the tiles, radix_scan_rows, and adds the
+//             smaller digits' totals in the downsweep);
 //   downsweep every tile ranks its keys by digit in input order (stable),
 //             reorders keys and values through LDS into digit order, and
 //             writes each digit's run to its global position -- consecutive
@@ -86,6 +89,7 @@ struct RadixSmem {
   uint32_t tstart[kRadixDigits];            // tile position of each digit's first key
   uint32_t delta[kRadixDigits];             // global position - tile position, per digit
   uint32_t tcount[kRadixDigits];
+  uint32_t gtotal[kRadixDigits];            // row-scanned counts: the sort's keys per digit
 };
 
 // Tile of workgroup b when every XCD takes a contiguous range of tiles:
@@ -99,9 +103,11 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t tiles) {
 }
 
 // Upsweep of tiles [tile0, tile0 + TPW): counts[d * num_tiles + t] = keys
-// of digit d in tile t.  One workgroup counts TPW tiles with all their key
-// loads issued first (TPW * ITEMS keys per lane in flight).
-template <class K, bool GREATER, int BLOCK, int ITEMS, int TPW>
+// of digit d in tile t (digit-major), or with TILE_MAJOR counts[t * 256 + d]
+// (each tile's 256 counts one coalesced 1 KB row).  One workgroup counts
+// TPW tiles with all their key loads issued first (TPW * ITEMS keys per lane
+// in flight).
+template <class K, bool GREATER, int BLOCK, int ITEMS, int TPW, bool TILE_MAJOR = false>
 __device__ __forceinline__ void radix_count_tiles(uint32_t (*s_hist)[kRadixDigits], const K* keys,
                                                   int64_t n, int shift, uint32_t* counts,
                                                   int64_t num_tiles, int64_t tile0) {
@@ -142,7 +148,8 @@ __device__ __forceinline__ void radix_count_tiles(uint32_t (*s_hist)[kRadixDigit
     uint32_t c = 0;
 #pragma unroll
     for (int w = 0; w < NW; ++w) c += s_hist[u * NW + w][d];
-    counts[int64_t(d) * num_tiles + tile0 + u] = c;
+    if constexpr (TILE_MAJOR) counts[(tile0 + u) * kRadixDigits + d] = c;
+    else counts[int64_t(d) * num_tiles + tile0 + u] = c;
   }
 }
 
@@ -165,8 +172,11 @@ __device__ __forceinline__ uint64_t radix_peers(uint32_t d, uint64_t active) {
 
 // Downsweep of tile `tile`: keys (and values) of the tile to their global
 // positions, offsets[d * num_tiles + tile] being the global position of the
-// tile's first key of digit d.  FULL tiles (all but the last) run without
-// per-element bounds.
+// tile's first key of digit d.  With `digit_totals` the offsets are instead
+// tile-major and row-scanned -- offsets[tile * 256 + d] counts the keys of
+// digit d in the tiles before this one -- and the keys of smaller digits
+// (the exclusive scan of digit_totals) are added here.  FULL tiles (all but
+// the last) run without per-element bounds.
 template <class K, class V, bool HAS_V, bool GREATER, int BLOCK, int ITEMS, bool FULL>
 __device__ __forceinline__ void radix_scatter_tile_impl(RadixSmem<K, V, HAS_V, BLOCK, ITEMS>& sm,
                                                         const K* __restrict__ kin,
@@ -174,7 +184,8 @@ __device__ __forceinline__ void radix_scatter_tile_impl(RadixSmem<K, V, HAS_V, B
                                                         int shift,
                                                         const uint32_t* __restrict__ offsets,
                                                         int64_t num_tiles, K* __restrict__ kout,
-                                                        V* __restrict__ vout, int64_t tile) {
+                                                        V* __restrict__ vout, int64_t tile,
+                                                        const uint32_t* __restrict__ digit_totals) {
   constexpr int NW = BLOCK / kWave;
   constexpr int TILE = BLOCK * ITEMS;
   constexpr int CHUNK = kWave * ITEMS;
@@ -187,8 +198,15 @@ __device__ __forceinline__ void radix_scatter_tile_impl(RadixSmem<K, V, HAS_V, B
   const int ccount = FULL ? CHUNK : (tcnt - cb <= 0 ? 0 : (tcnt - cb >= CHUNK ? CHUNK : tcnt - cb));
 
   // the global base of each digit: its load is in flight during the ranking
-  uint32_t gbase = 0;
-  if (threadIdx.x < kRadixDigits) gbase = offsets[int64_t(threadIdx.x) * num_tiles + tile];
+  uint32_t gbase = 0, gtotal = 0;
+  if (threadIdx.x < kRadixDigits) {
+    if (digit_totals) {
+      gbase = offsets[tile * kRadixDigits + threadIdx.x];
+      gtotal = digit_totals[threadIdx.x];
+    } else {
+      gbase = offsets[int64_t(threadIdx.x) * num_tiles + tile];
+    }
+  }
   for (int i = threadIdx.x; i < NW * kRadixDigits; i += BLOCK) (&sm.wcnt[0][0])[i] = 0;
 
   // rows of the wave's chunk: key (i, lane) = chunk[i * 64 + lane]; values
@@ -239,6 +257,7 @@ __device__ __forceinline__ void radix_scatter_tile_impl(RadixSmem<K, V, HAS_V, B
     }
     sm.tcount[d] = run;
     sm.delta[d] = gbase;
+    sm.gtotal[d] = gtotal;
   }
   __syncthreads();
   // the tile's digit starts (exclusive scan over 256 digits, 4 per lane of wave 0)
@@ -256,6 +275,16 @@ __device__ __forceinline__ void radix_scatter_tile_impl(RadixSmem<K, V, HAS_V, B
     sm.delta[4 * lane + 1] -= s1;
     sm.delta[4 * lane + 2] -= s2;
     sm.delta[4 * lane + 3] -= s3;
+    if (digit_totals) {  // keys of the smaller digits, over the whole sort
+      const uint32_t g0 = sm.gtotal[4 * lane], g1 = sm.gtotal[4 * lane + 1];
+      const uint32_t g2 = sm.gtotal[4 * lane + 2], g3 = sm.gtotal[4 * lane + 3];
+      const uint32_t ginc = wave_inclusive_scan<OpPlus>(g0 + g1 + g2 + g3);
+      const uint32_t b0 = ginc - (g0 + g1 + g2 + g3), b1 = b0 + g0, b2 = b1 + g1, b3 = b2 + g2;
+      sm.delta[4 * lane] += b0;
+      sm.delta[4 * lane + 1] += b1;
+      sm.delta[4 * lane + 2] += b2;
+      sm.delta[4 * lane + 3] += b3;
+    }
   }
   __syncthreads();
 
@@ -303,39 +332,57 @@ __device__ __forceinline__ void radix_scatter_tile(RadixSmem<K, V, HAS_V, BLOCK,
                                                    const V* __restrict__ vin, int64_t n, int shift,
                                                    const uint32_t* __restrict__ offsets,
                                                    int64_t num_tiles, K* __restrict__ kout,
-                                                   V* __restrict__ vout, int64_t tile) {
+                                                   V* __restrict__ vout, int64_t tile,
+                                                   const uint32_t* __restrict__ digit_totals = nullptr) {
   if ((tile + 1) * int64_t(BLOCK * ITEMS) <= n)
-    radix_scatter_tile_impl<K, V, HAS_V, GREATER, BLOCK, ITEMS, true>(sm, kin, vin, n, shift, offsets,
-                                                                      num_tiles, kout, vout, tile);
+    radix_scatter_tile_impl<K, V, HAS_V, GREATER, BLOCK, ITEMS, true>(
+        sm, kin, vin, n, shift, offsets, num_tiles, kout, vout, tile, digit_totals);
   else
     radix_scatter_tile_impl<K, V, HAS_V, GREATER, BLOCK, ITEMS, false>(
-        sm, kin, vin, n, shift, offsets, num_tiles, kout, vout, tile);
+        sm, kin, vin, n, shift, offsets, num_tiles, kout, vout, tile, digit_totals);
 }
 
-// Exclusive scan in place of counts[0, m) by one workgroup of 1024 lanes
-// (a batch job's digit counts: m = 256 * tiles, at most 2^17 here).
-__device__ __forceinline__ void radix_scan_counts_block(uint32_t* counts, int64_t m,
-                                                        uint32_t* s_wave /* 16 */) {
-  constexpr int BLOCK = 1024, ITEMS = 4, STEP = BLOCK * ITEMS;
-  uint32_t carry = 0;
-  for (int64_t base = 0; base < m; base += STEP) {
-    const int64_t i0 = base + int64_t(threadIdx.x) * ITEMS;
-    uint32_t c[ITEMS];
-    uint32_t sum = 0;
+// Row scan of tile-major counts (counts[t * 256 + d], T tiles): digits
+// [16 * group, 16 * group + 16) of one sort, by one workgroup of 256 lanes.
+// Lane (s, dl) = (tid / 16, tid % 16) sums digit 16 * group + dl over its
+// 16th of the tiles (each wavefront load instruction reads 16 tiles' 64-byte
+// runs of the group's 16 counts), the sixteen slices' sums are scanned in
+// LDS, and a second sweep replaces every count by the keys of that digit in
+// the tiles before it; totals[d] receives the digit's keys over the sort.
+__device__ __forceinline__ void radix_scan_rows(uint32_t* __restrict__ counts, int64_t T,
+                                                uint32_t* __restrict__ totals, int group,
+                                                uint32_t (*s_part)[16] /* [16][16] */) {
+  const int s = threadIdx.x / 16, dl = threadIdx.x % 16;
+  const int d = group * 16 + dl;
+  const int64_t lo = T * s / 16, hi = T * (s + 1) / 16;
+  constexpr int U = 8;
+  uint32_t sum = 0;
+  for (int64_t t0 = lo; t0 < hi; t0 += U) {
+    uint32_t c[U];
 #pragma unroll
-    for (int q = 0; q < ITEMS; ++q) {
-      c[q] = i0 + q < m ? counts[i0 + q] : 0u;
-      sum += c[q];
-    }
-    uint32_t total;
-    uint32_t ex = block_exclusive_scan<BLOCK, OpPlus>(sum, s_wave, &total);
-    ex += carry;
+    for (int u = 0; u < U; ++u) c[u] = t0 + u < hi ? counts[(t0 + u) * kRadixDigits + d] : 0u;
 #pragma unroll
-    for (int q = 0; q < ITEMS; ++q) {
-      if (i0 + q < m) counts[i0 + q] = ex;
-      ex += c[q];
+    for (int u = 0; u < U; ++u) sum += c[u];
+  }
+  s_part[s][dl] = sum;
+  __syncthreads();
+  uint32_t run = 0, total = 0;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const uint32_t p = s_part[q][dl];
+    run += q < s ? p : 0u;
+    total += p;
+  }
+  if (s == 0) totals[d] = total;
+  for (int64_t t0 = lo; t0 < hi; t0 += U) {
+    uint32_t c[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) c[u] = t0 + u < hi ? counts[(t0 + u) * kRadixDigits + d] : 0u;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (t0 + u < hi) counts[(t0 + u) * kRadixDigits + d] = run;
+      run += c[u];
     }
-    carry += total;
   }
 }
 
