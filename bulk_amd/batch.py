@@ -80,8 +80,11 @@ class Batch:
         C.check(C.lib().bulkx_batch_set_jobs(self._h, arr, len(jobs)), "bulkx_batch_set_jobs")
         self._jobs = list(jobs)
 
-    def launch(self, use_graph: bool = True) -> None:
-        C.check(C.lib().bulkx_batch_launch(self._h, 1 if use_graph else 0), "bulkx_batch_launch")
+    def launch(self, use_graph: bool | int = True) -> None:
+        """use_graph True / 1: graph replay; False / 0: eager launches on the
+        batch's stream (timed per stage); 2: the graph's branches launched
+        eagerly on side streams."""
+        C.check(C.lib().bulkx_batch_launch(self._h, int(use_graph)), "bulkx_batch_launch")
 
     def launch_per_call(self) -> None:
         C.check(C.lib().bulkx_batch_launch_per_call(self._h), "bulkx_batch_launch_per_call")

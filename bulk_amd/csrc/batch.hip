@@ -1116,9 +1116,14 @@ static int batch_copy_err(bulkx_batch* b) {
 extern "C" int bulkx_batch_launch(bulkx_batch* b, int use_graph) {
   if (!b) return BULKX_ERR_INVALID_ARGUMENT;
   b->last_per_call = false;
-  if (!use_graph) {
+  if (use_graph == 0) {
     b->timed = true;
     BULKX_TRY(batch_enqueue_kernels(b, b->stream, true));
+    return batch_copy_err(b);
+  }
+  if (use_graph == 2) {  // the graph's branches as eager launches on the side streams
+    b->timed = false;
+    BULKX_TRY(batch_enqueue_graph(b, b->stream));
     return batch_copy_err(b);
   }
   BatchGraph& G = b->graph;

@@ -318,7 +318,10 @@ int bulkx_batch_create(int max_jobs, int64_t max_elements, void* stream, bulkx_b
 int bulkx_batch_destroy(bulkx_batch* batch);
 /* Replace the batch's job list (host array, copied). */
 int bulkx_batch_set_jobs(bulkx_batch* batch, const bulkx_job* jobs, int num_jobs);
-/* Enqueue the batch on its stream (graph replay when use_graph != 0). */
+/* Enqueue the batch on its stream: use_graph 1 replays the recorded graph
+ * (four concurrent branches), 0 launches every kernel eagerly on the batch's
+ * stream with per-stage timing events, 2 launches the graph's branches
+ * eagerly on the batch's side streams (no graph, no timing). */
 int bulkx_batch_launch(bulkx_batch* batch, int use_graph);
 /* Run each job as its own plain call on the batch's stream (the per-call baseline). */
 int bulkx_batch_launch_per_call(bulkx_batch* batch);

@@ -114,7 +114,7 @@ SIZES = [1, 1000, 4095, 4096, 4097, 12345, 16384, 16385, 65537, 200001, 3, 1 << 
          77777, 8191, 8193, 1 << 20, 50000, 30000, 2, 4096 * 3 + 1, 123457, 9999, 1 << 16, 7]
 
 
-@pytest.mark.parametrize("mode", ["graph", "eager", "per_call"])
+@pytest.mark.parametrize("mode", ["graph", "eager", "branches", "per_call"])
 def test_batch_matches_oracle(bx, orc, mode):
     from bulk_amd.batch import Batch
     jobs, checks = _make_jobs(orc, SIZES, seed=1)
@@ -124,7 +124,7 @@ def test_batch_matches_oracle(bx, orc, mode):
     if mode == "per_call":
         b.launch_per_call()
     else:
-        b.launch(use_graph=(mode == "graph"))
+        b.launch({"graph": 1, "eager": 0, "branches": 2}[mode])
     torch.cuda.synchronize()
     bad = [i for i, c in enumerate(checks) if not c()]
     assert bad == [], f"jobs {bad} differ (sizes {[SIZES[i] for i in bad]})"
