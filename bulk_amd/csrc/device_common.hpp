@@ -139,13 +139,37 @@ struct Acc<double, OpPlus> {
   }
 };
 
+// The order of the comparators.  Integers: <.  Floats: the IEEE order for
+// non-NaN values (-0.0 and +0.0 compare equal); a NaN, which < leaves
+// unordered, orders by its bits -- before -inf when its sign bit is set,
+// after +inf otherwise -- the order of the radix sort's key image, so the
+// merge, the comparison sorts and the radix sort agree on inputs holding NaNs.
+template <class U, class F>
+__host__ __device__ __forceinline__ U float_order_image(F x) {
+  constexpr U sign = U(1) << (8 * sizeof(U) - 1);
+  U b = __builtin_bit_cast(U, x);
+  if (b == sign) b = 0;  // -0.0 orders as +0.0
+  return (b & sign) ? U(~b) : U(b | sign);
+}
+template <class T>
+__host__ __device__ __forceinline__ bool order_less(T a, T b) {
+  if constexpr (std::is_same_v<T, float>) {
+    if (__builtin_expect(a != a || b != b, 0))
+      return float_order_image<uint32_t>(a) < float_order_image<uint32_t>(b);
+  } else if constexpr (std::is_same_v<T, double>) {
+    if (__builtin_expect(a != a || b != b, 0))
+      return float_order_image<uint64_t>(a) < float_order_image<uint64_t>(b);
+  }
+  return a < b;
+}
+
 struct CmpLess {
   template <class T>
-  __host__ __device__ __forceinline__ static bool apply(T a, T b) { return a < b; }
+  __host__ __device__ __forceinline__ static bool apply(T a, T b) { return order_less(a, b); }
 };
 struct CmpGreater {
   template <class T>
-  __host__ __device__ __forceinline__ static bool apply(T a, T b) { return b < a; }
+  __host__ __device__ __forceinline__ static bool apply(T a, T b) { return order_less(b, a); }
 };
 
 // ---------------------------------------------------------------------------
