@@ -9,7 +9,8 @@ namespace bulkx {
 template <class T, class Op, int BLOCK>
 struct ScanSmem {
   using A = typename Acc<T, Op>::type;
-  A wave[BLOCK / kWave];
+  using R = typename Acc<T, Op>::row_t;
+  R wave[BLOCK / kWave];
   A prefix;
 };
 
@@ -48,31 +49,36 @@ __device__ __forceinline__ void scan_tile(ScanSmem<T, Op, BLOCK>& sm, const T* i
     for (int q = 1; q < SL::VE; ++q)
       x[j * SL::VE + q] = Op::apply(x[j * SL::VE + q - 1], x[j * SL::VE + q]);
 
-  // DPP scan across each row (row accumulator), carried from row to row
-  // (carry accumulator).
+  // DPP scan across each row, then the rows and the waves of the tile are
+  // combined in the row accumulator (double for float and float64 input);
+  // only the chain from tile to tile uses the carry accumulator (the
+  // double-double of float64), whose error must not grow with the number of
+  // tiles.  Inside a tile an element's prefix passes through at most
+  // 1 + 6 + 7 + 15 + 1 additions (lane, row, rows, waves, tile prefix), so
+  // its error stays below 4 * ceil(log2 n) ulps of sum |x| at every n.
   using R = typename AccP::row_t;
   using ROp = typename AccP::row_op;
+  const R rid = ROp::template identity<R>();
   R rowp[SL::NV];
   R rowt[SL::NV];  // row totals (wave-uniform)
-  A carry = aid;
+  R carry = rid;
 #pragma unroll
   for (int j = 0; j < SL::NV; ++j) {
     const R inc = wave_inclusive_scan<ROp>(AccP::row(x[j * SL::VE + SL::VE - 1]));
     rowp[j] = wave_shift_right<ROp>(inc);
     rowt[j] = readlane(inc, kWave - 1);
-    carry = AOp::apply(carry, AccP::lift(rowt[j]));
+    carry = ROp::apply(carry, rowt[j]);
   }
 
-  // Combine the waves of the tile.
+  // Combine the waves of the tile: lane w of every wavefront takes wave w's
+  // total and one DPP scan over the lanes gives the wave's exclusive prefix
+  // and the tile's aggregate (one scan instead of 2 * NW additions per lane).
+  static_assert(NW <= kWave, "one lane per wavefront of the tile");
   if (lane_id() == 0) sm.wave[wid] = carry;
   __syncthreads();
-  A wave_prefix = aid, tile_agg = aid;
-#pragma unroll
-  for (int w = 0; w < NW; ++w) {
-    const A t = sm.wave[w];
-    if (w < wid) wave_prefix = AOp::apply(wave_prefix, t);
-    tile_agg = AOp::apply(tile_agg, t);
-  }
+  const R winc = wave_inclusive_scan<ROp>(lane_id() < NW ? sm.wave[lane_id()] : rid);
+  const R wave_prefix = wid > 0 ? readlane(winc, wid - 1) : rid;
+  const A tile_agg = AccP::lift(readlane(winc, NW - 1));
 
   A tile_prefix;  // everything before this tile, init included
   if (tile == 0) {
@@ -92,12 +98,13 @@ __device__ __forceinline__ void scan_tile(ScanSmem<T, Op, BLOCK>& sm, const T* i
     tile_prefix = sm.prefix;
   }
 
-  A rc = AOp::apply(tile_prefix, wave_prefix);  // carry into row j
+  R rc = wave_prefix;  // the tile's elements before row j
 #pragma unroll
   for (int j = 0; j < SL::NV; ++j) {
-    // prefix of this lane's vector in row j: carry level + row level
-    const A p = AOp::apply(rc, AccP::lift(rowp[j]));
-    rc = AOp::apply(rc, AccP::lift(rowt[j]));
+    // prefix of this lane's vector in row j: the tiles before, then the
+    // tile's own elements before it
+    const A p = AOp::apply(tile_prefix, AccP::lift(ROp::apply(rc, rowp[j])));
+    rc = ROp::apply(rc, rowt[j]);
     if constexpr (EXCLUSIVE) {
 #pragma unroll
       for (int q = SL::VE - 1; q > 0; --q)
