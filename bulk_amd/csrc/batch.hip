@@ -410,7 +410,7 @@ __device__ __forceinline__ void radix_down_job(BatchRadixSmem& sm, const BatchJo
 
 // Scatter of pass `pass` for every radix tile; each XCD takes a contiguous
 // range of the tiles (xcd_tile), so a job's neighbouring tiles share L2.
-__global__ __launch_bounds__(BatchRadix::BLOCK) void batch_radix_downsweep_kernel(
+__global__ __launch_bounds__(BatchRadix::BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void batch_radix_downsweep_kernel(
     BatchDesc* d, const BatchJobDev* __restrict__ jobs, const uint32_t* __restrict__ sort_tile_job,
     int pass) {
   __shared__ BatchRadixSmem sm;

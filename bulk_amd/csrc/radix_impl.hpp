@@ -83,8 +83,12 @@ struct RadixSmem {
   static constexpr int NW = BLOCK / kWave;
   static constexpr int TILE = BLOCK * ITEMS;
   static constexpr int cmax(int a, int b) { return a > b ? a : b; }
-  static constexpr int STAGE = TILE * cmax(int(sizeof(K)), HAS_V ? int(sizeof(V)) : 0);
-  __align__(16) char stage[STAGE];          // keys, then values, in tile digit order
+  // 4-byte keys with a 4-byte payload stage as one 8-byte element each (one
+  // round through LDS instead of two)
+  static constexpr bool PACK = HAS_V && sizeof(K) == 4 && sizeof(V) == 4;
+  static constexpr int STAGE =
+      TILE * (PACK ? 8 : cmax(int(sizeof(K)), HAS_V ? int(sizeof(V)) : 0));
+  __align__(16) char stage[STAGE];          // keys, then values (or both packed), in tile digit order
   uint32_t wcnt[NW][kRadixDigits];          // per wave: running count, then wave prefix
   uint32_t tstart[kRadixDigits];            // tile position of each digit's first key
   uint32_t delta[kRadixDigits];             // global position - tile position, per digit
@@ -288,6 +292,35 @@ __device__ __forceinline__ void radix_scatter_tile_impl(RadixSmem<K, V, HAS_V, B
   }
   __syncthreads();
 
+  if constexpr (RadixSmem<K, V, HAS_V, BLOCK, ITEMS>::PACK) {
+    // keys and values to their tile positions in digit order, packed; then
+    // each digit's run to its global position (lane t of the group writes
+    // tile positions t, t + BLOCK, ...)
+    uint64_t* sp = reinterpret_cast<uint64_t*>(sm.stage);
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      if (FULL || i * kWave + lane < ccount) {
+        const uint32_t d = radix_digit<K, GREATER>(k[i], shift);
+        pos[i] += sm.tstart[d] + wc[d];
+        sp[pos[i]] = (uint64_t(bitcast<uint32_t>(v[i])) << 32) | bitcast<uint32_t>(k[i]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i) {
+      const int j = i * BLOCK + int(threadIdx.x);
+      if (FULL || j < tcnt) {
+        const uint64_t e = sp[j];
+        const K key = bitcast<K>(uint32_t(e));
+        const uint32_t dst = uint32_t(j) + sm.delta[radix_digit<K, GREATER>(key, shift)];
+        if (dst < n) {  // always true when the counts are consistent
+          kout[dst] = key;
+          vout[dst] = bitcast<V>(uint32_t(e >> 32));
+        }
+      }
+    }
+    return;
+  }
   // keys to their tile positions in digit order
   K* sk = reinterpret_cast<K*>(sm.stage);
 #pragma unroll
@@ -407,7 +440,7 @@ __global__ __launch_bounds__(BLOCK) void radix_upsweep_kernel(const K* __restric
 }
 
 template <class K, class V, bool HAS_V, bool GREATER, int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void radix_downsweep_kernel(
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void radix_downsweep_kernel(
     const K* __restrict__ kin, const V* __restrict__ vin, int64_t n, int shift,
     const uint32_t* __restrict__ offsets, int64_t num_tiles, K* __restrict__ kout,
     V* __restrict__ vout) {
