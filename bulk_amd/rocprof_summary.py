@@ -8,6 +8,11 @@ sources.
       graph replays run independent branches concurrently, eager launches
       (the benchmark's per-stage HIP-event timing) one kernel at a time
 
+  python -m bulk_amd.rocprof_summary counters <dir> [--command "..."]
+      per library kernel: dispatches, then each counter of a --pmc run
+      (*_counter_collection.csv) summed over its instances and averaged
+      over the kernel's dispatches
+
   python -m bulk_amd.rocprof_summary traffic <fetch_dir> <write_dir> [--jobs J --seed S]
       HBM bytes per dispatch from a FETCH_SIZE run and a WRITE_SIZE run
       (*_counter_collection.csv).  The counters are scaled by what they read
@@ -110,6 +115,31 @@ def _per_dispatch(d: str, counter: str):
     return {k: (meta[k][0], meta[k][1], v) for k, v in vals.items()}
 
 
+def counters(d: str, command: str | None) -> str:
+    out = []
+    if command:
+        out.append(f"command: {command}\n")
+    out.append("Per library kernel: dispatches, then each counter averaged per dispatch "
+               "(summed over the counter instances).\n")
+    per = collections.defaultdict(float)  # (dispatch, counter) -> value
+    name_of = {}
+    for r in csv.DictReader(open(_find(d, "counter_collection.csv"))):
+        if "bulkx" not in r["Kernel_Name"]:
+            continue
+        did = int(r["Dispatch_Id"])
+        per[(did, r["Counter_Name"])] += float(r["Counter_Value"])
+        name_of[did] = _short(r["Kernel_Name"], 200)
+    by_kernel = collections.defaultdict(lambda: collections.defaultdict(list))
+    for (did, c), v in per.items():
+        by_kernel[name_of[did]][c].append(v)
+    for k in sorted(by_kernel):
+        cs = by_kernel[k]
+        n = max(len(v) for v in cs.values())
+        fields = " ".join(f"{c}={sum(v) / len(v):.4g}" for c, v in sorted(cs.items()))
+        out.append(f"{k} dispatches {n} {fields}")
+    return "\n".join(out) + "\n"
+
+
 def traffic(fetch_dir: str, write_dir: str, jobs: int, seed: int, command: str | None) -> str:
     out = []
     if command:
@@ -158,6 +188,9 @@ def main(argv=None):
     s.add_argument("dir")
     s.add_argument("--command")
     s.add_argument("--top", type=int, default=30)
+    c = sub.add_parser("counters")
+    c.add_argument("dir")
+    c.add_argument("--command")
     t = sub.add_parser("traffic")
     t.add_argument("fetch_dir")
     t.add_argument("write_dir")
@@ -167,6 +200,8 @@ def main(argv=None):
     a = p.parse_args(argv)
     if a.mode == "stats":
         print(stats(a.dir, a.command, a.top), end="")
+    elif a.mode == "counters":
+        print(counters(a.dir, a.command), end="")
     else:
         print(traffic(a.fetch_dir, a.write_dir, a.jobs, a.seed, a.command))
 
