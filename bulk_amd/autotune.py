@@ -7,7 +7,8 @@ current GPU and writes the tuning table the library reads at start-up
 For each (algorithm, dtype, size class) the representative input (2^14,
 2^18, 2^22, 2^24, 2^26 elements) is timed with HIP events under every variant
 (median of `--iters` calls, workspace preallocated); the fastest variant is
-stored.  Exclusive scan shares the inclusive scan's choice and merge (keys
+stored.  reduce_by_key is timed on both run-length distributions of the
+single-op benchmark (geometric mean 8 and Zipf), the two calls together.  Exclusive scan shares the inclusive scan's choice and merge (keys
 only) the merge_by_key choice.  The batch executor's op families are tuned
 on the benchmark's mixed-job stream (tune_batch): each family keeps the tile
 shape under which its stage ran fastest.
@@ -107,24 +108,31 @@ def _cases(g):
                 k.copy_(k0)
                 api.stable_merge_sort_by_key(k, v, workspace=ws)
             yield C.ALGO_SORT_BY_KEY, code, sc, srt
-        # runs of geometric length, mean 8 (the single-op benchmark's keys)
-        keys = opbench._runs(n, "geometric", g, torch.device(dev))
+        # the single-op benchmark's two run-length distributions, geometric
+        # (mean 8: a run ends every few elements) and Zipf (long runs): each
+        # variant is timed on both, one after the other
+        keysets = [opbench._runs(n, kind, g, torch.device(dev)) for kind in ("geometric", "zipf")]
         for code in (C.INT32, C.FLOAT32, C.INT64):
             dt = DTYPES[code]
             vals = torch.ones(n, dtype=dt, device=dev)
-            ko = torch.empty_like(keys)
+            ko = torch.empty_like(keysets[0])
             vo = torch.empty_like(vals)
             cnt = torch.empty(1, dtype=torch.int64, device=dev)
             wsk = api.Workspace.for_algo(C.ALGO_REDUCE_BY_KEY, C.INT32, code, n)
-            yield C.ALGO_REDUCE_BY_KEY, code, sc, (lambda keys=keys, vals=vals, ko=ko, vo=vo, cnt=cnt, ws=wsk:
-                                                   api.reduce_by_key_async(keys, vals, ko, vo, cnt, workspace=ws))
+
+            def rbk(keysets=keysets, vals=vals, ko=ko, vo=vo, cnt=cnt, ws=wsk):
+                for keys in keysets:
+                    api.reduce_by_key_async(keys, vals, ko, vo, cnt, workspace=ws)
+            yield C.ALGO_REDUCE_BY_KEY, code, sc, rbk
 
 
-def tune(iters: int = 10, log=print) -> dict:
+def tune(iters: int = 10, log=print, only=None) -> dict:
     g = torch.Generator(device="cuda")
     g.manual_seed(0)
     entries = []
     for algo, code, sc, fn in _cases(g):
+        if only is not None and algo not in only:
+            continue
         times = []
         for v in range(_variants(algo)):
             if algo == C.ALGO_INCLUSIVE_SCAN and v == WIDE_SCAN and code not in (C.INT32, C.UINT32):
@@ -211,6 +219,9 @@ def main(argv=None):
     p.add_argument("--no-batch", action="store_true", help="skip the batch executor's families")
     p.add_argument("--batch-only", action="store_true",
                    help="tune only the batch executor's families; keep the other entries of --out")
+    p.add_argument("--only", default=None,
+                   help="comma-separated algorithm codes to tune (bulkx_algo); the other entries "
+                        "of --out are kept")
     a = p.parse_args(argv)
     C.lib()
     C.check(C.lib().bulkx_load_tuning(b"none", None), "bulkx_load_tuning")  # start from defaults
@@ -219,8 +230,15 @@ def main(argv=None):
         with open(a.out) as f:
             table = json.load(f)
         table["entries"] = [e for e in table["entries"] if e["algo"] < C.ALGO_BATCH_SCAN]
-    else:
-        table = tune(a.iters, log=log)
+    elif a.only:
+        only = {int(x) for x in a.only.split(",")}
+        with open(a.out) as f:
+            table = json.load(f)
+        kept = [e for e in table["entries"] if e["algo"] not in only]
+        fresh = tune(a.iters, log=log, only=only)
+        table["entries"] = kept + fresh["entries"]
+        table["generated"] = fresh["generated"]
+        a.no_batch = True
     if not a.no_batch:
         table["entries"] += tune_batch(log=log)
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)

@@ -23,7 +23,17 @@ size_t reduce_by_key_workspace_bytes(int kdtype, int vdtype, int64_t n) {
   if (ks == 0 || vs == 0 || n <= 0) return 0;
   const int64_t tiles = (n + kRbkMinTile - 1) / kRbkMinTile;  // fits every tile shape
   if (tiles <= 1) return 0;
-  return kRbkHeaderBytes + size_t(tiles) * kRbkMaxStatusWords * sizeof(uint64_t);
+  // look-back records (single pass) or aggregates, totals and flags (three pass)
+  const size_t words = kRbkHeaderBytes + size_t(tiles) * kRbkMaxStatusWords * sizeof(uint64_t);
+  // the compact form's aggregates, totals, meta and scratch runs (T + 1
+  // slots per tile of T), for each of its tile sizes, with the widest
+  // aggregate and the shortest scan segment (so the most segment totals)
+  size_t compact = 0;
+  for (int64_t T : {int64_t(kRbkMinTile), int64_t(kRbkCompactMaxTile)}) {
+    const RbkCompactLayout lay(ks, vs, kRbkMaxStatusWords * 4, n, T, rbk_scan_seg<Seg<dd>>());
+    compact = lay.total > compact ? lay.total : compact;
+  }
+  return words > compact ? words : compact;
 }
 
 template <class K, bool TAIL>

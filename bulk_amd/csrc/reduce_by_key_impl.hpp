@@ -83,10 +83,15 @@ struct RbkConfig {
   static constexpr int TILE = BLOCK * ITEMS;
 };
 constexpr int kRbkShapes = 6;
-constexpr int kRbkVariants = 9;
-// tile shape of a variant
-constexpr int rbk_shape(int variant) { return variant == 6 ? 0 : variant == 7 ? 2 : variant == 8 ? 5 : variant; }
-enum RbkForm { kRbkSinglePass = 0, kRbkThreePass = 1 };
+constexpr int kRbkVariants = 12;
+// tile shape of a variant: 0-5 single pass, 6-8 three-pass and 9-11 compact
+// forms of shapes 0, 2 and 5
+constexpr int rbk_shape(int variant) {
+  return variant == 6 || variant == 9 ? 0 : variant == 7 || variant == 10 ? 2
+         : variant == 8 || variant == 11 ? 5 : variant;
+}
+enum RbkForm { kRbkSinglePass = 0, kRbkThreePass = 1, kRbkCompact = 2 };
+constexpr int kRbkCompactMaxTile = 8192;  // largest tile of a compact variant
 constexpr int kRbkMinTile = 4096;  // smallest tile of any variant (workspace sizing)
 
 // Look-back window of 64*Q tiles (Q tiles per lane).  Q = 1 measured
@@ -124,6 +129,7 @@ struct RbkSmem {
   using S = Seg<typename Acc<V, Op>::type>;
   S wave[BLOCK / kWave];
   S prefix;
+  uint32_t ntails[BLOCK / kWave];  // compact form: run ends per wavefront
   __align__(16) char region[BLOCK / kWave][RbkLds<K, V, ITEMS>::BYTES];
 };
 
@@ -376,6 +382,18 @@ __device__ __forceinline__ void rbk_write_runs(const RbkLane<K, V, Op, ITEMS>& L
   for (uint32_t x = lane; x < cnt; x += kWave) vout[base + x] = sv[x];
 }
 
+// Compact form, pass 1: where a tile leaves its runs and what it reports.
+// Every tile writes the runs that end in it to its own slots of a scratch
+// array (slot s of tile t at t * (TILE + 1) + s, the run's tail being the
+// s-th head of the tile counting a run carried in as slot 0), its aggregate
+// (head count, fold since its last head) and meta = number of runs ending
+// in it | (its first element is a head) << 31.
+template <class S>
+struct RbkCompactOut {
+  S* aggs;
+  uint32_t* meta;
+};
+
 // One tile of reduce_by_key (shared by the device-wide kernels and the
 // batch executor).
 template <class K, class V, class Op, int BLOCK, int ITEMS>
@@ -385,7 +403,8 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
                                          LookbackGuard* guard, bool single_tile,
                                          const Seg<typename Acc<V, Op>::type>* prefixes = nullptr,
                                          const Seg<typename Acc<V, Op>::type>* group_prefixes = nullptr,
-                                         const uint8_t* skip = nullptr, int64_t skip_tiles = 0) {
+                                         const uint8_t* skip = nullptr, int64_t skip_tiles = 0,
+                                         const RbkCompactOut<Seg<typename Acc<V, Op>::type>>* co = nullptr) {
   using AccP = Acc<V, Op>;
   using A = typename AccP::type;
   using S = Seg<A>;
@@ -438,8 +457,8 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
   // their keys: that only needs the in-tile prefix.  With `prefixes` (the
   // three-pass form) the tile's exclusive prefix is already known: nothing
   // is published and nothing looked back.
-  const bool chained = !single_tile && tile > 0 && prefixes == nullptr;
-  if (!single_tile && prefixes == nullptr && threadIdx.x == 0)
+  const bool chained = !single_tile && tile > 0 && prefixes == nullptr && co == nullptr;
+  if (!single_tile && prefixes == nullptr && co == nullptr && threadIdx.x == 0)
     st.publish(tile, tile == 0 ? kStatusPrefix : kStatusAggregate, tile_agg);
   if (chained && threadIdx.x < kWave) {
     S excl = lookback_exclusive_prefix<SOp, kRbkLookbackQ>(st, tile, guard);
@@ -450,7 +469,24 @@ __device__ __forceinline__ void rbk_tile(RbkSmem<K, V, Op, BLOCK, ITEMS>& sm, co
   }
   const RbkWalk<K, V, Op, ITEMS> w =
       rbk_stage_keys(L, SOp::apply(wave_prefix, rbk_lift<V, Op>(lane_prefix)), region, lane);
+  if (co && lane == 0) sm.ntails[wid] = w.ntails;
   __syncthreads();  // the tile's prefix is in sm.prefix
+  if (co) {
+    // slots relative to the tile: a prefix of one head puts the run carried
+    // in at slot 0 and the tile's k-th head's run at slot k
+    S one = SOp::template identity<S>();
+    one.c = 1;
+    if (threadIdx.x == 0) {
+      uint32_t tails = 0;
+#pragma unroll
+      for (int q = 0; q < NW; ++q) tails += sm.ntails[q];
+      co->aggs[tile] = tile_agg;
+      co->meta[tile] = tails | ((L.h & 1u) << 31);  // lane 0 of wave 0 holds the first element
+    }
+    const int64_t off = tile * int64_t(TILE + 1);
+    rbk_write_runs(L, w, one, int64_t(TILE + 1), kout + off, vout + off, region, lane);
+    return;
+  }
   const S tile_prefix = prefixes ? known : chained ? sm.prefix : SOp::template identity<S>();
   // The tile holding element n-1 reports the run count.
   if (threadIdx.x == 0 && tile * TILE + TILE >= n)
@@ -743,6 +779,74 @@ __global__ __launch_bounds__(BLOCK) void rbk_downsweep_kernel(
                                    sizeof(V) >= 8 ? nullptr : flags, tiles);
 }
 
+// Compact form (variants 9-11): keys and values are read once.  Pass 1
+// (rbk_compact_kernel) is the tile above writing the runs that end in a
+// tile to the tile's own scratch slots with the tile-local value of a run
+// carried in, plus the tile's aggregate; pass 2 scans the aggregates (as the
+// three-pass form); pass 3 (rbk_move_kernel) moves each tile's runs to
+// their places, folding the carried-in value into its first one.  Bytes
+// per element for 4-byte keys and values: 8 + 24 * runs / n (11 for runs of
+// 8 elements, ~8 for long runs) against the three-pass form's 12-13.
+template <class K, class V, class Op, int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void rbk_compact_kernel(
+    const K* __restrict__ keys, const V* __restrict__ vals, int64_t n, K* __restrict__ tk,
+    V* __restrict__ tv, Seg<typename Acc<V, Op>::type>* __restrict__ aggs,
+    uint32_t* __restrict__ meta) {
+  __shared__ RbkSmem<K, V, Op, BLOCK, ITEMS> sm;
+  const RbkCompactOut<Seg<typename Acc<V, Op>::type>> co{aggs, meta};
+  rbk_tile<K, V, Op, BLOCK, ITEMS>(sm, keys, vals, n, tk, tv, nullptr, blockIdx.x, nullptr,
+                                   nullptr, false, nullptr, nullptr, nullptr, 0, &co);
+}
+
+// Pass 3 of the compact form: tile t's runs, scratch slots [h0, h0 + m),
+// go to output slots prefix.c + slot - 1; slot 0 (the run carried into the
+// tile) gets the prefix's value folded in.  The last tile reports the run
+// count.
+template <class K, class V, class Op, int TILE>
+__global__ __launch_bounds__(256) void rbk_move_kernel(
+    const K* __restrict__ tk, const V* __restrict__ tv,
+    const Seg<typename Acc<V, Op>::type>* __restrict__ prefixes,
+    const Seg<typename Acc<V, Op>::type>* __restrict__ group_prefixes,
+    const uint32_t* __restrict__ meta, int64_t tiles, int64_t n, K* __restrict__ kout,
+    V* __restrict__ vout, int64_t* __restrict__ num_runs) {
+  using AccP = Acc<V, Op>;
+  using S = Seg<typename AccP::type>;
+  using SOp = SegOp<typename AccP::op, typename AccP::type>;
+  const int64_t tile = blockIdx.x;
+  const uint32_t m = meta[tile];
+  const uint32_t cnt = m & 0x7fffffffu, h0 = m >> 31;
+  const S pre = group_prefixes ? SOp::apply(group_prefixes[tile / rbk_scan_seg<S>()], prefixes[tile])
+                               : prefixes[tile];
+  const int64_t src = tile * int64_t(TILE + 1) + h0;
+  const int64_t dst = int64_t(pre.c) + h0 - 1;
+  // bounds guard: slots stay inside [0, n) when the scan is consistent
+  const uint32_t c = dst < 0 || dst + int64_t(cnt) > n ? 0u : cnt;
+  for (uint32_t x = threadIdx.x; x < c; x += 256) {
+    kout[dst + x] = tk[src + x];
+    V v = tv[src + x];
+    if (h0 == 0 && x == 0) v = AccP::to(AccP::op::apply(pre.v, AccP::from(v)));
+    vout[dst + x] = v;
+  }
+  if (tile == tiles - 1 && threadIdx.x == 0) *num_runs = int64_t(pre.c) + h0 + cnt - 1;
+}
+
+// Workspace of the compact form of tile T over n elements: aggregates,
+// segment totals and meta per tile, then the scratch keys and values.
+struct RbkCompactLayout {
+  size_t aggs, totals, meta, tk, tv, total;
+  RbkCompactLayout(size_t ks, size_t vs, size_t ss, int64_t n, int64_t T, int64_t seg) {
+    const int64_t tiles = (n + T - 1) / T;
+    const int64_t groups = (tiles + seg - 1) / seg;
+    const size_t slots = size_t(tiles) * size_t(T + 1);
+    aggs = kRbkHeaderBytes;
+    totals = aggs + size_t(tiles) * ss;
+    meta = totals + size_t(groups) * ss;
+    tk = align_up(meta + size_t(tiles) * sizeof(uint32_t), 256);
+    tv = align_up(tk + slots * ks, 256);
+    total = tv + slots * vs;
+  }
+};
+
 __global__ void rbk_empty_kernel(int64_t* num_runs);
 __global__ void rbk_prepare_kernel(uint32_t* header, uint64_t* words, int64_t num_words);
 
@@ -765,6 +869,30 @@ int rbk_launch(const K* keys, const V* vals, int64_t n, K* kout, V* vout, int64_
   if (tiles == 1) {
     reduce_by_key_kernel<K, V, Op, C::BLOCK, C::ITEMS><<<1, C::BLOCK, 0, stream>>>(
         keys, vals, n, kout, vout, num_runs, nullptr, nullptr, nullptr, true);
+    BULKX_LAUNCHED();
+    return BULKX_SUCCESS;
+  }
+  if constexpr (FORM == kRbkCompact) {
+    static_assert(C::TILE <= kRbkCompactMaxTile, "scratch sized for the largest compact tile");
+    constexpr int64_t SEG = rbk_scan_seg<S>();
+    const RbkCompactLayout lay(sizeof(K), sizeof(V), sizeof(S), n, C::TILE, SEG);
+    S* aggs = reinterpret_cast<S*>(ws + lay.aggs);
+    const int64_t groups = (tiles + SEG - 1) / SEG;
+    S* totals = groups > 1 ? reinterpret_cast<S*>(ws + lay.totals) : nullptr;
+    uint32_t* meta = reinterpret_cast<uint32_t*>(ws + lay.meta);
+    K* tk = reinterpret_cast<K*>(ws + lay.tk);
+    V* tv = reinterpret_cast<V*>(ws + lay.tv);
+    rbk_compact_kernel<K, V, Op, C::BLOCK, C::ITEMS><<<dim3(uint32_t(tiles)), C::BLOCK, 0, stream>>>(
+        keys, vals, n, tk, tv, aggs, meta);
+    BULKX_LAUNCHED();
+    rbk_scan_aggs_kernel<V, Op><<<dim3(uint32_t(groups)), 256, 0, stream>>>(aggs, tiles, totals);
+    BULKX_LAUNCHED();
+    if (groups > 1) {
+      rbk_scan_aggs_kernel<V, Op><<<1, 256, 0, stream>>>(totals, groups, nullptr);
+      BULKX_LAUNCHED();
+    }
+    rbk_move_kernel<K, V, Op, C::TILE><<<dim3(uint32_t(tiles)), 256, 0, stream>>>(
+        tk, tv, aggs, totals, meta, tiles, n, kout, vout, num_runs);
     BULKX_LAUNCHED();
     return BULKX_SUCCESS;
   }
@@ -843,6 +971,10 @@ int rbk_dispatch_values(int vdtype, int op, const void* keys, const void* vals, 
         case 6: st = rbk_launch<K, V, OP, 0, kRbkThreePass>(k, v, n, ko, vo, num_runs, ws, stream); break;
         case 7: st = rbk_launch<K, V, OP, 2, kRbkThreePass>(k, v, n, ko, vo, num_runs, ws, stream); break;
         case 8: st = rbk_launch<K, V, OP, 5, kRbkThreePass>(k, v, n, ko, vo, num_runs, ws, stream); break;
+        // compact forms of shapes 0, 2 and 5
+        case 9: st = rbk_launch<K, V, OP, 0, kRbkCompact>(k, v, n, ko, vo, num_runs, ws, stream); break;
+        case 10: st = rbk_launch<K, V, OP, 2, kRbkCompact>(k, v, n, ko, vo, num_runs, ws, stream); break;
+        case 11: st = rbk_launch<K, V, OP, 5, kRbkCompact>(k, v, n, ko, vo, num_runs, ws, stream); break;
         default: st = rbk_launch<K, V, OP, 0>(k, v, n, ko, vo, num_runs, ws, stream); break;
       }
     } else {

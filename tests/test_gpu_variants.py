@@ -101,7 +101,7 @@ def test_merge_and_sort_variants(force, orc, v):
         assert np.array_equal(d8.cpu().numpy(), np.sort(k8, kind="stable")), n
 
 
-@pytest.mark.parametrize("v", list(range(9)))
+@pytest.mark.parametrize("v", list(range(12)))
 def test_reduce_by_key_variants(force, orc, v):
     from bulk_amd import api
     for code in (C.INT32, C.INT64, C.FLOAT32):
@@ -140,7 +140,7 @@ def test_reduce_by_key_variants(force, orc, v):
         assert np.all(np.abs(vo[:m].cpu().numpy().astype(np.float64) - exact) <= bound), n
 
 
-@pytest.mark.parametrize("v", list(range(9)))
+@pytest.mark.parametrize("v", list(range(12)))
 def test_reduce_by_key_last_run_key_zero(force, orc, v):
     """A run ending the array whose key equals the loads' fill value (0),
     with the array's end on a lane boundary inside a partial chunk and
