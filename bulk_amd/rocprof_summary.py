@@ -13,6 +13,14 @@ sources.
       (*_counter_collection.csv) summed over its instances and averaged
       over the kernel's dispatches
 
+  python -m bulk_amd.rocprof_summary requests <read_dir> <write_dir> [--command "..."]
+      request sizes at the L2-to-memory interface per library kernel, from a
+      --pmc run of TCC_BUBBLE_sum TCC_EA0_RDREQ_sum TCC_EA0_RDREQ_32B_sum
+      (reads: 128-byte requests are TCC_BUBBLE, 32-byte ones RDREQ_32B, the
+      rest 64-byte -- the split the FETCH_SIZE expression uses) and one of
+      TCC_EA0_WRREQ_sum TCC_EA0_WRREQ_64B_sum (writes: 64 or 32 bytes),
+      summed over each kernel's dispatches
+
   python -m bulk_amd.rocprof_summary traffic <fetch_dir> <write_dir> [--jobs J --seed S]
       HBM bytes per dispatch from a FETCH_SIZE run and a WRITE_SIZE run
       (*_counter_collection.csv).  The counters are scaled by what they read
@@ -140,6 +148,42 @@ def counters(d: str, command: str | None) -> str:
     return "\n".join(out) + "\n"
 
 
+def _kernel_counters(d: str) -> dict:
+    """{short kernel name: {counter: value summed over instances and dispatches}}"""
+    out = collections.defaultdict(lambda: collections.defaultdict(float))
+    for r in csv.DictReader(open(_find(d, "counter_collection.csv"))):
+        if "bulkx" in r["Kernel_Name"] or "copy" in r["Kernel_Name"].lower():
+            out[_short(r["Kernel_Name"], 200)][r["Counter_Name"]] += float(r["Counter_Value"])
+    return out
+
+
+def requests(read_dir: str, write_dir: str, command: str | None) -> str:
+    out = []
+    if command:
+        out.append(f"command: {command}\n")
+    out.append("Per library kernel, summed over its dispatches: read requests by size "
+               "(128 / 64 / 32 bytes) and the share of read bytes in 128-byte requests; "
+               "write requests by size (64 / 32 bytes) and the share of written bytes in "
+               "64-byte requests.\n")
+    rd, wr = _kernel_counters(read_dir), _kernel_counters(write_dir)
+    out.append("| kernel | reads 128 B | 64 B | 32 B | read bytes in 128 B | "
+               "writes 64 B | 32 B | written bytes in 64 B |")
+    out.append("|---|---:|---:|---:|---:|---:|---:|---:|")
+    for k in sorted(set(rd) | set(wr)):
+        r, w = rd.get(k, {}), wr.get(k, {})
+        n128 = r.get("TCC_BUBBLE_sum", 0.0)
+        n32 = r.get("TCC_EA0_RDREQ_32B_sum", 0.0)
+        n64 = max(0.0, r.get("TCC_EA0_RDREQ_sum", 0.0) - n128 - n32)
+        rb = 128 * n128 + 64 * n64 + 32 * n32
+        w64 = w.get("TCC_EA0_WRREQ_64B_sum", 0.0)
+        w32 = max(0.0, w.get("TCC_EA0_WRREQ_sum", 0.0) - w64)
+        wb = 64 * w64 + 32 * w32
+        fr = f"{100 * 128 * n128 / rb:.1f} %" if rb else "-"
+        fw = f"{100 * 64 * w64 / wb:.1f} %" if wb else "-"
+        out.append(f"| {k} | {n128:.4g} | {n64:.4g} | {n32:.4g} | {fr} | {w64:.4g} | {w32:.4g} | {fw} |")
+    return "\n".join(out) + "\n"
+
+
 def traffic(fetch_dir: str, write_dir: str, jobs: int, seed: int, command: str | None) -> str:
     out = []
     if command:
@@ -191,6 +235,10 @@ def main(argv=None):
     c = sub.add_parser("counters")
     c.add_argument("dir")
     c.add_argument("--command")
+    q = sub.add_parser("requests")
+    q.add_argument("read_dir")
+    q.add_argument("write_dir")
+    q.add_argument("--command")
     t = sub.add_parser("traffic")
     t.add_argument("fetch_dir")
     t.add_argument("write_dir")
@@ -202,6 +250,8 @@ def main(argv=None):
         print(stats(a.dir, a.command, a.top), end="")
     elif a.mode == "counters":
         print(counters(a.dir, a.command), end="")
+    elif a.mode == "requests":
+        print(requests(a.read_dir, a.write_dir, a.command), end="")
     else:
         print(traffic(a.fetch_dir, a.write_dir, a.jobs, a.seed, a.command))
 
