@@ -362,7 +362,7 @@ struct RadixBuffers {
 __global__ __launch_bounds__(BatchRadix::BLOCK) void batch_radix_upsweep_kernel(
     BatchDesc* d, const BatchJobDev* __restrict__ jobs, const uint32_t* __restrict__ sort_tile_job,
     int pass) {
-  __shared__ uint32_t s_hist[BatchRadix::BLOCK / kWave][kRadixDigits];
+  __shared__ uint32_t s_hist[kRadixCountWords];
   const int64_t t = blockIdx.x;
   if (t >= d->total_sort_tiles) return;
   const BatchJobDev& job = jobs[sort_tile_job[t]];
@@ -370,15 +370,15 @@ __global__ __launch_bounds__(BatchRadix::BLOCK) void batch_radix_upsweep_kernel(
   const int shift = pass * kRadixBits;
   switch (job.op) {
     case OP_SORT_I32:
-      radix_count_tiles<int32_t, false, BatchRadix::BLOCK, BatchRadix::ITEMS, 1, true>(
+      radix_count_tile<int32_t, false, BatchRadix::BLOCK, BatchRadix::ITEMS, true>(
           s_hist, RadixBuffers<int32_t>(job, pass).sk, job.n0, shift, job.counts, job.sort_tiles, lt);
       break;
     case OP_SORT_U32:
-      radix_count_tiles<uint32_t, false, BatchRadix::BLOCK, BatchRadix::ITEMS, 1, true>(
+      radix_count_tile<uint32_t, false, BatchRadix::BLOCK, BatchRadix::ITEMS, true>(
           s_hist, RadixBuffers<uint32_t>(job, pass).sk, job.n0, shift, job.counts, job.sort_tiles, lt);
       break;
     default:
-      radix_count_tiles<float, false, BatchRadix::BLOCK, BatchRadix::ITEMS, 1, true>(
+      radix_count_tile<float, false, BatchRadix::BLOCK, BatchRadix::ITEMS, true>(
           s_hist, RadixBuffers<float>(job, pass).sk, job.n0, shift, job.counts, job.sort_tiles, lt);
       break;
   }
@@ -389,14 +389,16 @@ __global__ __launch_bounds__(BatchRadix::BLOCK) void batch_radix_upsweep_kernel(
 // the device; each count becomes the keys of its digit in the job's earlier
 // tiles, and the job's per-digit totals are written beside them.
 constexpr int kRadixRowGroups = kRadixDigits / 16;
-__global__ __launch_bounds__(256) void batch_radix_scan_kernel(BatchDesc* d,
-                                                               const BatchJobDev* __restrict__ jobs,
-                                                               const uint32_t* __restrict__ radix_jobs) {
-  __shared__ uint32_t s_part[16][16];
+constexpr int kRadixRowSlices = 16;
+constexpr int kRadixRowBlock = 16 * kRadixRowSlices;
+__global__ __launch_bounds__(kRadixRowBlock) void batch_radix_scan_kernel(
+    BatchDesc* d, const BatchJobDev* __restrict__ jobs, const uint32_t* __restrict__ radix_jobs) {
+  __shared__ uint32_t s_part[kRadixRowSlices][16];
   const int64_t b = blockIdx.x / kRadixRowGroups;
   if (b >= d->num_radix_jobs) return;
   const BatchJobDev& job = jobs[radix_jobs[b]];
-  radix_scan_rows(job.counts, job.sort_tiles, job.totals, int(blockIdx.x % kRadixRowGroups), s_part);
+  radix_scan_rows<kRadixRowSlices>(job.counts, job.sort_tiles, job.totals,
+                                   int(blockIdx.x % kRadixRowGroups), s_part);
 }
 
 template <class K>
@@ -646,8 +648,8 @@ static int batch_enqueue_radix(bulkx_batch* b, hipStream_t s, const hipEvent_t* 
         b->d_desc, b->d_jobs, b->d_sort_tile_job, p);
     BULKX_LAUNCHED();
     if (rev) BULKX_HIP(hipEventRecord(rev[3 * p + 1], s));
-    batch_radix_scan_kernel<<<batch_grid(b, kGridRadixScan), 256, 0, s>>>(b->d_desc, b->d_jobs,
-                                                                           b->d_radix_jobs);
+    batch_radix_scan_kernel<<<batch_grid(b, kGridRadixScan), kRadixRowBlock, 0, s>>>(
+        b->d_desc, b->d_jobs, b->d_radix_jobs);
     BULKX_LAUNCHED();
     if (rev) BULKX_HIP(hipEventRecord(rev[3 * p + 2], s));
     batch_radix_downsweep_kernel<<<batch_grid(b, kGridRadixDown), BatchRadix::BLOCK, 0, s>>>(

@@ -106,54 +106,56 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t tiles) {
   return x * q + (x < r ? x : r) + i;
 }
 
-// Upsweep of tiles [tile0, tile0 + TPW): counts[d * num_tiles + t] = keys
-// of digit d in tile t (digit-major), or with TILE_MAJOR counts[t * 256 + d]
-// (each tile's 256 counts one coalesced 1 KB row).  One workgroup counts
-// TPW tiles with all their key loads issued first (TPW * ITEMS keys per lane
-// in flight).
-template <class K, bool GREATER, int BLOCK, int ITEMS, int TPW, bool TILE_MAJOR = false>
-__device__ __forceinline__ void radix_count_tiles(uint32_t (*s_hist)[kRadixDigits], const K* keys,
-                                                  int64_t n, int shift, uint32_t* counts,
-                                                  int64_t num_tiles, int64_t tile0) {
-  constexpr int NW = BLOCK / kWave;
+// Histogram copies of the upsweep: lane l counts into copy l % 32, so one
+// LDS atomic instruction puts at most two lanes on any counter, whatever
+// the digits are.  Rows are 257 words apart, so the copies of one digit
+// fall in different banks.  With one histogram per wavefront, keys whose
+// digit is mostly one value (float32 in [0, 1) in the top byte, keys with
+// few distinct values, presorted runs) serialise up to 64 lanes on one
+// counter: 2^26 keys count in 44 us per pass with the copies on every
+// distribution, against 132 us (float32 in [0, 1), top byte) to 238 us
+// (all keys equal) with one histogram per wavefront.
+constexpr int kRadixCountCopies = 32;
+constexpr int kRadixCountStride = kRadixDigits + 1;
+constexpr int kRadixCountWords = kRadixCountCopies * kRadixCountStride;
+
+// Upsweep of tile `tile`: counts[d * num_tiles + tile] = keys of digit d in
+// the tile (digit-major), or with TILE_MAJOR counts[tile * 256 + d] (each
+// tile's 256 counts one coalesced 1 KB row).  s_hist holds
+// kRadixCountWords words.
+template <class K, bool GREATER, int BLOCK, int ITEMS, bool TILE_MAJOR = false>
+__device__ __forceinline__ void radix_count_tile(uint32_t* s_hist, const K* keys, int64_t n,
+                                                 int shift, uint32_t* counts, int64_t num_tiles,
+                                                 int64_t tile) {
   constexpr int TILE = BLOCK * ITEMS;
   using SL = Striped<K, ITEMS>;
   const int wid = threadIdx.x / kWave;
   const int lane = lane_id();
-  uint32_t* h = &s_hist[0][0];
-  for (int i = threadIdx.x; i < TPW * NW * kRadixDigits; i += BLOCK) h[i] = 0;
-  K x[TPW][ITEMS];
-  int count[TPW];
+  for (int i = threadIdx.x; i < kRadixCountWords; i += BLOCK) s_hist[i] = 0;
+  K x[ITEMS];
+  const int64_t cb = tile * TILE + int64_t(wid) * SL::N;
+  const int64_t rem = n - cb;
+  const int count = rem <= 0 ? 0 : (rem >= SL::N ? SL::N : int(rem));
+  striped_load<K, ITEMS, SL::VE, true>(keys + (count > 0 ? cb : 0), count, x, K());
+  __syncthreads();
+  uint32_t* h = s_hist + (lane % kRadixCountCopies) * kRadixCountStride;
+  if (count == SL::N) {
 #pragma unroll
-  for (int u = 0; u < TPW; ++u) {
-    const int64_t cb = (tile0 + u) * TILE + int64_t(wid) * SL::N;
-    const int64_t rem = n - cb;
-    count[u] = rem <= 0 ? 0 : (rem >= SL::N ? SL::N : int(rem));
-    striped_load<K, ITEMS, SL::VE, true>(keys + (count[u] > 0 ? cb : 0), count[u], x[u], K());
+    for (int i = 0; i < ITEMS; ++i) atomicAdd(&h[radix_digit<K, GREATER>(x[i], shift)], 1u);
+  } else {
+#pragma unroll
+    for (int i = 0; i < ITEMS; ++i)
+      if (SL::offset(i, lane) < count) atomicAdd(&h[radix_digit<K, GREATER>(x[i], shift)], 1u);
   }
   __syncthreads();
-#pragma unroll
-  for (int u = 0; u < TPW; ++u) {
-    uint32_t* hw = s_hist[u * NW + wid];
-    if (count[u] == SL::N) {
-#pragma unroll
-      for (int i = 0; i < ITEMS; ++i) atomicAdd(&hw[radix_digit<K, GREATER>(x[u][i], shift)], 1u);
-    } else {
-#pragma unroll
-      for (int i = 0; i < ITEMS; ++i)
-        if (SL::offset(i, lane) < count[u])
-          atomicAdd(&hw[radix_digit<K, GREATER>(x[u][i], shift)], 1u);
-    }
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < TPW * kRadixDigits; e += BLOCK) {
-    const int u = e / kRadixDigits, d = e % kRadixDigits;
-    if (tile0 + u >= num_tiles) continue;
+  static_assert(BLOCK >= kRadixDigits, "one lane per digit");
+  if (threadIdx.x < kRadixDigits && tile < num_tiles) {
+    const int d = threadIdx.x;
     uint32_t c = 0;
 #pragma unroll
-    for (int w = 0; w < NW; ++w) c += s_hist[u * NW + w][d];
-    if constexpr (TILE_MAJOR) counts[(tile0 + u) * kRadixDigits + d] = c;
-    else counts[int64_t(d) * num_tiles + tile0 + u] = c;
+    for (int r = 0; r < kRadixCountCopies; ++r) c += s_hist[r * kRadixCountStride + d];
+    if constexpr (TILE_MAJOR) counts[tile * kRadixDigits + d] = c;
+    else counts[int64_t(d) * num_tiles + tile] = c;
   }
 }
 
@@ -376,19 +378,25 @@ __device__ __forceinline__ void radix_scatter_tile(RadixSmem<K, V, HAS_V, BLOCK,
 }
 
 // Row scan of tile-major counts (counts[t * 256 + d], T tiles): digits
-// [16 * group, 16 * group + 16) of one sort, by one workgroup of 256 lanes.
-// Lane (s, dl) = (tid / 16, tid % 16) sums digit 16 * group + dl over its
-// 16th of the tiles (each wavefront load instruction reads 16 tiles' 64-byte
-// runs of the group's 16 counts), the sixteen slices' sums are scanned in
-// LDS, and a second sweep replaces every count by the keys of that digit in
-// the tiles before it; totals[d] receives the digit's keys over the sort.
+// [16 * group, 16 * group + 16) of one sort, by one workgroup of
+// 16 * SLICES lanes.  Lane (s, dl) = (tid / 16, tid % 16) sums digit
+// 16 * group + dl over its SLICES-th of the tiles (each wavefront load
+// instruction reads four tiles' 64-byte runs of the group's 16 counts), the
+// slices' sums are scanned in LDS, and a second sweep replaces every count
+// by the keys of that digit in the tiles before it; totals[d] receives the
+// digit's keys over the sort.  A lane keeps 32 loads in flight.  One
+// 2^26-key sort (8192 tiles) scans in 61 us per pass with 16 slices (72 us
+// with 8 loads in flight); 64 slices of 1024 lanes take it to 34 us but
+// double the time of a batch of 311 sorts of 2^13..2^22 keys (26 -> 55 us),
+// whose small jobs leave most of those lanes idle.
+template <int SLICES>
 __device__ __forceinline__ void radix_scan_rows(uint32_t* __restrict__ counts, int64_t T,
                                                 uint32_t* __restrict__ totals, int group,
-                                                uint32_t (*s_part)[16] /* [16][16] */) {
+                                                uint32_t (*s_part)[16] /* [SLICES][16] */) {
   const int s = threadIdx.x / 16, dl = threadIdx.x % 16;
   const int d = group * 16 + dl;
-  const int64_t lo = T * s / 16, hi = T * (s + 1) / 16;
-  constexpr int U = 8;
+  const int64_t lo = T * s / SLICES, hi = T * (s + 1) / SLICES;
+  constexpr int U = 32;
   uint32_t sum = 0;
   for (int64_t t0 = lo; t0 < hi; t0 += U) {
     uint32_t c[U];
@@ -400,8 +408,8 @@ __device__ __forceinline__ void radix_scan_rows(uint32_t* __restrict__ counts, i
   s_part[s][dl] = sum;
   __syncthreads();
   uint32_t run = 0, total = 0;
-#pragma unroll
-  for (int q = 0; q < 16; ++q) {
+#pragma unroll 16
+  for (int q = 0; q < SLICES; ++q) {
     const uint32_t p = s_part[q][dl];
     run += q < s ? p : 0u;
     total += p;
@@ -426,17 +434,19 @@ __device__ __forceinline__ void radix_scan_rows(uint32_t* __restrict__ counts, i
 // ---------------------------------------------------------------------------
 namespace bulkx {
 
-// Tiles counted by one upsweep workgroup.
-constexpr int kRadixCountTiles = 1;
-
+// One upsweep workgroup per tile, tiles XCD-contiguous (xcd_tile): the
+// digit-major counts of neighbouring tiles share 64-byte lines, and a line
+// whose counts all come from one XCD's L2 leaves it as one full write
+// instead of one partial write per XCD that touched it (2^26 keys: 59 us
+// per pass with the plain block order, 43 us with this one).
 template <class K, bool GREATER, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void radix_upsweep_kernel(const K* __restrict__ keys,
                                                               int64_t n, int shift,
                                                               uint32_t* __restrict__ counts,
                                                               int64_t num_tiles) {
-  __shared__ uint32_t s_hist[kRadixCountTiles * (BLOCK / kWave)][kRadixDigits];
-  radix_count_tiles<K, GREATER, BLOCK, ITEMS, kRadixCountTiles>(
-      s_hist, keys, n, shift, counts, num_tiles, int64_t(blockIdx.x) * kRadixCountTiles);
+  __shared__ uint32_t s_hist[kRadixCountWords];
+  radix_count_tile<K, GREATER, BLOCK, ITEMS>(s_hist, keys, n, shift, counts, num_tiles,
+                                             xcd_tile(blockIdx.x, num_tiles));
 }
 
 template <class K, class V, bool HAS_V, bool GREATER, int BLOCK, int ITEMS>
@@ -484,8 +494,7 @@ int radix_sort_launch(K* keys, V* vals, int64_t n, char* ws, hipStream_t stream)
   V* dv = tv;
   for (int p = 0; p < kRadixPasses; ++p) {
     const int shift = p * kRadixBits;
-    const int64_t up_groups = (tiles + kRadixCountTiles - 1) / kRadixCountTiles;
-    radix_upsweep_kernel<K, G, C::BLOCK, C::ITEMS><<<dim3(uint32_t(up_groups)), C::BLOCK, 0, stream>>>(
+    radix_upsweep_kernel<K, G, C::BLOCK, C::ITEMS><<<dim3(uint32_t(tiles)), C::BLOCK, 0, stream>>>(
         sk, n, shift, counts, tiles);
     BULKX_LAUNCHED();
     BULKX_TRY(exclusive_scan_u32(counts, counts, tiles * kRadixDigits, scan_ws, stream));

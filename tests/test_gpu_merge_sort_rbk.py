@@ -117,6 +117,30 @@ def test_stable_sort_other_key_types(bx, orc, dt):
     assert np.array_equal(_host(dk), keys[order]) and np.array_equal(_host(dv), vals[order])
 
 
+@pytest.mark.parametrize("dist", ["f32 [0,1)", "i32 all equal", "i32 87% one value",
+                                  "i32 presorted", "f32 [0,1) greater"])
+def test_stable_sort_skewed_digits(bx, dist):
+    """Inputs whose digits are mostly one value in some radix pass (the
+    counting histograms see up to 64 lanes on one digit), over many tiles."""
+    rng = np.random.default_rng(21)
+    n = 1_000_003
+    if dist.startswith("f32"):
+        keys = rng.random(n, dtype=np.float32)
+    elif dist == "i32 all equal":
+        keys = np.full(n, -77, np.int32)
+    elif dist == "i32 87% one value":
+        keys = rng.integers(-(1 << 31), 1 << 31, size=n, dtype=np.int64).astype(np.int32)
+        keys[rng.random(n) < 0.87] = 123456
+    else:
+        keys = np.sort(rng.integers(-(1 << 31), 1 << 31, size=n, dtype=np.int64).astype(np.int32))
+    comp = "greater" if dist.endswith("greater") else "less"
+    vals = np.arange(n, dtype=np.int32)
+    dk, dv = _dev(keys), _dev(vals)
+    bx.stable_merge_sort_by_key(dk, dv, comp=comp)
+    order = np.argsort(-keys if comp == "greater" else keys, kind="stable")
+    assert np.array_equal(_host(dk), keys[order]) and np.array_equal(_host(dv), vals[order])
+
+
 def test_stable_sort_keys_only(bx, orc):
     keys = orc.ref_hash_i32(200000)
     dk = _dev(keys)
