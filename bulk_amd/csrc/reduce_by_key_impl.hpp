@@ -801,18 +801,18 @@ __global__ __launch_bounds__(BLOCK) void rbk_compact_kernel(
 // Pass 3 of the compact form: tile t's runs, scratch slots [h0, h0 + m),
 // go to output slots prefix.c + slot - 1; slot 0 (the run carried into the
 // tile) gets the prefix's value folded in.  The last tile reports the run
-// count.
+// count.  One workgroup of 256 lanes per tile (shared with the batch
+// executor).
 template <class K, class V, class Op, int TILE>
-__global__ __launch_bounds__(256) void rbk_move_kernel(
-    const K* __restrict__ tk, const V* __restrict__ tv,
-    const Seg<typename Acc<V, Op>::type>* __restrict__ prefixes,
-    const Seg<typename Acc<V, Op>::type>* __restrict__ group_prefixes,
-    const uint32_t* __restrict__ meta, int64_t tiles, int64_t n, K* __restrict__ kout,
-    V* __restrict__ vout, int64_t* __restrict__ num_runs) {
+__device__ __forceinline__ void rbk_move_tile(const K* __restrict__ tk, const V* __restrict__ tv,
+                                              const Seg<typename Acc<V, Op>::type>* __restrict__ prefixes,
+                                              const Seg<typename Acc<V, Op>::type>* __restrict__ group_prefixes,
+                                              const uint32_t* __restrict__ meta, int64_t tiles,
+                                              int64_t n, K* __restrict__ kout, V* __restrict__ vout,
+                                              int64_t* __restrict__ num_runs, int64_t tile) {
   using AccP = Acc<V, Op>;
   using S = Seg<typename AccP::type>;
   using SOp = SegOp<typename AccP::op, typename AccP::type>;
-  const int64_t tile = blockIdx.x;
   const uint32_t m = meta[tile];
   const uint32_t cnt = m & 0x7fffffffu, h0 = m >> 31;
   const S pre = group_prefixes ? SOp::apply(group_prefixes[tile / rbk_scan_seg<S>()], prefixes[tile])
@@ -821,13 +821,24 @@ __global__ __launch_bounds__(256) void rbk_move_kernel(
   const int64_t dst = int64_t(pre.c) + h0 - 1;
   // bounds guard: slots stay inside [0, n) when the scan is consistent
   const uint32_t c = dst < 0 || dst + int64_t(cnt) > n ? 0u : cnt;
-  for (uint32_t x = threadIdx.x; x < c; x += 256) {
+  for (uint32_t x = threadIdx.x; x < c; x += blockDim.x) {
     kout[dst + x] = tk[src + x];
     V v = tv[src + x];
     if (h0 == 0 && x == 0) v = AccP::to(AccP::op::apply(pre.v, AccP::from(v)));
     vout[dst + x] = v;
   }
   if (tile == tiles - 1 && threadIdx.x == 0) *num_runs = int64_t(pre.c) + h0 + cnt - 1;
+}
+
+template <class K, class V, class Op, int TILE>
+__global__ __launch_bounds__(256) void rbk_move_kernel(
+    const K* __restrict__ tk, const V* __restrict__ tv,
+    const Seg<typename Acc<V, Op>::type>* __restrict__ prefixes,
+    const Seg<typename Acc<V, Op>::type>* __restrict__ group_prefixes,
+    const uint32_t* __restrict__ meta, int64_t tiles, int64_t n, K* __restrict__ kout,
+    V* __restrict__ vout, int64_t* __restrict__ num_runs) {
+  rbk_move_tile<K, V, Op, TILE>(tk, tv, prefixes, group_prefixes, meta, tiles, n, kout, vout,
+                                num_runs, int64_t(blockIdx.x));
 }
 
 // Workspace of the compact form of tile T over n elements: aggregates,
