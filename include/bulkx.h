@@ -329,7 +329,8 @@ int bulkx_batch_launch_per_call(bulkx_batch* batch);
  * its look-back waits gave up (see bulkx_stream_status), else BULKX_SUCCESS. */
 int bulkx_batch_status(bulkx_batch* batch);
 /* Allocates now the scratch that job lists of up to `elements` elements can
- * need (sort ping-pong space, the per-call path's workspace), so that later
+ * need (sort ping-pong space, reduce_by_key run slots -- about 12 bytes per
+ * element -- and the per-call path's workspace), so that later
  * bulkx_batch_set_jobs calls within that size allocate and free nothing. */
 int bulkx_batch_reserve(bulkx_batch* batch, int64_t elements);
 /* Record the batch's graph ahead of time (it is recorded on first use
