@@ -98,6 +98,7 @@ SIGNATURES = {
     "bulkx_batch_launch_per_call": (_I, [_P]),
     "bulkx_batch_status": (_I, [_P]),
     "bulkx_batch_reserve": (_I, [_P, _L]),
+    "bulkx_batch_scratch_bytes": (_I, [_P, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
     "bulkx_batch_last_timing": (_I, [_P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
     "bulkx_batch_stage_timing": (_I, [_P, ctypes.POINTER(ctypes.c_float), _I]),
     "bulkx_batch_sort_geometry": (_I, [ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]),

@@ -106,6 +106,13 @@ class Batch:
                 "bulkx_batch_last_timing")
         return a.value, b.value
 
+    def scratch_bytes(self) -> tuple[int, int]:
+        """(sort / reduce_by_key scratch, per-call workspace) bytes the batch holds."""
+        t, w = ctypes.c_size_t(), ctypes.c_size_t()
+        C.check(C.lib().bulkx_batch_scratch_bytes(self._h, ctypes.byref(t), ctypes.byref(w)),
+                "bulkx_batch_scratch_bytes")
+        return int(t.value), int(w.value)
+
     def family_shapes(self) -> dict:
         """{family stage name: (shape, tile elements)} of the first seven
         stages: the tile shape each op family of this batch runs (read from

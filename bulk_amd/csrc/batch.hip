@@ -1292,6 +1292,13 @@ extern "C" int bulkx_batch_sort_geometry(int64_t* run_elements, int64_t* radix_t
   return BULKX_SUCCESS;
 }
 
+extern "C" int bulkx_batch_scratch_bytes(bulkx_batch* b, size_t* tmp_bytes, size_t* workspace_bytes) {
+  if (!b) return BULKX_ERR_INVALID_ARGUMENT;
+  if (tmp_bytes) *tmp_bytes = b->tmp_bytes_cap;
+  if (workspace_bytes) *workspace_bytes = b->ws_bytes;
+  return BULKX_SUCCESS;
+}
+
 extern "C" int bulkx_batch_family_shapes(bulkx_batch* b, int* shape, int64_t* tile_elements) {
   if (!b) return BULKX_ERR_INVALID_ARGUMENT;
   for (int f = 0; f < FAM_COUNT; ++f) {

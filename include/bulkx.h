@@ -333,6 +333,12 @@ int bulkx_batch_status(bulkx_batch* batch);
  * element -- and the per-call path's workspace), so that later
  * bulkx_batch_set_jobs calls within that size allocate and free nothing. */
 int bulkx_batch_reserve(bulkx_batch* batch, int64_t elements);
+/* Bytes of the growable scratch the batch holds now: the sort / reduce_by_key
+ * space (*tmp_bytes) and the per-call path's workspace (*workspace_bytes);
+ * either pointer may be NULL.  A set_jobs call within the reserved size
+ * leaves both unchanged.  No counterpart in the reference; for callers that
+ * budget HBM. */
+int bulkx_batch_scratch_bytes(bulkx_batch* batch, size_t* tmp_bytes, size_t* workspace_bytes);
 /* Record the batch's graph ahead of time (it is recorded on first use
  * otherwise), so no later graph launch pays a capture.  No counterpart in the reference; for
  * latency-sensitive callers (the server calls it for each of its slots). */
