@@ -27,15 +27,17 @@ constexpr bool fam_is_rbk(int fam) {
 
 // Threads per workgroup of family `fam` in shape `sh`:
 //   scan / reduce     shape 0: 512 x 32 (16384 elements), 1: 512 x 16
-//   reduce_by_key     shape 0: 512 x 16 (8192 elements),  1: 512 x 8
+//   reduce_by_key     shape 0: 512 x 16 (8192 elements),  1: 256 x 32 (8192)
 //   merge             shape 0: 512 x 8 (4096 outputs),    1: 256 x 16
 //   sort (one tile)   512 x 16 in both
 constexpr int fam_block(int fam, int sh) {
-  return fam == FAM_SORT ? kBatchSortBlock : (fam == FAM_MERGE && sh == 1) ? 256 : kBatchBlock;
+  return fam == FAM_SORT                         ? kBatchSortBlock
+         : (fam == FAM_MERGE || fam_is_rbk(fam)) && sh == 1 ? 256
+                                                            : kBatchBlock;
 }
 constexpr int fam_items(int fam, int sh) {
   return (fam == FAM_SCAN_I32 || fam == FAM_SCAN_F32) ? (sh == 0 ? 32 : 16)
-         : fam_is_rbk(fam)                           ? (sh == 0 ? 16 : 8)
+         : fam_is_rbk(fam)                           ? (sh == 0 ? 16 : 32)
          : fam == FAM_MERGE                          ? (sh == 0 ? 8 : 16)
                                                      : kBatchSortItems;
 }

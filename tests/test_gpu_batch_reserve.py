@@ -4,8 +4,8 @@ batch's scratch.  The job lists are the worst cases of the per-job terms:
 many reduce_by_key jobs of int64 values one element past a tile (two tiles,
 2 * (T + 1) run slots each), many sort jobs one key past the block-sorted
 size (the radix passes' ping-pong space and a partial tile's counts), both
-mixed, one job of all E elements of each kind, and the 4096-element
-reduce_by_key tile shape.  Nothing is launched: set_jobs only lays out and
+mixed, one job of all E elements of each kind, and the reduce_by_key
+families' second tile shape.  Nothing is launched: set_jobs only lays out and
 uploads the job table, so the jobs may share one buffer."""
 import pytest
 import torch
@@ -61,10 +61,9 @@ def test_reserve_covers_job_lists(bx, case):
     _check_no_growth(B, jobs)
 
 
-def test_reserve_covers_small_rbk_tiles(bx):
-    """The 4096-element reduce_by_key tile shape (set through the tuning
-    table before the batch is created): more tiles, so more slots per
-    element."""
+def test_reserve_covers_other_rbk_shape(bx):
+    """The reduce_by_key families' second tile shape (set through the tuning
+    table before the batch is created), jobs one element past its tile."""
     from bulk_amd import batch as B
     lib = C.lib()
     buf32 = torch.zeros(E, dtype=torch.int32, device="cuda")
@@ -73,10 +72,11 @@ def test_reserve_covers_small_rbk_tiles(bx):
     try:
         C.check(lib.bulkx_set_tuning(C.ALGO_BATCH_REDUCE_BY_KEY, C.INT64, 0, 1), "bulkx_set_tuning")
         b = B.Batch(max_jobs=MAX_JOBS, max_elements=E)
-        assert b.family_shapes()["reduce_by_key_i64"] == (1, 4096)
+        shape, tile = b.family_shapes()["reduce_by_key_i64"]
+        assert shape == 1
         b.close()
-        jobs = _rbk_jobs(B, buf32, buf64, cnt, 4097, min(MAX_JOBS, E // 4097))
-        print(f"reserve case=rbk_i64_tile4096 jobs={len(jobs)}", flush=True)
+        jobs = _rbk_jobs(B, buf32, buf64, cnt, tile + 1, min(MAX_JOBS, E // (tile + 1)))
+        print(f"reserve case=rbk_i64_shape1 tile={tile} jobs={len(jobs)}", flush=True)
         _check_no_growth(B, jobs)
     finally:
         C.check(lib.bulkx_load_tuning(None, None), "bulkx_load_tuning")
