@@ -245,3 +245,61 @@ def test_batch_reduce_by_key_long_runs(bx, orc, mode):
         bad = [k for k, c in enumerate(checks) if not c()]
         assert bad == [], f"replay {rep}: jobs {bad} differ"
     b.close()
+
+
+@pytest.mark.parametrize("mode", ["graph", "eager"])
+def test_batch_reduce_by_key_tile_edges(bx, orc, mode):
+    """Run layouts at the edges of the compact form's per-tile slots: every
+    element its own run over many tiles (each tile fills all its slots),
+    runs exactly one tile long (every tile starts with a head and ends one
+    run), half-tile runs with a one-element last tile, runs one short of a
+    tile (the boundary moves through the tiles), a job of exactly one tile
+    and one of a tile plus one element; int32, int64 and float32 values."""
+    from bulk_amd import batch as B
+    probe = B.Batch(max_jobs=1, max_elements=1)
+    T = probe.family_shapes()["reduce_by_key_i32"][1]
+    probe.close()
+    layouts = [(3 * T + 5, 1), (4 * T, T), (4 * T + 1, T // 2), (5 * T + 17, T - 1), (T, 7), (T + 1, 2)]
+    jobs, checks = [], []
+    for i, (n, run) in enumerate(layouts):
+        keys = (np.arange(n) // run).astype(np.int32)
+        for vk in ("i32", "i64", "f32"):
+            if vk == "i32":
+                vals = orc.gen_uniform_i32(70 + i, -50, 50, n)
+            elif vk == "i64":
+                vals = orc.gen_uniform_i64(80 + i, -(1 << 40), 1 << 40, n)
+            else:
+                vals = orc.gen_uniform_f32(90 + i, n, -1.0, 1.0)
+            dk, dv = _dev(keys), _dev(vals)
+            ko = torch.empty(n, dtype=torch.int32, device="cuda")
+            vo = torch.empty(n, dtype=dv.dtype, device="cuda")
+            cnt = torch.full((1,), -1, dtype=torch.int64, device="cuda")
+            jobs.append(B.reduce_by_key_job(dk, dv, ko, vo, cnt))
+            if vk == "f32":
+                v64 = vals.astype(np.float64)
+                starts = np.flatnonzero(np.r_[True, keys[1:] != keys[:-1]])
+                exact = np.add.reduceat(v64, starts)
+                lens = np.diff(np.r_[starts, n])
+                bound = 4 * np.maximum(1, np.ceil(np.log2(np.maximum(lens, 2)))) * 2.0 ** -24 * \
+                    np.add.reduceat(np.abs(v64), starts)
+                rk = keys[starts + lens - 1]
+                checks.append((f"{n}/{run}/{vk}", lambda ko=ko, vo=vo, cnt=cnt, rk=rk, ex=exact, bd=bound:
+                               int(cnt.item()) == rk.size
+                               and np.array_equal(ko.cpu().numpy()[:rk.size], rk)
+                               and bool(np.all(np.abs(vo.cpu().numpy()[:rk.size].astype(np.float64) - ex)
+                                               <= bd))))
+            else:
+                rk, rv = orc.reduce_by_key(keys, vals, textbook=True)
+                checks.append((f"{n}/{run}/{vk}", lambda ko=ko, vo=vo, cnt=cnt, rk=rk, rv=rv:
+                               int(cnt.item()) == rk.size
+                               and np.array_equal(ko.cpu().numpy()[:rk.size], rk)
+                               and np.array_equal(vo.cpu().numpy()[:rk.size], rv)))
+    b = B.Batch(max_jobs=len(jobs), max_elements=sum(j.elements for j in jobs))
+    b.set_jobs(jobs)
+    print(f"batch reduce_by_key tile edges mode={mode} tile={T} jobs={len(jobs)}", flush=True)
+    b.launch(1 if mode == "graph" else 0)
+    torch.cuda.synchronize()
+    assert b.status() == 0
+    bad = [name for name, ok in checks if not ok()]
+    b.close()
+    assert bad == [], bad
