@@ -1,12 +1,14 @@
 """Condense rocprofv3 CSV output into the short text summaries kept with the
 sources.
 
-  python -m bulk_amd.rocprof_summary stats <dir> [--command "..."]
+  python -m bulk_amd.rocprof_summary stats <dir> [--command "..."] [--last KERNEL:N]
       per-kernel calls / average / total from *_kernel_stats.csv, and for
       the library's kernels the average over the dispatches that ran alone
       (no other dispatch overlapping them in time, from *_kernel_trace.csv):
       graph replays run independent branches concurrently, eager launches
-      (the benchmark's per-stage HIP-event timing) one kernel at a time
+      (the benchmark's per-stage HIP-event timing) one kernel at a time.
+      --last adds the average over the last N dispatches of KERNEL (the
+      benchmark's eager per-stage launches come after its graph replays)
 
   python -m bulk_amd.rocprof_summary counters <dir> [--command "..."]
       per library kernel: dispatches, then each counter of a --pmc run
@@ -54,7 +56,7 @@ def _find(d: str, suffix: str) -> str:
     return hits[0]
 
 
-def stats(d: str, command: str | None, top: int) -> str:
+def stats(d: str, command: str | None, top: int, last: str | None = None) -> str:
     rows = list(csv.DictReader(open(_find(d, "kernel_stats.csv"))))
     rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
     out = []
@@ -107,6 +109,15 @@ def stats(d: str, command: str | None, top: int) -> str:
                 out.append(f"| {g} | {len(v)} | {means[-1] / 1e3:.1f} |")
             out.append("")
             out.append(f"mean over the {len(means)} grid sizes: {sum(means) / len(means) / 1e3:.1f} us")
+        if last:
+            name, count = last.rsplit(":", 1)
+            count = int(count)
+            mine = [e - s for s, e, n, _ in disp if _short(n) == name]
+            if mine:
+                tail = mine[-count:]
+                out.append("")
+                out.append(f"{name}, the last {len(tail)} of its {len(mine)} dispatches "
+                           f"(the eager per-stage launches): avg {sum(tail) / len(tail) / 1e3:.1f} us")
     return "\n".join(out) + "\n"
 
 
@@ -232,6 +243,7 @@ def main(argv=None):
     s.add_argument("dir")
     s.add_argument("--command")
     s.add_argument("--top", type=int, default=30)
+    s.add_argument("--last", default=None, help="KERNEL:N -- the average of its last N dispatches")
     c = sub.add_parser("counters")
     c.add_argument("dir")
     c.add_argument("--command")
@@ -247,7 +259,7 @@ def main(argv=None):
     t.add_argument("--command")
     a = p.parse_args(argv)
     if a.mode == "stats":
-        print(stats(a.dir, a.command, a.top), end="")
+        print(stats(a.dir, a.command, a.top, a.last), end="")
     elif a.mode == "counters":
         print(counters(a.dir, a.command), end="")
     elif a.mode == "requests":
