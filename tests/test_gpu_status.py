@@ -57,7 +57,8 @@ def test_scan_give_up_raises_through_check(give_up):
 
 def test_single_pass_reduce_by_key_give_up(give_up):
     """The single-pass (look-back) reduce_by_key, forced by a tuning
-    override, reports the give-up; the three-pass form has no look-back."""
+    override, reports the give-up; the three-pass and compact forms have no
+    look-back."""
     from bulk_amd import api
     n = 1 << 18  # size class 1
     lib = C.lib()
